@@ -1,0 +1,186 @@
+/*
+ * libjues_hip -- C ABI for JuES's correlation hot path on MI355X (gfx950).
+ *
+ * Conventions
+ *   - Every array is COLUMN-MAJOR (Fortran order) with 0-based indices, i.e.
+ *     exactly the memory a Julia Array{Float64,N} hands to ccall.
+ *     A Julia T2[i,j,a,b] (o x o x v x v) is element i + o*(j + o*(a + v*b)).
+ *   - Host pointers are only read/written during the call; the library keeps no
+ *     host pointer after it returns.  Device state lives in a jues_ctx.
+ *   - Calls are synchronous with respect to the host.  A jues_ctx is not
+ *     thread-safe: use one per host thread.
+ *   - Every call returns a jues_status; on error jues_last_error() describes it.
+ *
+ * Reference interfaces (paths relative to the JuES.jl tree) are cited on each
+ * entry point.  The reference has no FFI of its own; these entry points replace
+ * the bodies of the Julia functions named there (see the ccall stubs in
+ * jues.jl_amd/julia/JuESHip.jl).
+ */
+#ifndef JUES_H
+#define JUES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jues_ctx jues_ctx;
+
+typedef enum {
+    JUES_OK = 0,
+    JUES_NOT_CONVERGED = 1,  /* iteration limit reached; results hold the last iterate */
+    JUES_NONFINITE = 2,      /* a NaN/Inf appeared (e.g. a zero orbital-energy denominator) */
+    JUES_OOM = 3,            /* device allocation failed */
+    JUES_BAD_ARG = 4,        /* invalid sizes, strings, null pointers, missing integrals */
+    JUES_HIP_ERROR = 5       /* a HIP runtime call failed */
+} jues_status;
+
+typedef enum {
+    JUES_FP64 = 0,        /* plain form: every contraction in FP64 MFMA */
+    JUES_FP32_MIXED = 1   /* contractions in FP32 MFMA (FP32 accumulate); energies,
+                             denominators, DIIS and the final refinement in FP64 */
+} jues_prec;
+
+/* Which reference RCCSD equations to follow.  They differ only in the T1 exchange
+ * term of Fae: mRCCSD.jl:140 contracts t1[m,f] with <me|af>, RCCSD.jl:189 with
+ * <ma|ef>.  Every other term is identical. */
+typedef enum {
+    JUES_CCSD_MRCCSD = 0,  /* CoupledCluster/mRCCSD.jl (the module the reference test uses) */
+    JUES_CCSD_RCCSD = 1    /* CoupledCluster/RCCSD.jl */
+} jues_ccsd_eqs;
+
+/* Initial T2 guess. */
+typedef enum {
+    JUES_GUESS_ZERO = 0,   /* t = 0, as mRCCD.jl:245 */
+    JUES_GUESS_MP2 = 1,    /* t = <ij|ab>/D, as mRCCSD.jl:66 */
+    JUES_GUESS_OVOV = 2,   /* t = (ij|ab)/D, as RCCD.jl:145-160 */
+    JUES_GUESS_USER = 3    /* t = T2_inout on entry (warm start) */
+} jues_guess;
+
+/* Options; jues_cc_default_opts() fills the defaults noted here.
+ * Names mirror CoupledCluster.jl:36-43 (cc_max_iter, cc_e_conv, cc_max_rms, diis). */
+typedef struct {
+    int32_t maxit;            /* cc_max_iter, default 100 */
+    double e_conv;            /* |E_k - E_{k-1}| threshold, default 1e-10 */
+    double r_conv;            /* ||t_new - t_old||_2 threshold, default 1e-9 */
+    int32_t diis;             /* 1 = DIIS (mRCCD.jl:171-203), 0 = plain Jacobi (RCCD.jl:55-74) */
+    int32_t diis_max;         /* (t, error) pairs kept; mRCCD.jl's max_diis=6 keeps 5. default 5 */
+    int32_t guess;            /* jues_guess, default JUES_GUESS_MP2 */
+    int32_t ccsd_eqs;         /* jues_ccsd_eqs, default JUES_CCSD_MRCCSD */
+    int32_t fcn;              /* frozen core: drop the first fcn occupied orbitals (get_eri's fcn) */
+    /* FP32_MIXED only: the FP32 stage stops once ||dt||_2 < fp32_r_conv (or it stalls);
+     * FP64 refinement iterations then run to e_conv / r_conv, at most refine_maxit. */
+    double fp32_r_conv;       /* default 1e-5 */
+    int32_t refine_maxit;     /* default 60 */
+    int32_t verbose;          /* print one line per iteration to stderr */
+} jues_cc_opts;
+
+/* Per-iteration record passed to the optional callback. */
+typedef struct {
+    int32_t iter;             /* 1-based */
+    int32_t stage;            /* 0 = FP64, 1 = FP32 (mixed form) */
+    double energy;            /* correlation energy of the Jacobi update t_new */
+    double delta_e;           /* energy - previous energy */
+    double rnorm;             /* ||t_new - t_old||_2 (T1 and T2 together for CCSD) */
+    int32_t diis_size;        /* number of DIIS pairs used this iteration */
+    double ms;                /* device time of the iteration (HIP events on the ctx stream) */
+} jues_iter_info;
+
+typedef void (*jues_iter_cb)(void* user, const jues_iter_info* info);
+
+/* ---- context -------------------------------------------------------------- */
+jues_status jues_create(jues_ctx** ctx, int device);
+void jues_destroy(jues_ctx* ctx);
+const char* jues_last_error(const jues_ctx* ctx);
+void jues_cc_default_opts(jues_cc_opts* opts);
+/* Bytes of device memory currently held by the context (integrals, scratch). */
+size_t jues_device_bytes(const jues_ctx* ctx);
+/* Release all device buffers held by the context (keeps the stream). */
+jues_status jues_release(jues_ctx* ctx);
+
+/* ---- AO integrals (the Wfn.ao_eri carrier, Backend/Wavefunction.jl:87) ------
+ * Exactly one representation is current; setting one drops the other.
+ * dense: g[mu,nu,lam,sig], n^4, chemist order (mu fastest), copied to the device.
+ * factors: B[mu,nu,P], n x n x naux, with (mu nu|lam sig) = sum_P B[mu,nu,P] B[lam,sig,P];
+ *          the dense tensor is never formed (configs whose n^4 does not fit). */
+jues_status jues_set_ao_eri(jues_ctx* ctx, const double* gao, int n);
+jues_status jues_set_ao_factors(jues_ctx* ctx, const double* B, int n, int naux);
+
+/* ---- Transformation.tei_transform (Backend/Transformation.jl:39-93) ----------
+ * out[p,q,r,s] = sum C1[mu,p] C2[nu,q] C3[lam,r] C4[sig,s] g[mu,nu,lam,sig]
+ * (chemist order, out is d1 x d2 x d3 x d4).  C_k is n x d_k.  Uses the dense AO
+ * tensor if set, else the factors.  FP32_MIXED runs the quarter GEMMs in FP32. */
+jues_status jues_tei_transform(jues_ctx* ctx, int prec,
+                               const double* C1, int d1, const double* C2, int d2,
+                               const double* C3, int d3, const double* C4, int d4,
+                               int n, double* out);
+
+/* ---- IntegralTransformation.get_eri (Backend/IntegralTransformation.jl:38-101) --
+ * eri_string: 4 chars from {o,O,v,V}; notation: "phys" (default when NULL) or "chem".
+ * Cocc (n x o), Cvir (n x v); fcn drops the first fcn occupied columns.  out has the
+ * dimensions the string names, in the requested notation. */
+jues_status jues_get_eri(jues_ctx* ctx, int prec, const char* eri_string, const char* notation,
+                         const double* Cocc, int o, const double* Cvir, int v, int n, int fcn,
+                         double* out);
+
+/* ---- MollerPlesset.do_rmp2 (MollerPlesset/RMP2.jl:11-45) ----------------------
+ * E = sum_{ijab} <ij|ab>(2<ij|ab> - <ij|ba>) / (e_i + e_j - e_a - e_b). eps has n = o+v entries. */
+jues_status jues_rmp2(jues_ctx* ctx, int prec, const double* Cocc, int o, const double* Cvir,
+                      int v, int n, const double* eps, double* E);
+
+/* MP2 adjoint (no reference counterpart; the reverse rule of jues_rmp2 for a
+ * ChainRulesCore rrule).  C is the full n x n matrix [Cocc Cvir] (o occupied
+ * columns first); eps is held fixed.  dE_dC (n x n) and dE_dg (n^4, only with a
+ * dense AO tensor) may be NULL.  Every entry of C and g is an independent variable. */
+jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, int o, int n,
+                           const double* eps, double* E, double* dE_dC, double* dE_dg);
+
+/* ---- CoupledCluster.{RCCD,mRCCD}.do_rccd (RCCD.jl:33-83, mRCCD.jl:38-118) ------
+ * Converges the closed-shell CCD equations of RCCD.jl:134-303.  T2_inout (o x o x v x v,
+ * Julia order) may be NULL; when given it receives the final amplitudes, and with
+ * guess == JUES_GUESS_USER it also supplies the starting amplitudes. */
+jues_status jues_rccd(jues_ctx* ctx, int prec, const jues_cc_opts* opts,
+                      const double* Cocc, int o, const double* Cvir, int v, int n,
+                      const double* eps, double* E, double* T2_inout,
+                      jues_iter_cb cb, void* user);
+
+/* ---- CoupledCluster.{RCCSD,mRCCSD}.do_rccsd (RCCSD.jl:33-101, mRCCSD.jl:35-101) --
+ * T1_inout is o x v.  opts->ccsd_eqs selects the reference module's equations. */
+jues_status jues_rccsd(jues_ctx* ctx, int prec, const jues_cc_opts* opts,
+                       const double* Cocc, int o, const double* Cvir, int v, int n,
+                       const double* eps, double* E, double* T1_inout, double* T2_inout,
+                       jues_iter_cb cb, void* user);
+
+/* ---- stepwise solver (the same loop as jues_rccd / jues_rccsd, split so that a
+ * caller can time iterations with every input already resident on the device) ----
+ * method: 0 = RCCD, 1 = RCCSD.  setup builds the MO integral blocks on the device and
+ * the starting amplitudes; iterate runs up to nsteps iterations in the given stage
+ * precision (JUES_FP64 or JUES_FP32_MIXED), stopping early only if stop_on_conv is set
+ * and the convergence test passes.  info receives the last iteration's record. */
+jues_status jues_cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts,
+                          const double* Cocc, int o, const double* Cvir, int v, int n,
+                          const double* eps, const double* T1_init, const double* T2_init);
+jues_status jues_cc_iterate(jues_ctx* ctx, int stage_prec, int nsteps, int stop_on_conv,
+                            jues_iter_info* info, int* converged);
+/* Reset the DIIS history (e.g. at the FP32 -> FP64 hand-over). */
+jues_status jues_cc_reset_diis(jues_ctx* ctx);
+jues_status jues_cc_get(jues_ctx* ctx, double* E, double* T1_out, double* T2_out);
+
+/* Accumulated device time per kernel class since the last reset (HIP events on
+ * the ctx stream).  names/ms/flops/bytes arrays of length cap; returns the count. */
+int jues_kernel_stats(jues_ctx* ctx, const char** names, double* ms, double* calls,
+                      double* flops, double* bytes, int cap);
+void jues_kernel_stats_reset(jues_ctx* ctx, int enable);
+
+/* Raw MFMA GEMM (column-major): C = alpha*op(A)*op(B) + beta*C, on device copies of
+ * host arrays.  Exposed for the GEMM unit tests; prec selects FP64 or FP32 MFMA. */
+jues_status jues_gemm_test(jues_ctx* ctx, int prec, char transa, char transb, int M, int N,
+                           int K, double alpha, const double* A, int lda, const double* B,
+                           int ldb, double beta, double* C, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JUES_H */

@@ -1,0 +1,167 @@
+// Internal declarations shared by the libjues_hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/jues.h"
+
+namespace jues {
+
+struct Error : std::exception {
+  jues_status st;
+  std::string msg;
+  Error(jues_status s, std::string m) : st(s), msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+
+#define JUES_HIP(x)                                                                    \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess)                                                              \
+      throw ::jues::Error(JUES_HIP_ERROR, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define JUES_REQUIRE(cond, msg)                                          \
+  do {                                                                   \
+    if (!(cond)) throw ::jues::Error(JUES_BAD_ARG, std::string(msg));    \
+  } while (0)
+
+// Owning device allocation.  Allocation failure raises JUES_OOM.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t nbytes) { alloc(nbytes); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t nbytes);
+  void release();
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+size_t device_bytes_in_use();
+
+struct KStat {
+  double ms = 0, calls = 0, flops = 0, bytes = 0;
+};
+
+struct CCState;  // cc.hip
+struct CCDeleter {
+  void operator()(CCState* p) const;  // defined in cc.hip
+};
+
+}  // namespace jues
+
+struct jues_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // AO integrals: exactly one of gao (n^4) / Bfac (n*n*naux) is set.
+  int n = 0, naux = 0;
+  jues::DevBuf gao;
+  jues::DevBuf Bfac;
+  jues::DevBuf gao32, Bfac32;  // FP32 copies, made on first FP32 use
+  // kernel timing (HIP events on `stream`)
+  bool timing = false;
+  std::map<std::string, jues::KStat> kstats;
+  struct Pending { std::string name; hipEvent_t a, b; double flops, bytes; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::unique_ptr<jues::CCState, jues::CCDeleter> cc;
+  jues_ctx();
+  ~jues_ctx();
+};
+
+namespace jues {
+
+// Scoped kernel timer: records events around the enclosed launches when timing is on.
+struct KTimer {
+  jues_ctx* ctx;
+  bool on;
+  size_t slot = 0;
+  KTimer(jues_ctx* c, const char* name, double flops, double bytes);
+  ~KTimer();
+};
+void flush_timers(jues_ctx* ctx);  // synchronises and folds pending events into kstats
+
+// ---- GEMM (gemm.hip) -------------------------------------------------------
+// Column-major C[M,N] = alpha*op(A)*op(B) + beta*C, batched with element strides.
+template <typename T>
+void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
+          long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch = 1,
+          const char* tag = nullptr);
+
+// ---- elementwise / data movement (kernels.hip) -------------------------------
+// out = alpha * permutedims(in, perm) + beta * out   (perm 0-based; out dim k = in dim perm[k])
+template <typename Ti, typename To>
+void permute(jues_ctx* ctx, const Ti* in, const std::vector<long>& in_dims,
+             const std::vector<int>& perm, To* out, double alpha, double beta,
+             const char* tag = nullptr);
+template <typename Ti, typename To>
+void convert(jues_ctx* ctx, const Ti* in, To* out, long n);
+template <typename T>
+void fill(jues_ctx* ctx, T* out, long n, double value);
+// y = a*x + b*y
+template <typename T>
+void axpby(jues_ctx* ctx, long n, double a, const T* x, double b, T* y);
+// deterministic FP64 dot of two vectors (any element types)
+template <typename Ta, typename Tb>
+double dot(jues_ctx* ctx, const Ta* x, const Tb* y, long n);
+double max_abs(jues_ctx* ctx, const double* x, long n);
+
+// ---- contraction engine (einsum.cpp) -----------------------------------------
+template <typename T>
+struct Tens {
+  T* p = nullptr;
+  std::string idx;         // one letter per dimension, first letter fastest
+  std::vector<long> dims;
+  long size() const { long s = 1; for (long d : dims) s *= d; return s; }
+};
+// C[idxC] = alpha * sum A[idxA] B[idxB] + beta * C  (any index order; TTGT on the device)
+template <typename T>
+void contract(jues_ctx* ctx, double alpha, const Tens<T>& A, const Tens<T>& B, double beta,
+              const Tens<T>& C, const char* tag = nullptr);
+// C[idxC] = alpha * P(A) + beta * C
+template <typename Ti, typename To>
+void add_perm(jues_ctx* ctx, double alpha, const Tens<Ti>& A, double beta, const Tens<To>& C,
+              const char* tag = nullptr);
+
+// scratch pool owned by the context for temporaries of the contraction engine
+void* scratch(jues_ctx* ctx, int slot, size_t bytes);
+
+void release_scratch(jues_ctx* ctx);
+
+// ---- integral transformation (transform.hip) -------------------------------------
+// AO integrals in precision T (FP32 copies are made on first use and cached).
+template <typename T> const T* ao_dense(jues_ctx* ctx);
+template <typename T> const T* ao_factors(jues_ctx* ctx);
+// out[p,q,r,s] (chemist, d1 x d2 x d3 x d4) from device C_k (n x d_k)
+template <typename T>
+void tei_transform_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, const T* C3, int d3,
+                       const T* C4, int d4, T* out);
+// MO factors L[p,q,P] = (C1^T B_P C2)[p,q]  (needs factors)
+template <typename T>
+void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* L);
+// RMP2 energy from K[i,a,j,b] = (ia|jb) (device, o x v x o x v) and eps (device FP64, n)
+template <typename T>
+double mp2_energy_dev(jues_ctx* ctx, const T* K, const double* eps, int o, int v);
+
+// ---- host <-> device helpers ---------------------------------------------------
+void h2d(jues_ctx* ctx, void* dst, const void* src, size_t bytes);
+void d2h(jues_ctx* ctx, void* dst, const void* src, size_t bytes);
+void sync(jues_ctx* ctx);
+
+}  // namespace jues

@@ -1,0 +1,171 @@
+// AO -> MO four-index transformation (Backend/Transformation.jl:39-93) and the
+// RMP2 energy reduction (MollerPlesset/RMP2.jl:27-41).
+//
+// Dense route: four quarter transforms, each one MFMA GEMM over the full AO
+// tensor.  The outer index (mu or sigma) with the smaller target dimension is
+// contracted first, then the other outer index, then one tiled permutation
+// brings nu and lambda to the outside for the last two quarters.  Factor route:
+// (pq|rs) = sum_P (C1^T B_P C2)_pq (C3^T B_P C4)_rs, two batched GEMMs per side
+// and one GEMM over P; the n^4 AO tensor is never formed.
+#include "jues_internal.h"
+
+#include <algorithm>
+
+namespace jues {
+
+template <>
+const double* ao_dense<double>(jues_ctx* ctx) {
+  JUES_REQUIRE(ctx->gao.p, "no dense AO integrals set");
+  return ctx->gao.as<double>();
+}
+template <>
+const float* ao_dense<float>(jues_ctx* ctx) {
+  JUES_REQUIRE(ctx->gao.p, "no dense AO integrals set");
+  if (!ctx->gao32.p) {
+    const long nn = (long)ctx->n * ctx->n * ctx->n * ctx->n;
+    ctx->gao32.alloc(sizeof(float) * nn);
+    convert<double, float>(ctx, ctx->gao.as<double>(), ctx->gao32.as<float>(), nn);
+  }
+  return ctx->gao32.as<float>();
+}
+template <>
+const double* ao_factors<double>(jues_ctx* ctx) {
+  JUES_REQUIRE(ctx->Bfac.p, "no AO factors set");
+  return ctx->Bfac.as<double>();
+}
+template <>
+const float* ao_factors<float>(jues_ctx* ctx) {
+  JUES_REQUIRE(ctx->Bfac.p, "no AO factors set");
+  if (!ctx->Bfac32.p) {
+    const long nn = (long)ctx->n * ctx->n * ctx->naux;
+    ctx->Bfac32.alloc(sizeof(float) * nn);
+    convert<double, float>(ctx, ctx->Bfac.as<double>(), ctx->Bfac32.as<float>(), nn);
+  }
+  return ctx->Bfac32.as<float>();
+}
+
+template <typename T>
+void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* L) {
+  const long n = ctx->n, naux = ctx->naux;
+  const T* B = ao_factors<T>(ctx);
+  T* Y = static_cast<T*>(scratch(ctx, 4, sizeof(T) * (size_t)(n * d2 * naux)));
+  // Y[mu, q, P] = B_P C2 ; L[p, q, P] = C1^T Y_P
+  gemm<T>(ctx, false, false, n, d2, n, T(1), B, n, n * n, C2, n, 0, T(0), Y, n, n * d2, (int)naux, "xform_factor");
+  gemm<T>(ctx, true, false, d1, d2, n, T(1), C1, n, 0, Y, n, n * d2, T(0), L, d1, (long)d1 * d2, (int)naux,
+          "xform_factor");
+}
+
+template <typename T>
+void tei_transform_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, const T* C3, int d3,
+                       const T* C4, int d4, T* out) {
+  const long n = ctx->n;
+  JUES_REQUIRE(n > 0, "no AO integrals set");
+  if (ctx->gao.p) {
+    const T* G = ao_dense<T>(ctx);
+    const long n2 = n * n, n3 = n2 * n;
+    T* X2 = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(d1 * n2 * d4)));
+    if (d1 <= d4) {
+      T* X1 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)(d1 * n3)));
+      gemm<T>(ctx, true, false, d1, n3, n, T(1), C1, n, 0, G, n, 0, T(0), X1, d1, 0, 1, "xform_q1");
+      gemm<T>(ctx, false, false, d1 * n2, d4, n, T(1), X1, d1 * n2, 0, C4, n, 0, T(0), X2, d1 * n2, 0, 1,
+              "xform_q2");
+    } else {
+      T* Y1 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)(n3 * d4)));
+      gemm<T>(ctx, false, false, n3, d4, n, T(1), G, n3, 0, C4, n, 0, T(0), Y1, n3, 0, 1, "xform_q1");
+      gemm<T>(ctx, true, false, d1, n2 * d4, n, T(1), C1, n, 0, Y1, n, 0, T(0), X2, d1, 0, 1, "xform_q2");
+    }
+    // X2[p, nu, lam, s] -> X3[nu, p, s, lam]
+    T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)(d1 * n2 * d4)));
+    permute<T, T>(ctx, X2, {d1, n, n, d4}, {1, 0, 3, 2}, X3, 1.0, 0.0, "xform_perm");
+    T* X5 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)((long)d1 * d2 * d3 * d4)));
+    const long pd = (long)d1 * d4;
+    if (d2 <= d3) {
+      T* X4 = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(d2 * pd * n)));
+      gemm<T>(ctx, true, false, d2, pd * n, n, T(1), C2, n, 0, X3, n, 0, T(0), X4, d2, 0, 1, "xform_q3");
+      gemm<T>(ctx, false, false, d2 * pd, d3, n, T(1), X4, d2 * pd, 0, C3, n, 0, T(0), X5, d2 * pd, 0, 1,
+              "xform_q4");
+    } else {
+      T* Y = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(n * pd * d3)));
+      gemm<T>(ctx, false, false, n * pd, d3, n, T(1), X3, n * pd, 0, C3, n, 0, T(0), Y, n * pd, 0, 1, "xform_q3");
+      gemm<T>(ctx, true, false, d2, pd * d3, n, T(1), C2, n, 0, Y, n, 0, T(0), X5, d2, 0, 1, "xform_q4");
+    }
+    // X5[q, p, s, r] -> out[p, q, r, s]
+    permute<T, T>(ctx, X5, {d2, d1, d4, d3}, {1, 0, 3, 2}, out, 1.0, 0.0, "xform_perm");
+  } else {
+    const long naux = ctx->naux;
+    JUES_REQUIRE(ctx->Bfac.p, "no AO integrals set");
+    T* L = static_cast<T*>(scratch(ctx, 5, sizeof(T) * (size_t)((long)d1 * d2 * naux)));
+    T* R = static_cast<T*>(scratch(ctx, 6, sizeof(T) * (size_t)((long)d3 * d4 * naux)));
+    mo_factors_dev<T>(ctx, C1, d1, C2, d2, L);
+    mo_factors_dev<T>(ctx, C3, d3, C4, d4, R);
+    const long m = (long)d1 * d2, nn = (long)d3 * d4;
+    gemm<T>(ctx, false, true, m, nn, naux, T(1), L, m, 0, R, nn, 0, T(0), out, m, 0, 1, "xform_factor_gemm");
+  }
+}
+
+namespace {
+
+// One block per (j,b) column of K[i,a,j,b]; each thread walks (i,a) with stride 256.
+template <typename T>
+__global__ __launch_bounds__(256) void mp2_energy_kernel(const T* __restrict__ K, const double* __restrict__ eps,
+                                                         int o, int v, double* __restrict__ part) {
+  __shared__ double sh[4];
+  const int ov = o * v;
+  const int jb = blockIdx.x;
+  const int j = jb % o, b = jb / o;
+  const double ejb = eps[j] - eps[o + b];
+  const long col = (long)jb * ov;
+  double s = 0.0;
+  for (int ia = threadIdx.x; ia < ov; ia += 256) {
+    const int i = ia % o, a = ia / o;
+    const double k = (double)K[col + ia];
+    const double kx = (double)K[(long)(i + o * b) + (long)ov * (j + o * a)];  // (ib|ja)
+    s += k * (2.0 * k - kx) / (eps[i] - eps[o + a] + ejb);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[jb] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void sum_fixed_kernel(const double* __restrict__ part, int count,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += 256) s += part[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+}  // namespace
+
+template <typename T>
+double mp2_energy_dev(jues_ctx* ctx, const T* K, const double* eps, int o, int v) {
+  const int ov = o * v;
+  double* part = static_cast<double*>(scratch(ctx, 14, sizeof(double) * (size_t)(ov + 1)));
+  {
+    KTimer t(ctx, "mp2_energy", 6.0 * ov * ov, (double)ov * ov * sizeof(T) * 2);
+    hipLaunchKernelGGL((mp2_energy_kernel<T>), dim3(ov), dim3(256), 0, ctx->stream, K, eps, o, v, part);
+    hipLaunchKernelGGL(sum_fixed_kernel, dim3(1), dim3(256), 0, ctx->stream, part, ov, part + ov);
+    JUES_HIP(hipGetLastError());
+  }
+  double E = 0.0;
+  d2h(ctx, &E, part + ov, sizeof(double));
+  return E;
+}
+
+template void mo_factors_dev<double>(jues_ctx*, const double*, int, const double*, int, double*);
+template void mo_factors_dev<float>(jues_ctx*, const float*, int, const float*, int, float*);
+template void tei_transform_dev<double>(jues_ctx*, const double*, int, const double*, int, const double*, int,
+                                        const double*, int, double*);
+template void tei_transform_dev<float>(jues_ctx*, const float*, int, const float*, int, const float*, int,
+                                       const float*, int, float*);
+template double mp2_energy_dev<double>(jues_ctx*, const double*, const double*, int, int);
+template double mp2_energy_dev<float>(jues_ctx*, const float*, const double*, int, int);
+
+}  // namespace jues

@@ -1,0 +1,34 @@
+"""``get_eri`` (Backend/IntegralTransformation.jl:38-101) on the GPU."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .Wavefunction import Wfn, ensure_ao, prec_code
+
+
+def get_eri(wfn: Wfn, eri_string: str, notation: str = "phys", fcn: int = 0, *, prec: str = "fp64",
+            device: int = 0) -> np.ndarray:
+    """MO integral block named by a 4-letter string over {o,O,v,V}.
+
+    ``notation="phys"`` returns <pq|rs> (the default, as the reference);
+    ``"chem"`` returns (pq|rs).  ``fcn`` drops the first fcn occupied orbitals.
+    Raises ``JuesError`` (BAD_ARG) for an invalid string, like the reference's ``error``.
+    """
+    ctx = _lib.context(device)
+    n = ensure_ao(ctx, wfn.ao_eri)
+    o, v = wfn.nalpha, wfn.nvira
+    s = eri_string if isinstance(eri_string, str) else str(eri_string)
+    if len(s.encode()) != 4 or any(c not in "oOvV" for c in s):
+        # same message the library returns; checked here to size the output
+        raise _lib.JuesError(_lib.JUES_BAD_ARG, f"Invalid string given to get_eri: {s}")
+    shape_src = s
+    dims = [(o - fcn) if c in "oO" else v for c in shape_src]
+    out = np.empty(dims, dtype=np.float64, order="F")
+    Co, Cv = _lib.f64(wfn.Cao), _lib.f64(wfn.Cav)
+    ctx.check(ctx.lib.jues_get_eri(ctx.h, prec_code(prec), s.encode(), notation.encode(), _lib.ptr(Co), o,
+                                   _lib.ptr(Cv), v, n, fcn, _lib.ptr(out)))
+    return out
+
+
+__all__ = ["get_eri"]

@@ -1,0 +1,41 @@
+"""``do_rmp2`` (MollerPlesset/RMP2.jl:11-45) and its adjoint on the GPU."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .Wavefunction import Wfn, FactoredERI, ensure_ao, prec_code
+
+
+def do_rmp2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0, **kwargs) -> float:
+    """Closed-shell MP2 correlation energy (RMP2.jl:33-37)."""
+    ctx = _lib.context(device)
+    n = ensure_ao(ctx, refWfn.ao_eri)
+    Co, Cv = _lib.f64(refWfn.Cao), _lib.f64(refWfn.Cav)
+    eps = _lib.f64(refWfn.epsa)
+    E = np.zeros(1)
+    ctx.check(ctx.lib.jues_rmp2(ctx.h, prec_code(prec), _lib.ptr(Co), refWfn.nalpha, _lib.ptr(Cv), refWfn.nvira,
+                                n, _lib.ptr(eps), _lib.ptr(E)))
+    return float(E[0])
+
+
+def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: int = 0):
+    """(E, dE/dC (n x n), dE/dg (n^4 or None)) with every C and g entry independent.
+
+    ``dE/dg`` needs dense AO integrals; with a ``FactoredERI`` it is None.
+    """
+    ctx = _lib.context(device)
+    n = ensure_ao(ctx, refWfn.ao_eri)
+    C = _lib.f64(refWfn.Ca)
+    eps = _lib.f64(refWfn.epsa)
+    E = np.zeros(1)
+    dC = np.zeros((n, n), order="F")
+    dg = None
+    if want_dg and not isinstance(refWfn.ao_eri, FactoredERI):
+        dg = np.zeros((n, n, n, n), order="F")
+    ctx.check(ctx.lib.jues_rmp2_grad(ctx.h, prec_code(prec), _lib.ptr(C), refWfn.nalpha, n, _lib.ptr(eps),
+                                     _lib.ptr(E), _lib.ptr(dC), _lib.ptr(dg)))
+    return float(E[0]), dC, dg
+
+
+__all__ = ["do_rmp2", "rmp2_grad"]

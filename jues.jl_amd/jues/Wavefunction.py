@@ -1,0 +1,93 @@
+"""The ``Wfn`` carrier (Backend/Wavefunction.jl:67-141), restricted closed shell.
+
+Field names follow the Julia struct.  ``ao_eri`` is either a dense n^4 array
+(chemist order) or a ``FactoredERI`` holding B[mu,nu,P] with
+(mu nu|lam sig) = sum_P B[mu,nu,P] B[lam,sig,P]; the second form plays the
+role the reference gives ``DiskFourTensor`` (Wavefunction.jl:87): an ERI carrier
+for systems whose dense tensor does not fit.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Any, Optional, Union
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclasses.dataclass
+class FactoredERI:
+    B: np.ndarray  # (n, n, naux)
+
+    @property
+    def n(self) -> int:
+        return self.B.shape[0]
+
+
+@dataclasses.dataclass
+class Wfn:
+    energy: float
+    vnuc: float
+    nalpha: int
+    nbeta: int
+    nvira: int
+    nvirb: int
+    nmo: int
+    unrestricted: bool
+    basis: Any
+    molecule: Any
+    Ca: np.ndarray
+    Cb: np.ndarray
+    Cao: np.ndarray
+    Cav: np.ndarray
+    Cbo: np.ndarray
+    Cbv: np.ndarray
+    hao: Optional[np.ndarray]
+    epsa: np.ndarray
+    epsb: np.ndarray
+    ao_eri: Union[np.ndarray, FactoredERI]
+
+    @classmethod
+    def from_orbitals(cls, C: np.ndarray, eps: np.ndarray, nocc: int, ao_eri, energy: float = 0.0,
+                      vnuc: float = 0.0, hao=None) -> "Wfn":
+        """Restricted Wfn from MO coefficients, orbital energies and AO integrals (Wavefunction.jl:94-141)."""
+        C = np.asfortranarray(C, dtype=np.float64)
+        n = C.shape[0]
+        nvir = n - nocc
+        Co = np.asfortranarray(C[:, :nocc])
+        Cv = np.asfortranarray(C[:, nocc:])
+        eps = np.asarray(eps, dtype=np.float64)
+        return cls(energy, vnuc, nocc, nocc, nvir, nvir, n, False, None, None, C, C, Co, Cv, Co, Cv, hao,
+                   eps, eps, ao_eri)
+
+    @classmethod
+    def from_synthetic(cls, system, dense: bool = True) -> "Wfn":
+        eri = system.ao_eri() if dense else FactoredERI(system.B)
+        return cls.from_orbitals(system.C, system.eps, system.o, eri)
+
+
+def _ao_key(eri) -> tuple:
+    if isinstance(eri, FactoredERI):
+        return ("factors", id(eri.B), eri.B.shape)
+    return ("dense", id(eri), eri.shape)
+
+
+def ensure_ao(ctx: "_lib.Context", eri) -> int:
+    """Make ``eri`` the context's current AO integrals (uploads only when it changed). Returns n."""
+    key = _ao_key(eri)
+    if ctx.ao_key != key:
+        if isinstance(eri, FactoredERI):
+            ctx.set_ao_factors(eri.B, key)
+        else:
+            ctx.set_ao_eri(eri, key)
+    return eri.n if isinstance(eri, FactoredERI) else eri.shape[0]
+
+
+def prec_code(prec: str) -> int:
+    p = prec.lower()
+    if p in ("fp64", "float64", "double"):
+        return _lib.FP64
+    if p in ("fp32_mixed", "fp32", "mixed"):
+        return _lib.FP32_MIXED
+    raise ValueError(f"unknown precision {prec!r}")

@@ -1,0 +1,620 @@
+"""CPU FP64 restatement of JuES's correlation hot path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s ``cpu_baseline`` leg may import it; the product path
+(``libjues_hip.so`` and the ``jues`` host package) never routes through it.
+
+Every function restates one reference function from its source text (paths
+relative to the JuES.jl tree), index for index, with NumPy ``einsum`` /
+``reshape(order="F")`` standing in for TensorOperations ``@tensor`` and
+``BLAS.gemm!``.  Arrays are indexed exactly like the Julia arrays (0-based):
+``oovv[i,j,a,b]`` here is ``oovv[i+1,j+1,a+1,b+1]`` there.
+
+How each part is pinned (the checks live in ``tests/test_oracle.py``):
+
+* ``tei_transform`` / ``get_eri``: brute-force 8-loop sum at n<=4, the exact
+  3-index factor route ``(pq|rs) = sum_P (C^T B_P C)_pq (C^T B_P C)_rs``,
+  identity-C and Frobenius-norm invariants, chem<->phys permutations.
+* ``rmp2``: the literal 4-loop form of RMP2.jl, rank-1 closed form,
+  homogeneity ``E(lambda B) = lambda^4 E(B)``.
+* ``rccd_*``: two independent restatements (RCCD.jl ``@tensoropt`` form and
+  mRCCD.jl GEMM form) agree to 1e-12; first Jacobi step from t=0 gives E_MP2;
+  the converged residual vanishes; fixed point independent of guess and DIIS;
+  small-coupling limit ``E_CCD(lambda g) -> lambda^2 E_MP2(g)``.
+* ``rccsd_*``: mRCCSD.jl 6-block and RCCSD.jl 15-block restatements agree;
+  with ooov = ovvv = 0 RCCSD reduces to RCCD exactly.
+* ``rmp2_grad``: central finite differences.
+* Reference known answers (water STO-3G) through ``oracle/sto3g.py``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_es = lambda spec, *ops: np.einsum(spec, *ops, optimize=True)  # noqa: E731
+
+
+def _pd(A, perm):
+    """Julia ``permutedims(A, perm)`` (1-based perm)."""
+    return np.transpose(A, [p - 1 for p in perm])
+
+
+def _rs(A, *shape):
+    """Julia column-major ``reshape``."""
+    return np.reshape(A, shape, order="F")
+
+
+def _tperm(A, lhs):
+    """Julia ``@tensor X[lhs...] := A[1,2,3,4]`` (pure relabelling)."""
+    inv = [0] * len(lhs)
+    for k, l in enumerate(lhs):
+        inv[k] = l - 1
+    return np.transpose(A, inv)
+
+
+# ---------------------------------------------------------------------------
+# Backend/Transformation.jl:39-93 and Backend/IntegralTransformation.jl:38-101
+# ---------------------------------------------------------------------------
+
+def tei_transform(gao, C1, C2, C3, C4):
+    """``out[i,a,j,b] = sum C1[mu,i] C2[nu,a] C3[lam,j] C4[sig,b] g[mu,nu,lam,sig]``.
+
+    Quarter order sigma -> lambda -> nu -> mu as in Transformation.jl:69-91.
+    """
+    temp = _es("sb,mnls->mnlb", C4, gao)        # :71-73
+    temp2 = _es("lj,mnlb->mnjb", C3, temp)      # :77-79
+    temp = _es("na,mnjb->majb", C2, temp2)      # :83-85
+    return _es("mi,majb->iajb", C1, temp)       # :89-91
+
+
+def tei_transform_bruteforce(gao, C1, C2, C3, C4):
+    """Eight nested loops; only for n <= 4."""
+    n = gao.shape[0]
+    d1, d2, d3, d4 = C1.shape[1], C2.shape[1], C3.shape[1], C4.shape[1]
+    out = np.zeros((d1, d2, d3, d4))
+    for p in range(d1):
+        for q in range(d2):
+            for r in range(d3):
+                for s in range(d4):
+                    acc = 0.0
+                    for m in range(n):
+                        for nu in range(n):
+                            for l in range(n):
+                                for sg in range(n):
+                                    acc += (C1[m, p] * C2[nu, q] * C3[l, r] * C4[sg, s]
+                                            * gao[m, nu, l, sg])
+                    out[p, q, r, s] = acc
+    return out
+
+
+def mo_from_factors(B, C1, C2, C3, C4):
+    """Exact 3-index route: ``(pq|rs) = sum_P (C1^T B_P C2)_pq (C3^T B_P C4)_rs``."""
+    L = _es("mp,mnP,nq->pqP", C1, B, C2)
+    R = _es("mp,mnP,nq->pqP", C3, B, C4)
+    return _es("pqP,rsP->pqrs", L, R)
+
+
+def get_eri(gao, Cao, Cav, eri_string, notation="phys", fcn=0):
+    """IntegralTransformation.jl:38-101 (RHF: lower- and upper-case letters coincide)."""
+    if len(eri_string) != 4:
+        raise ValueError(f"Invalid string given to get_eri: {eri_string}")
+    s = eri_string
+    if notation == "phys":
+        s = s[0] + s[2] + s[1] + s[3]                       # :46-48
+    Cs = []
+    for ch in s:                                            # :52-68
+        if ch in "oO":
+            Cs.append(Cao[:, fcn:])
+        elif ch in "vV":
+            Cs.append(Cav)
+        else:
+            raise ValueError(f"Invalid string given to get_eri: {eri_string}")
+    V = tei_transform(gao, *Cs)                             # :94
+    if notation == "phys":
+        V = np.transpose(V, (0, 2, 1, 3))                   # :96-98
+    return np.ascontiguousarray(V)
+
+
+# ---------------------------------------------------------------------------
+# MollerPlesset/RMP2.jl:11-45 and CoupledCluster/Denominators.jl:14-33
+# ---------------------------------------------------------------------------
+
+def form_Dijab(eps, o, v):
+    """Denominators.jl:14-33: ``D[i,j,a,b] = e_i + e_j - e_{o+a} - e_{o+b}``."""
+    eo, ev = eps[:o], eps[o:o + v]
+    return (eo[:, None, None, None] + eo[None, :, None, None]
+            - ev[None, None, :, None] - ev[None, None, None, :])
+
+
+def form_Dia(eps, o, v):
+    """mRCCSD.jl:64: ``D[i,a] = e_i - e_{o+a}``."""
+    return eps[:o, None] - eps[None, o:o + v]
+
+
+def rmp2(moeri, eps, o):
+    """RMP2.jl:27-41 vectorised: ``sum K[ijab](2K[ijab] - K[ijba]) / D``."""
+    v = moeri.shape[2]
+    D = form_Dijab(eps, o, v)
+    return float(np.sum(moeri * (2.0 * moeri - moeri.transpose(0, 1, 3, 2)) / D))
+
+
+def rmp2_loops(moeri, eps, o):
+    """RMP2.jl:27-41 literally (b, a, j, i loop order); small inputs only."""
+    v = moeri.shape[2]
+    dmp2 = 0.0
+    for b in range(v):
+        for a in range(v):
+            for j in range(o):
+                for i in range(o):
+                    dmp2 += (moeri[i, j, a, b] * (2 * moeri[i, j, a, b] - moeri[i, j, b, a])
+                             / (eps[i] + eps[j] - eps[o + a] - eps[o + b]))
+    return dmp2
+
+
+def rmp2_grad(gao, C, eps, o):
+    """Analytic gradient of the RMP2 energy w.r.t. C (n x n) and g (n^4).
+
+    ``E = sum K_iajb (2K_iajb - K_ibja)/D_ijab`` with ``K = (ia|jb)``.  Every K
+    entry independent: ``G_iajb = dE/dK_iajb = 2(2K_iajb - K_ibja)/D_ijab``;
+    ``dE/dg = sum G C_mu,i C_nu,a C_lam,j C_sig,b``;
+    ``dE/dC[:,occ]`` and ``dE/dC[:,vir]`` collect the four positions C enters.
+    No symmetry of g is assumed.  Returns (E, dE/dC, dE/dg).
+    """
+    Co, Cv = C[:, :o], C[:, o:]
+    v = Cv.shape[1]
+    K = tei_transform(gao, Co, Cv, Co, Cv)                 # (ia|jb) as [i,a,j,b]
+    D = form_Dijab(eps, o, v).transpose(0, 2, 1, 3)        # D as [i,a,j,b]
+    Kx = K.transpose(0, 3, 2, 1)                           # K_ibja as [i,a,j,b]
+    E = float(np.sum(K * (2 * K - Kx) / D))
+    G = 2.0 * (2.0 * K - Kx) / D
+    dg = _es("iajb,mi,na,lj,sb->mnls", G, Co, Cv, Co, Cv)
+    # positions: C1 (occ, index mu), C2 (vir, nu), C3 (occ, lam), C4 (vir, sig)
+    h1 = _es("iajb,na,lj,sb->inls", G, Cv, Co, Cv)
+    dCo = _es("inls,mnls->mi", h1, gao)
+    h3 = _es("iajb,mi,na,sb->mnjs", G, Co, Cv, Cv)
+    dCo += _es("mnjs,mnls->lj", h3, gao)
+    h2 = _es("iajb,mi,lj,sb->mals", G, Co, Co, Cv)
+    dCv = _es("mals,mnls->na", h2, gao)
+    h4 = _es("iajb,mi,na,lj->mnlb", G, Co, Cv, Co)
+    dCv += _es("mnlb,mnls->sb", h4, gao)
+    dC = np.concatenate([dCo, dCv], axis=1)
+    return E, dC, dg
+
+
+def rmp2_energy_from_C(gao, C, eps, o):
+    Co, Cv = C[:, :o], C[:, o:]
+    K = tei_transform(gao, Co, Cv, Co, Cv)
+    return rmp2(K.transpose(0, 2, 1, 3), eps, o)
+
+
+# ---------------------------------------------------------------------------
+# CoupledCluster/RCCD.jl (the @tensoropt form)
+# ---------------------------------------------------------------------------
+
+def make_rccd_integrals(gao, Cao, Cav):
+    """RCCD.jl:85-97: physicist blocks oovv, ovov, ovvo, oooo, vvvv."""
+    oovv = np.transpose(tei_transform(gao, Cao, Cav, Cao, Cav), (0, 2, 1, 3))
+    vvvv = np.transpose(tei_transform(gao, Cav, Cav, Cav, Cav), (0, 2, 1, 3))
+    ovvo = np.transpose(tei_transform(gao, Cao, Cav, Cav, Cao), (0, 2, 1, 3))
+    ovov = np.transpose(tei_transform(gao, Cao, Cao, Cav, Cav), (0, 2, 1, 3))
+    oooo = np.transpose(tei_transform(gao, Cao, Cao, Cao, Cao), (0, 2, 1, 3))
+    return dict(oovv=oovv, ovov=ovov, ovvo=ovvo, oooo=oooo, vvvv=vvvv)
+
+
+def rccd_integrals_from_factors(B, Cao, Cav):
+    blk = lambda A, Bc, Cc, Dc: np.transpose(mo_from_factors(B, A, Bc, Cc, Dc), (0, 2, 1, 3))
+    return dict(oovv=blk(Cao, Cav, Cao, Cav), ovov=blk(Cao, Cao, Cav, Cav),
+                ovvo=blk(Cao, Cav, Cav, Cao), oooo=blk(Cao, Cao, Cao, Cao),
+                vvvv=blk(Cav, Cav, Cav, Cav))
+
+
+def ccd_energy(t, oovv):
+    """RCCD.jl:99-117: ``E = sum oovv[ijab](2t[ijab] - t[jiab])``."""
+    return float(np.sum(oovv * (2.0 * t - t.transpose(1, 0, 2, 3))))
+
+
+def rccd_residual_tensor(t, I):
+    """RCCD.jl:134-139 + form_T2 :242-256 before the division by D (:301)."""
+    oovv, ovov, ovvo, oooo, vvvv = I["oovv"], I["ovov"], I["ovvo"], I["oooo"], I["vvvv"]
+    Fae = -_es("mnef,mnaf->ae", oovv, 2 * t - t.transpose(1, 0, 2, 3))           # :178-180
+    Fmi = _es("mnef,inef->mi", oovv, 2 * t - t.transpose(0, 1, 3, 2))            # :215-217
+    Wmnij = oooo + _es("ijef,mnef->mnij", t, oovv) / 2                           # :320-322
+    Wabef = _es("mnab,mnef->abef", t, oovv) / 2 + vvvv                           # :358-360
+    WmBeJ = (ovvo + _es("mnef,njfb->mbej", oovv, 2 * t - t.transpose(1, 0, 2, 3)) / 2
+             - _es("nmef,njfb->mbej", oovv, t) / 2)                              # :399-402
+    WmBEj = -ovov.transpose(0, 1, 3, 2) + _es("jnfb,nmef->mbej", t, oovv) / 2    # :444-446
+    R = (oovv + _es("ijae,be->ijab", t, Fae) + _es("jibe,ae->ijab", t, Fae)
+         - _es("imab,mj->ijab", t, Fmi) - _es("mjab,mi->ijab", t, Fmi)
+         + _es("mnab,mnij->ijab", t, Wmnij)
+         + _es("ijef,abef->ijab", t, Wabef)
+         + 2 * _es("imae,mbej->ijab", t, WmBeJ)
+         - _es("miae,mbej->ijab", t, WmBeJ)
+         + _es("imae,mbej->ijab", t, WmBEj)
+         + _es("mibe,maej->ijab", t, WmBEj)
+         + _es("mjae,mbei->ijab", t, WmBEj)
+         + 2 * _es("jmbe,maei->ijab", t, WmBeJ)
+         - _es("mjbe,maei->ijab", t, WmBeJ)
+         + _es("jmbe,maei->ijab", t, WmBEj))
+    return R
+
+
+def rccd_step(t, I, D):
+    """One Jacobi update ``t_new = R(t) ./ D`` (RCCD.jl:301)."""
+    return rccd_residual_tensor(t, I) / D
+
+
+# ---------------------------------------------------------------------------
+# CoupledCluster/mRCCD.jl: the explicit GEMM formulation (second restatement)
+# ---------------------------------------------------------------------------
+
+def _gemm(tA, tB, alpha, A, B, beta, C):
+    opA = A.T if tA == "T" else A
+    opB = B.T if tB == "T" else B
+    return alpha * (opA @ opB) + beta * C
+
+
+def mrccd_residual_tensor(t, I):
+    """mRCCD.jl:159-164 + 265-488 (the ``./ Dijab`` of :391 left out)."""
+    oovv, ovov, ovvo, oooo, vvvv = I["oovv"], I["ovov"], I["ovvo"], I["oooo"], I["vvvv"]
+    o, v = t.shape[0], t.shape[2]
+    # form_Fae! :265-275
+    X = 2 * _rs(_pd(t, (3, 1, 2, 4)), v, o * o * v) - _rs(_pd(t, (3, 2, 1, 4)), v, o * o * v)
+    Fae = _gemm("N", "N", -1.0, X, _rs(_pd(oovv, (1, 2, 4, 3)), o * o * v, v), 0.0, 0)
+    # form_Fmi! :285-299
+    X = 2 * _rs(_pd(t, (2, 3, 4, 1)), o * v * v, o) - _rs(_pd(t, (2, 4, 3, 1)), o * v * v, o)
+    Fmi = _gemm("N", "N", 1.0, _rs(oovv, o, o * v * v), X, 0.0, 0)
+    # form_Wmnij! :403-415
+    Wmnij = _rs(_gemm("N", "T", 0.5, _rs(oovv, o * o, v * v), _rs(t, o * o, v * v), 1.0,
+                      _rs(oooo, o * o, o * o)), o, o, o, o)
+    # form_Wabef! :424-436
+    Wabef = _rs(_gemm("T", "N", 0.5, _rs(t, o * o, v * v), _rs(oovv, o * o, v * v), 1.0,
+                      _rs(vvvv, v * v, v * v)), v, v, v, v)
+    # form_WmBeJ! :446-464
+    W = _rs(_pd(ovvo, (1, 3, 4, 2)), o * v, o * v)
+    t1 = 2 * _rs(_pd(t, (1, 3, 2, 4)), o * v, o * v) - _rs(_pd(t, (2, 3, 1, 4)), o * v, o * v)
+    W = _gemm("N", "N", 0.5, _rs(_pd(oovv, (1, 3, 2, 4)), o * v, o * v), t1, 1.0, W)
+    W = _gemm("T", "N", -0.5, _rs(_pd(oovv, (1, 4, 2, 3)), o * v, o * v),
+              _rs(_pd(t, (1, 3, 2, 4)), o * v, o * v), 1.0, W)
+    WmBeJ = _pd(_rs(W, o, v, o, v), (1, 4, 2, 3))
+    # form_WmBEj! :475-488
+    W = -_pd(ovov, (1, 2, 4, 3))
+    W = _rs(_pd(W, (1, 3, 4, 2)), o * v, o * v)
+    W = _gemm("N", "N", 0.5, _rs(_pd(oovv, (2, 3, 1, 4)), o * v, o * v),
+              _rs(_pd(t, (2, 3, 1, 4)), o * v, o * v), 1.0, W)
+    WmBEj = _pd(_rs(W, o, v, o, v), (1, 4, 2, 3))
+
+    # form_T2 :301-394
+    d = oovv.copy()
+    d = _rs(_gemm("N", "T", 1.0, _rs(t, o * o * v, v), Fae, 1.0, _rs(d, o * o * v, v)), o, o, v, v)
+    X = _tperm(d, (2, 1, 4, 3))
+    X = _gemm("N", "T", 1.0, _rs(t, o * o * v, v), Fae, 1.0, _rs(X, o * o * v, v))
+    d = _pd(_rs(X, o, o, v, v), (2, 1, 4, 3))
+    X = _rs(_tperm(d, (1, 3, 4, 2)), o * v * v, o)
+    X = _gemm("T", "N", -1.0, _rs(_pd(t, (2, 1, 3, 4)), o, o * v * v), Fmi, 1.0, X)
+    d = _pd(_rs(X.T, o, o, v, v), (2, 1, 3, 4))
+    X = _rs(_tperm(d, (2, 3, 4, 1)), o * v * v, o)
+    X = _gemm("T", "N", -1.0, _rs(t, o, o * v * v), Fmi, 1.0, X)
+    d = _rs(X.T, o, o, v, v)
+    X = _rs(d, o * o, v * v)
+    X = _gemm("T", "N", 1.0, _rs(Wmnij, o * o, o * o), _rs(t, o * o, v * v), 1.0, X)
+    X = _gemm("N", "T", 1.0, _rs(t, o * o, v * v), _rs(Wabef, v * v, v * v), 1.0, X)
+    d = _rs(X, o, o, v, v)
+
+    X = _rs(_tperm(d, (1, 3, 4, 2)), o * v, v * o)
+    scr2 = _rs(_tperm(t, (1, 3, 2, 4)), o * v, o * v)
+    scr3 = 2 * _rs(_tperm(WmBeJ, (1, 3, 2, 4)), o * v, v * o)
+    it = _rs(_tperm(WmBEj, (1, 3, 2, 4)), o * v, v * o)
+    scr3 = scr3 + it
+    X = _gemm("N", "N", 1.0, scr2, scr3, 1.0, X)
+    scr3 = (scr3 - it) * 0.5
+    scr2 = _rs(_pd(t, (1, 4, 2, 3)), o * v, o * v)
+    X = _gemm("T", "N", -1.0, scr2, scr3, 1.0, X)
+    d = _pd(_rs(X, o, v, v, o), (1, 4, 2, 3))
+
+    X = _rs(_pd(d, (1, 4, 3, 2)), o * v, v * o)
+    scr2 = _rs(_pd(t, (1, 4, 2, 3)), o * v, o * v)
+    scr4 = _rs(_pd(WmBEj, (1, 3, 2, 4)), o * v, o * v)
+    tmp = _gemm("T", "N", 1.0, scr2, scr4, 0.0, 0)
+    X = X + tmp
+    d = _pd(_rs(X, o, v, v, o), (1, 4, 3, 2))
+    scr1 = _rs(_pd(d, (2, 3, 4, 1)), o * v, v * o) + tmp
+    d = _pd(_rs(scr1, o, v, v, o), (4, 1, 2, 3))
+    scr1 = _rs(_pd(d, (2, 4, 3, 1)), o * v, v * o)
+    scr2 = _rs(_pd(t, (1, 3, 2, 4)), o * v, o * v)
+    scr1 = _gemm("N", "N", 1.0, scr2, scr4, 1.0, scr1)
+    scr2 = 2 * scr2 - _rs(_pd(t, (2, 3, 1, 4)), o * v, o * v)
+    scr1 = _gemm("N", "N", 1.0, scr2, scr3, 1.0, scr1)
+    return _pd(_rs(scr1, o, v, v, o), (4, 1, 3, 2))
+
+
+# ---------------------------------------------------------------------------
+# DIIS as in mRCCD.jl:64-67, 171-208 (kept in FP64 here)
+# ---------------------------------------------------------------------------
+
+class DIIS:
+    """Pulay extrapolation with mRCCD.jl's bookkeeping.
+
+    ``max_vecs`` is the number of (amplitude, error) pairs kept; mRCCD.jl's
+    ``max_diis = 6`` (a list that also holds the starting guess) keeps 5.
+    """
+
+    def __init__(self, max_vecs=5):
+        self.max_vecs = max_vecs
+        self.vals, self.errs = [], []
+
+    def push(self, t_new, err):
+        self.vals.append(np.array(t_new, copy=True))
+        self.errs.append(np.array(err, copy=True))
+        if len(self.vals) > self.max_vecs:                  # :180-183
+            del self.vals[0]
+            del self.errs[0]
+
+    def extrapolate(self):
+        m = len(self.errs)
+        B = -np.ones((m + 1, m + 1))                        # :185-186
+        B[m, m] = 0.0
+        for a in range(m):
+            for b in range(m):
+                B[a, b] = np.vdot(self.errs[a].ravel(), self.errs[b].ravel())   # :188-192
+        B[:m, :m] /= np.max(np.abs(B[:m, :m]))             # :194
+        rhs = np.zeros(m + 1)
+        rhs[m] = -1.0                                       # :195-196
+        c = np.linalg.solve(B, rhs)                         # :198 (inv(B)*r)
+        out = np.zeros_like(self.vals[0])
+        for k in range(m):                                  # :199-202
+            out += c[k] * self.vals[k]
+        return out, c[:m]
+
+
+def solve_rccd(I, eps, o, *, guess="zero", diis=True, max_vecs=5, maxit=100,
+               e_conv=1e-13, r_conv=1e-12, residual=rccd_residual_tensor, history=None):
+    """Fixed point of ``t = R(t)/D``; Jacobi (RCCD.jl:55-74) or DIIS (mRCCD.jl:68-109).
+
+    ``guess``: "zero" (mRCCD.jl:245), "ovov" ((ij|ab)/D, RCCD.jl:145-160) or
+    "mp2" (oovv/D).  Converged when |dE| < e_conv and ||t_new - t||_2 < r_conv.
+    """
+    v = eps.shape[0] - o
+    D = form_Dijab(eps, o, v)
+    oovv = I["oovv"]
+    if guess == "zero":
+        t = np.zeros_like(oovv)
+    elif guess == "ovov":
+        t = I["ovov"].transpose(0, 2, 1, 3) / D
+    elif guess == "mp2":
+        t = oovv / D
+    else:
+        raise ValueError(guess)
+    d = DIIS(max_vecs) if diis else None
+    E_old = ccd_energy(t, oovv)
+    for it in range(maxit):
+        t_new = residual(t, I) / D
+        err = t_new - t
+        E_new = ccd_energy(t_new, oovv)
+        rn = float(np.linalg.norm(err))
+        if history is not None:
+            history.append((E_new, rn))
+        if d is not None:
+            d.push(t_new, err)
+            t_next, _ = d.extrapolate()
+        else:
+            t_next = t_new
+        dE = E_new - E_old
+        E_old = E_new
+        t = t_next
+        if abs(dE) < e_conv and rn < r_conv:
+            break
+    return ccd_energy(t, oovv), t, it + 1
+
+
+# ---------------------------------------------------------------------------
+# CoupledCluster/mRCCSD.jl (6 physicist blocks)
+# ---------------------------------------------------------------------------
+
+def make_rccsd_integrals(gao, Cao, Cav):
+    """mRCCSD.jl:102-110 via get_eri (physicist)."""
+    g = lambda s: get_eri(gao, Cao, Cav, s)
+    return dict(oooo=g("oooo"), ooov=g("ooov"), oovv=g("oovv"), ovov=g("ovov"),
+                ovvv=g("ovvv"), vvvv=g("vvvv"))
+
+
+def rccsd_integrals_from_factors(B, Cao, Cav):
+    C = {"o": Cao, "v": Cav}
+    def phys(s):
+        chem = s[0] + s[2] + s[1] + s[3]
+        X = mo_from_factors(B, *[C[ch] for ch in chem])
+        return np.ascontiguousarray(np.transpose(X, (0, 2, 1, 3)))
+    return {s: phys(s) for s in ("oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")}
+
+
+def ccsd_energy(oovv, t1, t2):
+    """mRCCSD.jl:111-117 with tiatia = t1 (x) t1."""
+    tt = _es("ma,nf->mnaf", t1, t1)
+    return float(np.sum(oovv * (2 * t2 + 2 * tt - t2.transpose(1, 0, 2, 3) - tt.transpose(1, 0, 2, 3))))
+
+
+def rccsd_update(t1, t2, I):
+    """mRCCSD.jl cciter :118-135 (intermediates :137-201, T1 :202-212, T2 :213-241)."""
+    oooo, ooov, oovv, ovov, ovvv, vvvv = (I["oooo"], I["ooov"], I["oovv"], I["ovov"],
+                                          I["ovvv"], I["vvvv"])
+    o, v = t1.shape
+    tt = _es("ma,nf->mnaf", t1, t1)
+    # form_Fae! :137-149 (GEMM form restated as contractions)
+    _ovvv = (2 * _rs(_pd(ovvv, (1, 3, 2, 4)), o * v, v * v)
+             - _rs(_pd(ovvv, (1, 4, 3, 2)), o * v, v * v))
+    Fae = _rs(_rs(t1, 1, o * v) @ _ovvv, v, v)
+    _t = _rs(_pd(t2, (3, 1, 2, 4)), v, o * o * v) + 0.5 * _rs(_pd(tt, (3, 1, 2, 4)), v, o * o * v)
+    _o = (2 * _rs(_pd(oovv, (1, 2, 4, 3)), o * o * v, v)
+          - _rs(_pd(oovv, (2, 1, 4, 3)), o * o * v, v))
+    Fae = Fae - _t @ _o
+    # form_Fmi! :150-162
+    _ooov = (2 * _rs(_pd(ooov, (2, 4, 1, 3)), o * v, o * o)
+             - _rs(_pd(ooov, (1, 4, 2, 3)), o * v, o * o))
+    Fmi = _rs(_rs(t1, 1, o * v) @ _ooov, o, o)
+    _o = 2 * _rs(oovv, o, v * v * o) - _rs(_pd(oovv, (1, 2, 4, 3)), o, v * v * o)
+    _t = _rs(_pd(t2, (2, 3, 4, 1)), v * v * o, o) + 0.5 * _rs(_pd(tt, (2, 3, 4, 1)), v * v * o, o)
+    Fmi = Fmi + _o @ _t
+    # form_Fme! :163-168
+    Fme = _es("nf,mnef->me", t1, 2 * oovv - oovv.transpose(1, 0, 2, 3))
+    # form_Wmnij! :169-176
+    Wmnij = (oooo + _es("je,mnie->mnij", t1, ooov) + _es("ie,nmje->mnij", t1, ooov)
+             + 0.5 * _es("ijef,mnef->mnij", t2 + tt, oovv))
+    # form_Wabef! :177-184
+    Wabef = (vvvv - _es("mb,mafe->abef", t1, ovvv) - _es("ma,mbef->abef", t1, ovvv)
+             + 0.5 * _es("mnab,mnef->abef", t2 + tt, oovv))
+    # form_WmBeJ! :185-193
+    WmBeJ = (oovv.transpose(0, 3, 2, 1) + _es("jf,mbef->mbej", t1, ovvv)
+             - _es("nb,nmje->mbej", t1, ooov)
+             - 0.5 * _es("jnfb,mnef->mbej", t2 + 2 * tt, oovv)
+             + 0.5 * _es("njfb,mnef->mbej", t2, 2 * oovv - oovv.transpose(1, 0, 2, 3)))
+    # form_WmBEj! :194-201
+    WmBEj = (-ovov.transpose(0, 1, 3, 2) - _es("jf,mbfe->mbej", t1, ovvv)
+             + _es("nb,mnje->mbej", t1, ooov)
+             + _es("jnfb,nmef->mbej", 0.5 * t2 + tt, oovv))
+    # form_T1 :202-212
+    n1 = (_es("ie,ae->ia", t1, Fae) - _es("ma,mi->ia", t1, Fmi)
+          + _es("me,imae->ia", Fme, 2 * t2 - t2.transpose(1, 0, 2, 3))
+          + _es("me,imae->ia", t1, 2 * oovv) - _es("me,maie->ia", t1, ovov)
+          - _es("mnae,mnie->ia", t2, 2 * ooov - ooov.transpose(1, 0, 2, 3))
+          + _es("imef,mafe->ia", t2, 2 * ovvv) - _es("imef,maef->ia", t2, ovvv))
+    # form_T2 :213-241
+    Fb = Fae - 0.5 * _es("mb,me->be", t1, Fme)
+    Fj = Fmi + 0.5 * _es("je,me->mj", t1, Fme)
+    n2 = (oovv
+          + _es("ijae,be->ijab", t2, Fb) + _es("ijeb,ae->ijab", t2, Fb)
+          - _es("imab,mj->ijab", t2, Fj) - _es("mjab,mi->ijab", t2, Fj)
+          + _es("mnab,mnij->ijab", t2 + tt, Wmnij)
+          + _es("ijef,abef->ijab", t2 + tt, Wabef)
+          + _es("imae,mbej->ijab", t2 - t2.transpose(1, 0, 2, 3), WmBeJ)
+          - _es("imea,mjeb->ijab", tt, oovv)
+          + _es("imae,mbej->ijab", t2, WmBeJ + WmBEj)
+          + _es("mibe,maej->ijab", t2, WmBEj) - _es("imeb,maje->ijab", tt, ovov)
+          + _es("mjae,mbei->ijab", t2, WmBEj) - _es("jmea,mbie->ijab", tt, ovov)
+          + _es("jmbe,maei->ijab", t2 - t2.transpose(1, 0, 2, 3), WmBeJ)
+          - _es("jmeb,miea->ijab", tt, oovv)
+          + _es("jmbe,maei->ijab", t2, WmBeJ + WmBEj)
+          + _es("ei,jabe->ijab", t1.T, ovvv) + _es("ej,ibae->ijab", t1.T, ovvv)
+          - _es("ma,mjib->ijab", t1, ooov) - _es("mb,jima->ijab", t1, ooov))
+    return n1, n2
+
+
+def rccsd_step(t1, t2, I, Dia, Dijab):
+    n1, n2 = rccsd_update(t1, t2, I)
+    return n1 / Dia, n2 / Dijab
+
+
+# ---------------------------------------------------------------------------
+# CoupledCluster/RCCSD.jl (15 blocks) -- second RCCSD restatement
+# ---------------------------------------------------------------------------
+
+def make_rccsd15_integrals(tei, Cao, Cav):
+    """RCCSD.jl:117-142: 15 blocks, each from its own transform and permutation.
+
+    ``tei(C1, C2, C3, C4)`` returns the chemist tensor (e.g. ``tei_transform``
+    on dense AO integrals or ``mo_from_factors`` on factors).
+    """
+    o, v = Cao, Cav
+    P = lambda X: np.transpose(X, (0, 2, 1, 3))            # permutedims(.,[1,3,2,4])
+    b = dict(
+        vvvv=P(tei(v, v, v, v)), ovvv=P(tei(o, v, v, v)), vovv=P(tei(v, v, o, v)),
+        vvov=P(tei(v, o, v, v)), vvvo=P(tei(v, v, v, o)), oovv=P(tei(o, v, o, v)),
+        ovvo=P(tei(o, v, v, o)), vovo=P(tei(v, v, o, o)), ovov=P(tei(o, o, v, v)),
+        voov=P(tei(v, o, o, v)), ooov=P(tei(o, o, o, v)), oovo=P(tei(o, v, o, o)),
+        ovoo=P(tei(o, o, v, o)), vooo=P(tei(v, o, o, o)), oooo=P(tei(o, o, o, o)))
+    b["vvov"] = _pd(b["vvov"], (4, 1, 2, 3))                # :137
+    b["vvvo"] = _pd(b["vvvo"], (3, 1, 2, 4))                # :138
+    b["vovv"] = _pd(b["vovv"], (2, 1, 3, 4))                # :139
+    b["vooo"] = _pd(b["vooo"], (2, 1, 3, 4))                # :140
+    return b
+
+
+def rccsd15_update(t1, t2, b):
+    """RCCSD.jl cciter (:150-172) with form_* (:187-289), argument wiring kept."""
+    oovv, vovv, ovvv, ooov, oooo = b["oovv"], b["vovv"], b["ovvv"], b["ooov"], b["oooo"]
+    oovo, vvvv, ovvo, voov, vovo = b["oovo"], b["vvvv"], b["ovvo"], b["voov"], b["vovo"]
+    ovov, vvvo, vvov, ovoo, vooo = b["ovov"], b["vvvo"], b["vvov"], b["ovoo"], b["vooo"]
+    tt = _es("ma,nf->mnaf", t1, t1)
+    # form_Fae!(Fae, maef=ovvv, amef=vovv, mnef=oovv)
+    maef, amef, mnef = ovvv, vovv, oovv
+    Fae = (_es("mf,maef->ae", t1, 2 * amef - maef)
+           - _es("mnaf,mnef->ae", t2 + 0.5 * tt, 2 * mnef - mnef.transpose(1, 0, 2, 3)))
+    # form_Fmi!(Fmi, mnie=ooov, mnef=oovv)
+    mnie = ooov
+    Fmi = (_es("ne,mnie->mi", t1, 2 * mnie - mnie.transpose(1, 0, 2, 3))
+           + _es("inef,mnef->mi", t2 + 0.5 * tt, 2 * mnef - mnef.transpose(0, 1, 3, 2)))
+    Fme = _es("nf,mnef->me", t1, 2 * mnef - mnef.transpose(1, 0, 2, 3))
+    # form_Wabef!(Wabef, abef=vvvv, amef=vovv, mbef=ovvv, mnef=oovv)
+    Wabef = (vvvv - _es("mb,maef->abef", t1, vovv) - _es("ma,mbef->abef", t1, ovvv)
+             + 0.5 * _es("mnab,mnef->abef", t2 + tt, oovv))
+    # form_Wmnij!(Wmnij, mnij=oooo, mnie=ooov, mnej=oovo, mnef=oovv)
+    Wmnij = (oooo + _es("je,mnie->mnij", t1, ooov) + _es("ie,mnej->mnij", t1, oovo)
+             + 0.5 * _es("ijef,mnef->mnij", t2 + tt, oovv))
+    # form_WmBeJ!(WmBeJ, mbej=ovvo, mbef=ovvv, mnej=oovo, mnef=oovv)
+    WmBeJ = (ovvo + _es("jf,mbef->mbej", t1, ovvv) - _es("nb,mnej->mbej", t1, oovo)
+             - 0.5 * _es("jnfb,mnef->mbej", t2 + 2 * tt, oovv)
+             + 0.5 * _es("njfb,mnef->mbej", t2, 2 * oovv - oovv.transpose(1, 0, 2, 3)))
+    # form_WmBEj!(WmBEj, bmej=vovo, mbfe=ovvv, nmej=oovo, nmef=oovv)
+    WmBEj = (-_es("bmej->mbej", vovo) - _es("jf,mbfe->mbej", t1, ovvv)
+             + _es("nb,nmej->mbej", t1, oovo)
+             + _es("jnfb,nmef->mbej", 0.5 * t2 + tt, oovv))
+    # form_T1(Fae,Fmi,Fme, amie=voov, maie=ovov, mnie=ooov, amef=vovv)
+    n1 = (_es("ie,ae->ia", t1, Fae) - _es("ma,mi->ia", t1, Fmi)
+          + _es("me,imae->ia", Fme, 2 * t2 - t2.transpose(1, 0, 2, 3))
+          + _es("me,amie->ia", t1, 2 * voov) - _es("me,maie->ia", t1, ovov)
+          - _es("mnae,mnie->ia", t2, 2 * ooov - ooov.transpose(1, 0, 2, 3))
+          + _es("imef,maef->ia", t2, 2 * vovv - vovv.transpose(0, 1, 3, 2)))
+    # form_T2(..., ijab=oovv, mbej=ovvo, amej=vovo, abej=vvvo, abie=vvov, mbij=ovoo, amij=vooo)
+    Fb = Fae - 0.5 * _es("mb,me->be", t1, Fme)
+    Fj = Fmi + 0.5 * _es("je,me->mj", t1, Fme)
+    n2 = (oovv
+          + _es("ijae,be->ijab", t2, Fb) + _es("ijeb,ae->ijab", t2, Fb)
+          - _es("imab,mj->ijab", t2, Fj) - _es("mjab,mi->ijab", t2, Fj)
+          + _es("mnab,mnij->ijab", t2 + tt, Wmnij)
+          + _es("ijef,abef->ijab", t2 + tt, Wabef)
+          + _es("imae,mbej->ijab", t2 - t2.transpose(1, 0, 2, 3), WmBeJ)
+          - _es("imea,mbej->ijab", tt, ovvo)
+          + _es("imae,mbej->ijab", t2, WmBeJ + WmBEj)
+          + _es("mibe,maej->ijab", t2, WmBEj) - _es("imeb,amej->ijab", tt, vovo)
+          + _es("mjae,mbei->ijab", t2, WmBEj) - _es("jmea,bmei->ijab", tt, vovo)
+          + _es("jmbe,maei->ijab", t2 - t2.transpose(1, 0, 2, 3), WmBeJ)
+          - _es("jmeb,maei->ijab", tt, ovvo)
+          + _es("jmbe,maei->ijab", t2, WmBeJ + WmBEj)
+          + _es("ei,eabj->ijab", t1.T, vvvo) + _es("ej,eabi->ijab", t1.T, vvov)
+          - _es("ma,mbij->ijab", t1, ovoo) - _es("mb,maij->ijab", t1, vooo))
+    return n1, n2
+
+
+def solve_rccsd(I, eps, o, *, maxit=200, e_conv=1e-13, r_conv=1e-12, diis=True, max_vecs=5,
+                update=rccsd_update, history=None):
+    """Fixed point of mRCCSD.jl's update from the MP2 guess (:66).
+
+    mRCCSD.jl runs 40 undamped iterations; here the same map is iterated to
+    convergence, optionally with the DIIS of mRCCD.jl applied to [t1; t2].
+    """
+    v = eps.shape[0] - o
+    Dia = form_Dia(eps, o, v)
+    Dijab = form_Dijab(eps, o, v)
+    oovv = I["oovv"]
+    t1 = np.zeros((o, v))
+    t2 = oovv / Dijab
+    d = DIIS(max_vecs) if diis else None
+    E_old = ccsd_energy(oovv, t1, t2)
+    n1s = o * v
+    for it in range(maxit):
+        n1, n2 = update(t1, t2, I)
+        nt1, nt2 = n1 / Dia, n2 / Dijab
+        e1, e2 = nt1 - t1, nt2 - t2
+        E_new = ccsd_energy(oovv, nt1, nt2)
+        rn = float(np.sqrt(np.sum(e1 ** 2) + np.sum(e2 ** 2)))
+        if history is not None:
+            history.append((E_new, rn))
+        if d is not None:
+            d.push(np.concatenate([nt1.ravel(), nt2.ravel()]),
+                   np.concatenate([e1.ravel(), e2.ravel()]))
+            x, _ = d.extrapolate()
+            t1, t2 = x[:n1s].reshape(o, v), x[n1s:].reshape(o, o, v, v)
+        else:
+            t1, t2 = nt1, nt2
+        dE = E_new - E_old
+        E_old = E_new
+        if abs(dE) < e_conv and rn < r_conv:
+            break
+    return ccsd_energy(oovv, t1, t2), t1, t2, it + 1

@@ -1,0 +1,95 @@
+"""GEMM, AO->MO transform, get_eri and RMP2 on the GPU against the CPU oracle."""
+import numpy as np
+import pytest
+
+import jues_oracle as O
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 2e-5)])
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (5, 7, 3), (127, 129, 33), (256, 128, 64), (300, 200, 257),
+                                   (64, 1000, 17)])
+def test_gemm(ctx, prec, tol, ta, tb, M, N, K):
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.normal(size=(K, M) if ta == "T" else (M, K))
+    B = rng.normal(size=(N, K) if tb == "T" else (K, N))
+    C0 = rng.normal(size=(M, N))
+    alpha, beta = 0.7, -1.3
+    ref = alpha * ((A.T if ta == "T" else A) @ (B.T if tb == "T" else B)) + beta * C0
+    out = ctx.gemm_test(prec, ta, tb, alpha, A, B, beta, C0)
+    assert rel_err(out, ref) < tol * max(1.0, np.sqrt(K) / 4)
+
+
+def test_gemm_beta_zero_ignores_nan(ctx):
+    rng = np.random.default_rng(1)
+    A, B = rng.normal(size=(70, 40)), rng.normal(size=(40, 90))
+    C0 = np.full((70, 90), np.nan)
+    out = ctx.gemm_test(0, "N", "N", 1.0, A, B, 0.0, C0)
+    assert rel_err(out, A @ B) < 1e-13
+
+
+@pytest.mark.parametrize("dense", [True, False])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 3e-6)])
+def test_tei_transform(cfg1, dense, prec, tol):
+    from jues import Transformation
+    from jues.Wavefunction import FactoredERI
+    s = cfg1
+    g = s.ao_eri()
+    eri = g if dense else FactoredERI(s.B)
+    Co, Cv = s.Co, s.Cv
+    for Cs in [(Co, Cv, Co, Cv), (Cv, Cv, Co, Co), (Co, Co, Co, Cv), (Cv, Co, Cv, Co), (s.C,) * 4]:
+        ref = O.tei_transform(g, *Cs)
+        out = Transformation.tei_transform(eri, *Cs, prec=prec)
+        assert out.shape == ref.shape
+        assert rel_err(out, ref) < tol, Cs
+
+
+def test_tei_transform_general_tensor(ctx):
+    """No symmetry assumed: random non-symmetric g and four different C's."""
+    from jues import Transformation
+    rng = np.random.default_rng(5)
+    n = 9
+    g = rng.normal(size=(n, n, n, n))
+    Cs = [rng.normal(size=(n, d)) for d in (3, 4, 5, 2)]
+    ref = O.tei_transform_bruteforce(g, *Cs) if n <= 4 else O.tei_transform(g, *Cs)
+    out = Transformation.tei_transform(np.asfortranarray(g), *Cs)
+    assert rel_err(out, ref) < 1e-12
+
+
+@pytest.mark.parametrize("dense", [True, False])
+def test_get_eri(cfg1, dense):
+    from jues import IntegralTransformation as IT
+    from jues.Wavefunction import Wfn
+    s = cfg1
+    g = s.ao_eri()
+    w = Wfn.from_synthetic(s, dense=dense)
+    for blk in ["OOVV", "OVOV", "OVVO", "OOOO", "VVVV", "ooov", "ovvv"]:
+        for note in ["phys", "chem"]:
+            ref = O.get_eri(g, s.Co, s.Cv, blk, notation=note)
+            out = IT.get_eri(w, blk, notation=note)
+            assert rel_err(out, ref) < 1e-12, (blk, note)
+    ref = O.get_eri(g, s.Co, s.Cv, "OOVV", fcn=1)
+    assert rel_err(IT.get_eri(w, "OOVV", fcn=1), ref) < 1e-12
+    from jues import JuesError
+    with pytest.raises(JuesError):
+        IT.get_eri(w, "OOV")
+    with pytest.raises(JuesError):
+        IT.get_eri(w, "OOXV")
+
+
+@pytest.mark.parametrize("dense", [True, False])
+def test_rmp2(cfg1, dense):
+    from jues import MollerPlesset
+    from jues.Wavefunction import Wfn
+    s = cfg1
+    g = s.ao_eri()
+    ref = O.rmp2(O.get_eri(g, s.Co, s.Cv, "OOVV"), s.eps, s.o)
+    w = Wfn.from_synthetic(s, dense=dense)
+    E = MollerPlesset.do_rmp2(w)
+    assert abs(E - ref) < 1e-10
+    E32 = MollerPlesset.do_rmp2(w, prec="fp32_mixed")
+    assert abs(E32 - ref) < 1e-5 * abs(ref)
