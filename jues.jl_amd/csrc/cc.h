@@ -1,0 +1,80 @@
+// Coupled-cluster state shared by cc.hip (RCCD, drivers) and ccsd.hip (RCCSD terms).
+#pragma once
+
+#include "cc_kernels.h"
+#include "jues_internal.h"
+
+namespace jues {
+
+// MO integral blocks in one precision.  ov layouts are [i,a,j,b]; chemist blocks
+// keep the tei_transform order.
+template <typename T>
+struct CCInts {
+  DevBuf Kov;     // (ia|jb) at [i,a,j,b]
+  DevBuf Ktov;    // (ib|ja) at [i,a,j,b]
+  DevBuf Jov;     // (ij|ab) at [i,a,j,b]
+  DevBuf Kpp;     // <ij|ab> = (ia|jb) at [i,j,a,b]
+  DevBuf Ka;      // <mn|ef> at [e,m,n,f]
+  DevBuf oooo;    // <mn|ij> at [m,n,i,j]
+  DevBuf Vp, Vm;  // packed pp-ladder integrals
+  DevBuf ooov_c;  // (ij|ka) at [i,j,k,a]      (RCCSD)
+  DevBuf ovvv_c;  // (ia|bc) at [i,a,b,c]      (RCCSD)
+};
+
+template <typename T>
+struct CCWork {
+  DevBuf Tpp, Upp, Tov, Ttov, Uov, Ua, Tp, Tm, WJ, WE, Y, H, Lhh, Wp, Fae, Fmi, Lp, Lm;
+  // RCCSD
+  DevBuf t1, Fme, X2, Z, Q, n1, U2, Tau;
+};
+
+struct CCState {
+  int method = 0;  // 0 RCCD, 1 RCCSD
+  int prec = 0;
+  jues_cc_opts opts{};
+  int o = 0, v = 0, n = 0;
+  long Nd = 0;     // amplitude vector length: o^2 v^2 (+ o v for RCCSD, T1 stored after T2)
+  PairTables P;
+  DevBuf pair_buf[4];
+  DevBuf eps;      // FP64 orbital energies (frozen core removed)
+  DevBuf Lener;    // FP64 2<ij|ab> - <ji|ab>
+  DevBuf Lener1;   // unused (T1 energy enters through the T2 kernel)
+  DevBuf guess_buf;
+  DevBuf t;        // FP64 master amplitudes [t2 | t1]
+  DevBuf vals, errs, small;  // DIIS storage
+  std::vector<int> active;   // DIIS slots, oldest first
+  DevBuf R1;       // FP64 T1 residual numerator (RCCSD)
+  DevBuf tau64, taup64, ttx64;  // FP64 helpers (RCCSD)
+  std::unique_ptr<CCInts<double>> i64;
+  std::unique_ptr<CCInts<float>> i32;
+  std::unique_ptr<CCWork<double>> w64;
+  std::unique_ptr<CCWork<float>> w32;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double E_prev = 0;
+  bool have_prev = false;
+  int iter = 0;
+  ~CCState() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+template <typename T>
+CCWork<T>& work(CCState& S);
+template <typename T>
+CCInts<T>& ints(CCState& S);
+template <typename T>
+T* bufp_t(DevBuf& b, size_t n) {
+  if (b.bytes < n * sizeof(T)) b.alloc(n * sizeof(T));
+  return b.as<T>();
+}
+
+template <typename T>
+void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2);
+template <typename T>
+void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const double* t1);
+void cc_build_ccsd_ints(jues_ctx* ctx, CCState& S);
+double cc_energy(jues_ctx* ctx, CCState& S);
+double cc_energy_t1(jues_ctx* ctx, CCState& S);
+
+}  // namespace jues

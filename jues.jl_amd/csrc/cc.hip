@@ -1,18 +1,591 @@
-// Coupled-cluster drivers (placeholder entry points until the device loop lands).
-#include "jues_internal.h"
+// Closed-shell coupled-cluster drivers on the device: RCCD (RCCD.jl / mRCCD.jl)
+// and RCCSD (mRCCSD.jl / RCCSD.jl) with DIIS, in FP64 or FP32-MFMA contractions.
+//
+// Layouts (all column-major, FP64 master amplitudes):
+//   ijab  t[i,j,a,b]                         (the Julia T2 order)
+//   ov    X[i,a,j,b], an (ov) x (ov) matrix  (ring terms)
+//   packed T+/T-, V+/V- over pairs i<=j, e<=f (i<j, e<f) for the pp-ladder
+//
+// RCCD residual (RCCD.jl:242-256 with Wabef eliminated, the v^4 intermediate is
+// never formed):
+//   R = <ij|ab> + S[ia,jb] + sum_mn t[mn,ab] W'[mn,ij] + sum_ef t[ij,ef] <ab|ef>
+//   W' = 2 Wmnij - <mn|ij> = <mn|ij> + sum_ef <mn|ef> t[ij,ef]
+//   S = H + H^T with H = U WJ + T WE + (Tx WE)^x + t.Fae^T - Fmi^T.t (ov layout)
+//   WJ = K + K U/2 - Kx T/2,  WE = -J + Kx Tx/2,  U = 2T - Tx
+// where K[ia,jb] = (ia|jb), Kx[ia,jb] = (ib|ja), J[ia,jb] = (ij|ab),
+// T[ia,jb] = t[i,j,a,b], Tx[ia,jb] = t[i,j,b,a] and X^x[ia,jb] = X[ib,ja].
+#include "cc.h"
 #include "api_util.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
 namespace jues {
-struct CCState {};
+
 void CCDeleter::operator()(CCState* p) const { delete p; }
-}  // namespace jues
-using namespace jues;
-extern "C" {
-#define JUES_TODO(ctx) JUES_API_BEGIN(ctx) throw Error(JUES_BAD_ARG, "not implemented"); JUES_API_END(ctx)
-jues_status jues_rmp2_grad(jues_ctx* ctx, int, const double*, int, int, const double*, double*, double*, double*) { JUES_TODO(ctx) }
-jues_status jues_rccd(jues_ctx* ctx, int, const jues_cc_opts*, const double*, int, const double*, int, int, const double*, double*, double*, jues_iter_cb, void*) { JUES_TODO(ctx) }
-jues_status jues_rccsd(jues_ctx* ctx, int, const jues_cc_opts*, const double*, int, const double*, int, int, const double*, double*, double*, double*, jues_iter_cb, void*) { JUES_TODO(ctx) }
-jues_status jues_cc_setup(jues_ctx* ctx, int, int, const jues_cc_opts*, const double*, int, const double*, int, int, const double*, const double*, const double*) { JUES_TODO(ctx) }
-jues_status jues_cc_iterate(jues_ctx* ctx, int, int, int, jues_iter_info*, int*) { JUES_TODO(ctx) }
-jues_status jues_cc_reset_diis(jues_ctx* ctx) { JUES_TODO(ctx) }
-jues_status jues_cc_get(jues_ctx* ctx, double*, double*, double*) { JUES_TODO(ctx) }
+
+namespace {
+
+template <typename T>
+T* bufp(DevBuf& b, size_t n) {
+  return bufp_t<T>(b, n);
 }
+
+void build_pairs(jues_ctx* ctx, CCState& S) {
+  auto mk = [&](int n, bool strict, DevBuf& buf) -> long {
+    std::vector<int2> h;
+    for (int q = 0; q < n; ++q)
+      for (int p = 0; p < (strict ? q : q + 1); ++p) h.push_back(make_int2(p, q));
+    if (h.empty()) h.push_back(make_int2(0, 0));
+    buf.alloc(h.size() * sizeof(int2));
+    h2d(ctx, buf.p, h.data(), h.size() * sizeof(int2));
+    return strict ? (long)n * (n - 1) / 2 : (long)n * (n + 1) / 2;
+  };
+  S.P.npo = mk(S.o, false, S.pair_buf[0]);
+  S.P.nmo = mk(S.o, true, S.pair_buf[1]);
+  S.P.npv = mk(S.v, false, S.pair_buf[2]);
+  S.P.nmv = mk(S.v, true, S.pair_buf[3]);
+  S.P.o_le = S.pair_buf[0].as<int2>();
+  S.P.o_lt = S.pair_buf[1].as<int2>();
+  S.P.v_le = S.pair_buf[2].as<int2>();
+  S.P.v_lt = S.pair_buf[3].as<int2>();
+}
+
+// chemist (vv|vv) slabs -> packed V+/V- (FP64); dense route transforms the whole block
+void build_vvvv(jues_ctx* ctx, CCState& S, CCInts<double>& I, const double* Cv) {
+  const int v = S.v;
+  const long v2 = (long)v * v;
+  double* Vp = bufp<double>(I.Vp, (size_t)(S.P.npv * S.P.npv));
+  double* Vm = bufp<double>(I.Vm, (size_t)std::max(1L, S.P.nmv * S.P.nmv));
+  if (ctx->gao.p) {
+    DevBuf X((size_t)v2 * v2 * sizeof(double));
+    tei_transform_dev<double>(ctx, Cv, v, Cv, v, Cv, v, Cv, v, X.as<double>());
+    pack_v<double, double>(ctx, S.P, X.as<double>(), v, 0, v, Vp, Vm);
+    sync(ctx);
+  } else {
+    const long naux = ctx->naux;
+    DevBuf Lvv((size_t)v2 * naux * sizeof(double));
+    mo_factors_dev<double>(ctx, Cv, v, Cv, v, Lvv.as<double>());
+    // slab of nf virtual indices f: X[a,e,b,f] = sum_P Lvv[ae,P] Lvv[bf,P] (~2 GB per slab)
+    long nf = std::max(1L, std::min<long>(v, (2L << 30) / (long)(v2 * v * sizeof(double))));
+    DevBuf X((size_t)(v2 * v * nf) * sizeof(double));
+    for (int f0 = 0; f0 < v; f0 += (int)nf) {
+      const int f1 = (int)std::min<long>(v, f0 + nf);
+      gemm<double>(ctx, false, true, v2, (long)v * (f1 - f0), naux, 1.0, Lvv.as<double>(), v2, 0,
+                   Lvv.as<double>() + (long)v * f0, v2, 0, 0.0, X.as<double>(), v2, 0, 1, "setup_vvvv");
+      pack_v<double, double>(ctx, S.P, X.as<double>(), v, f0, f1, Vp, Vm);
+    }
+    sync(ctx);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// integral setup
+// ---------------------------------------------------------------------------
+void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv) {
+  const int o = S.o, v = S.v;
+  const long ov = (long)o * v, N2 = ov * ov, oo = (long)o * o;
+  auto& I = *S.i64;
+  double* Kov = bufp<double>(I.Kov, N2);
+  double* Jc = static_cast<double*>(scratch(ctx, 7, N2 * sizeof(double)));
+  double* Oc = static_cast<double*>(scratch(ctx, 8, oo * oo * sizeof(double)));
+  if (ctx->gao.p) {
+    tei_transform_dev<double>(ctx, Co, o, Cv, v, Co, o, Cv, v, Kov);   // (ia|jb) -> [i,a,j,b]
+    tei_transform_dev<double>(ctx, Co, o, Co, o, Cv, v, Cv, v, Jc);    // (ij|ab) -> [i,j,a,b]
+    tei_transform_dev<double>(ctx, Co, o, Co, o, Co, o, Co, o, Oc);    // (mi|nj) -> [m,i,n,j]
+  } else {
+    const long naux = ctx->naux;
+    DevBuf Lov(ov * naux * sizeof(double)), Loo(oo * naux * sizeof(double)), Lvv((long)v * v * naux * sizeof(double));
+    mo_factors_dev<double>(ctx, Co, o, Cv, v, Lov.as<double>());
+    mo_factors_dev<double>(ctx, Co, o, Co, o, Loo.as<double>());
+    mo_factors_dev<double>(ctx, Cv, v, Cv, v, Lvv.as<double>());
+    gemm<double>(ctx, false, true, ov, ov, naux, 1.0, Lov.as<double>(), ov, 0, Lov.as<double>(), ov, 0, 0.0, Kov, ov,
+                 0, 1, "setup_ints");
+    gemm<double>(ctx, false, true, oo, (long)v * v, naux, 1.0, Loo.as<double>(), oo, 0, Lvv.as<double>(),
+                 (long)v * v, 0, 0.0, Jc, oo, 0, 1, "setup_ints");
+    gemm<double>(ctx, false, true, oo, oo, naux, 1.0, Loo.as<double>(), oo, 0, Loo.as<double>(), oo, 0, 0.0, Oc, oo,
+                 0, 1, "setup_ints");
+    if (S.method == 1) {
+      // (ij|ka) -> ooov chem [i,j,k,a];  (ia|bc) -> ovvv chem [i,a,b,c]
+      double* ooov = bufp<double>(I.ooov_c, oo * ov);
+      double* ovvv = bufp<double>(I.ovvv_c, ov * v * v);
+      gemm<double>(ctx, false, true, oo, ov, naux, 1.0, Loo.as<double>(), oo, 0, Lov.as<double>(), ov, 0, 0.0, ooov,
+                   oo, 0, 1, "setup_ints");
+      gemm<double>(ctx, false, true, ov, (long)v * v, naux, 1.0, Lov.as<double>(), ov, 0, Lvv.as<double>(),
+                   (long)v * v, 0, 0.0, ovvv, ov, 0, 1, "setup_ints");
+    }
+  }
+  if (ctx->gao.p && S.method == 1) {
+    tei_transform_dev<double>(ctx, Co, o, Co, o, Co, o, Cv, v, bufp<double>(I.ooov_c, oo * ov));
+    tei_transform_dev<double>(ctx, Co, o, Cv, v, Cv, v, Cv, v, bufp<double>(I.ovvv_c, ov * v * v));
+  }
+  // derived layouts
+  permute<double, double>(ctx, Jc, {o, o, v, v}, {0, 2, 1, 3}, bufp<double>(I.Jov, N2), 1.0, 0.0);
+  permute<double, double>(ctx, Oc, {o, o, o, o}, {0, 2, 1, 3}, bufp<double>(I.oooo, oo * oo), 1.0, 0.0);
+  permute<double, double>(ctx, Kov, {o, v, o, v}, {0, 2, 1, 3}, bufp<double>(I.Kpp, N2), 1.0, 0.0);
+  permute<double, double>(ctx, Kov, {o, v, o, v}, {0, 3, 2, 1}, bufp<double>(I.Ktov, N2), 1.0, 0.0);
+  permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {2, 0, 1, 3}, bufp<double>(I.Ka, N2), 1.0, 0.0);
+  // energy weights 2<ij|ab> - <ji|ab>
+  double* Le = bufp<double>(S.Lener, N2);
+  permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {0, 1, 2, 3}, Le, 2.0, 0.0);
+  permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {1, 0, 2, 3}, Le, -1.0, 1.0);
+  // guess helpers: (ij|ab) in ijab order
+  if (S.opts.guess == JUES_GUESS_OVOV) {
+    double* g = bufp<double>(S.guess_buf, N2);
+    JUES_HIP(hipMemcpyAsync(g, Jc, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  build_vvvv(ctx, S, I, Cv);
+  sync(ctx);
+}
+
+template <typename T>
+static void convert_buf(jues_ctx* ctx, DevBuf& src, DevBuf& dst) {
+  if (!src.p) return;
+  const size_t n = src.bytes / sizeof(double);
+  bufp<T>(dst, n);
+  convert<double, T>(ctx, src.as<double>(), dst.as<T>(), (long)n);
+}
+
+// FP32 copies of the contraction integrals (the FP64 set stays for refinement)
+void cc_make_ints32(jues_ctx* ctx, CCState& S) {
+  if (S.i32) return;
+  S.i32.reset(new CCInts<float>());
+  CCInts<double>& a = *S.i64;
+  CCInts<float>& b = *S.i32;
+  convert_buf<float>(ctx, a.Kov, b.Kov);
+  convert_buf<float>(ctx, a.Ktov, b.Ktov);
+  convert_buf<float>(ctx, a.Jov, b.Jov);
+  convert_buf<float>(ctx, a.Kpp, b.Kpp);
+  convert_buf<float>(ctx, a.Ka, b.Ka);
+  convert_buf<float>(ctx, a.oooo, b.oooo);
+  convert_buf<float>(ctx, a.Vp, b.Vp);
+  convert_buf<float>(ctx, a.Vm, b.Vm);
+  convert_buf<float>(ctx, a.ooov_c, b.ooov_c);
+  convert_buf<float>(ctx, a.ovvv_c, b.ovvv_c);
+  sync(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// residual
+// ---------------------------------------------------------------------------
+template <typename T>
+CCWork<T>& work(CCState& S);
+template <>
+CCWork<double>& work<double>(CCState& S) {
+  if (!S.w64) S.w64.reset(new CCWork<double>());
+  return *S.w64;
+}
+template <>
+CCWork<float>& work<float>(CCState& S) {
+  if (!S.w32) S.w32.reset(new CCWork<float>());
+  return *S.w32;
+}
+template <typename T>
+CCInts<T>& ints(CCState& S);
+template <>
+CCInts<double>& ints<double>(CCState& S) { return *S.i64; }
+template <>
+CCInts<float>& ints<float>(CCState& S) { return *S.i32; }
+
+// Doubles-residual contractions that depend only on the doubles amplitudes `tau2`
+// (t2 for RCCD; the CCSD driver passes its own amplitudes and adds T1 terms).
+// Produces W.S (ov, symmetrised), W.Lhh (ijab), W.Lp / W.Lm (packed).
+template <typename T>
+void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
+  const int o = S.o, v = S.v;
+  const long ov = (long)o * v, N2 = ov * ov, oo = (long)o * o, vv = (long)v * v;
+  CCInts<T>& I = ints<T>(S);
+  CCWork<T>& W = work<T>(S);
+  const std::vector<long> d4 = {o, o, v, v};
+  T* Tpp = bufp<T>(W.Tpp, N2);
+  T* Upp = bufp<T>(W.Upp, N2);
+  T* Tov = bufp<T>(W.Tov, N2);
+  T* Ttov = bufp<T>(W.Ttov, N2);
+  T* Uov = bufp<T>(W.Uov, N2);
+  T* Ua = bufp<T>(W.Ua, N2);
+  T* WJ = bufp<T>(W.WJ, N2);
+  T* WE = bufp<T>(W.WE, N2);
+  T* Y = bufp<T>(W.Y, N2);
+  T* H = bufp<T>(W.H, N2);
+  T* Lhh = bufp<T>(W.Lhh, N2);
+  T* Wp = bufp<T>(W.Wp, oo * oo);
+  T* Fae = bufp<T>(W.Fae, vv);
+  T* Fmi = bufp<T>(W.Fmi, oo);
+  T* Tp = bufp<T>(W.Tp, S.P.npo * S.P.npv);
+  T* Tm = bufp<T>(W.Tm, std::max(1L, S.P.nmo * S.P.nmv));
+  T* Lp = bufp<T>(W.Lp, S.P.npo * S.P.npv);
+  T* Lm = bufp<T>(W.Lm, std::max(1L, S.P.nmo * S.P.nmv));
+  const T one = T(1), zero = T(0), half = T(0.5);
+
+  // amplitudes in the compute type and the layouts the GEMMs read
+  permute<double, T>(ctx, t2, d4, {0, 1, 2, 3}, Tpp, 1.0, 0.0, "cc_prep");
+  permute<T, T>(ctx, Tpp, d4, {0, 1, 2, 3}, Upp, 2.0, 0.0, "cc_prep");
+  permute<T, T>(ctx, Tpp, d4, {1, 0, 2, 3}, Upp, -1.0, 1.0, "cc_prep");
+  permute<T, T>(ctx, Tpp, d4, {0, 2, 1, 3}, Tov, 1.0, 0.0, "cc_prep");    // T[i,a,j,b]
+  permute<T, T>(ctx, Tpp, d4, {0, 3, 1, 2}, Ttov, 1.0, 0.0, "cc_prep");   // t[i,j,b,a] at [i,a,j,b]
+  permute<T, T>(ctx, Upp, d4, {0, 2, 1, 3}, Uov, 1.0, 0.0, "cc_prep");
+  permute<T, T>(ctx, Upp, d4, {2, 0, 1, 3}, Ua, 1.0, 0.0, "cc_prep");     // U[a,m,n,f]
+  pack_t<T>(ctx, S.P, t2, o, v, Tp, Tm);
+
+  // Fae[a,e] = -sum U[a,(mnf)] K[e,(mnf)];  Fmi[m,i] = sum K[m,(nef)] U[i,(nef)]
+  gemm<T>(ctx, false, true, v, v, oo * v, T(-1), Ua, v, 0, I.Ka.template as<T>(), v, 0, zero, Fae, v, 0, 1, "cc_fock");
+  gemm<T>(ctx, false, true, o, o, ov * v, one, I.Kpp.template as<T>(), o, 0, Upp, o, 0, zero, Fmi, o, 0, 1, "cc_fock");
+  // hole-hole ladder: W' = oooo + K t^T ; Lhh = W'^T t
+  JUES_HIP(hipMemcpyAsync(Wp, I.oooo.p, oo * oo * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  gemm<T>(ctx, false, true, oo, oo, vv, one, I.Kpp.template as<T>(), oo, 0, Tpp, oo, 0, one, Wp, oo, 0, 1, "cc_hh");
+  gemm<T>(ctx, true, false, oo, vv, oo, one, Wp, oo, 0, Tpp, oo, 0, zero, Lhh, oo, 0, 1, "cc_hh");
+  // particle-particle ladder on the packed (anti)symmetric parts
+  gemm<T>(ctx, false, false, S.P.npo, S.P.npv, S.P.npv, one, Tp, S.P.npo, 0, I.Vp.template as<T>(), S.P.npv, 0, zero,
+          Lp, S.P.npo, 0, 1, "cc_pp_ladder");
+  if (S.P.nmo > 0 && S.P.nmv > 0)
+    gemm<T>(ctx, false, false, S.P.nmo, S.P.nmv, S.P.nmv, one, Tm, S.P.nmo, 0, I.Vm.template as<T>(), S.P.nmv, 0,
+            zero, Lm, S.P.nmo, 0, 1, "cc_pp_ladder");
+  // ring terms
+  const T* K = I.Kov.template as<T>();
+  const T* Kx = I.Ktov.template as<T>();
+  JUES_HIP(hipMemcpyAsync(WJ, K, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  gemm<T>(ctx, false, false, ov, ov, ov, half, K, ov, 0, Uov, ov, 0, one, WJ, ov, 0, 1, "cc_ring");
+  gemm<T>(ctx, false, false, ov, ov, ov, -half, Kx, ov, 0, Tov, ov, 0, one, WJ, ov, 0, 1, "cc_ring");
+  permute<T, T>(ctx, I.Jov.template as<T>(), {ov, ov}, {0, 1}, WE, -1.0, 0.0, "cc_prep");
+  gemm<T>(ctx, false, false, ov, ov, ov, half, Kx, ov, 0, Ttov, ov, 0, one, WE, ov, 0, 1, "cc_ring");
+  gemm<T>(ctx, false, false, ov, ov, ov, one, Tov, ov, 0, WE, ov, 0, zero, Y, ov, 0, 1, "cc_ring");
+  JUES_HIP(hipMemcpyAsync(H, Y, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  gemm<T>(ctx, false, false, ov, ov, ov, one, Uov, ov, 0, WJ, ov, 0, one, H, ov, 0, 1, "cc_ring");
+  // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
+  gemm<T>(ctx, false, false, ov, ov, ov, one, Ttov, ov, 0, WE, ov, 0, zero, Y, ov, 0, 1, "cc_ring");
+  // Fock-like terms: H += t[ia j, e] Fae[b,e] ; H -= Fmi[m,i] t[m, a jb]
+  gemm<T>(ctx, false, true, oo * v, v, v, one, Tov, oo * v, 0, Fae, v, 0, one, H, oo * v, 0, 1, "cc_fock");
+  gemm<T>(ctx, true, false, o, ov * v, o, T(-1), Fmi, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
+  // H += Y^x ; S = H + H^T (into WJ)
+  permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
+  JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  permute<T, T>(ctx, H, {ov, ov}, {1, 0}, WJ, 1.0, 1.0, "cc_sym");
+}
+
+template void rccd_contractions<double>(jues_ctx*, CCState&, const double*);
+template void rccd_contractions<float>(jues_ctx*, CCState&, const double*);
+
+// ---------------------------------------------------------------------------
+// iteration
+// ---------------------------------------------------------------------------
+static void diis_begin(CCState& S, DiisOrder& ord, int& slot) {
+  const int dmax = S.opts.diis ? S.opts.diis_max : 1;
+  if ((int)S.active.size() >= dmax) {
+    slot = S.active.front();
+    S.active.erase(S.active.begin());
+  } else {
+    slot = 0;
+    while (std::find(S.active.begin(), S.active.end(), slot) != S.active.end()) ++slot;
+  }
+  ord.nold = (int)S.active.size();
+  for (int k = 0; k < ord.nold; ++k) ord.old_slots[k] = S.active[k];
+  ord.newest = slot;
+}
+
+template <typename T>
+void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
+  const int o = S.o, v = S.v;
+  const long N2 = (long)o * o * v * v, Nd = S.Nd;
+  hipEvent_t e0 = S.ev[0], e1 = S.ev[1];
+  JUES_HIP(hipEventRecord(e0, ctx->stream));
+
+  DiisOrder ord{};
+  int slot = 0;
+  diis_begin(S, ord, slot);
+  double* vals = S.vals.as<double>();
+  double* errs = S.errs.as<double>();
+  double* t = S.t.as<double>();
+  double* tn = vals + (long)slot * Nd;
+  double* en = errs + (long)slot * Nd;
+
+  const long v2 = (long)v * v;
+  const int nb1 = S.method == 1 ? 64 : 0;
+  const long PB = v2 + nb1;
+  double* part = static_cast<double*>(scratch(ctx, 13, sizeof(double) * (size_t)(PB * (2 + MAX_DIIS))));
+
+  const T* R1extra = nullptr;
+  if (S.method == 0) {
+    rccd_contractions<T>(ctx, S, t);
+  } else {
+    rccsd_contractions<T>(ctx, S, t, t + N2);
+    R1extra = nullptr;
+  }
+  CCWork<T>& W = work<T>(S);
+  UpdateArgs<T> A{};
+  A.o = o;
+  A.v = v;
+  A.eps = S.eps.as<double>();
+  A.Kpp = S.i64->Kpp.as<double>();
+  A.Lener = S.Lener.as<double>();
+  A.S = W.WJ.template as<T>();
+  A.Lhh = W.Lhh.template as<T>();
+  A.Lp = W.Lp.template as<T>();
+  A.Lm = W.Lm.template as<T>();
+  A.extra = S.method == 1 ? W.X2.template as<T>() : nullptr;
+  A.t1n = nullptr;
+  A.npo = S.P.npo;
+  A.nmo = S.P.nmo;
+  A.t_old = t;
+  A.t_new = tn;
+  A.e_new = en;
+  A.nold = ord.nold;
+  for (int k = 0; k < ord.nold; ++k) A.e_old[k] = errs + (long)ord.old_slots[k] * Nd;
+  A.part = part;
+  A.part_stride = PB;
+  A.part_off = 0;
+  if (S.method == 1) {
+    // T1 first: the T2 kernel's energy term needs the new T1
+    T1UpdateArgs B{};
+    B.o = o;
+    B.v = v;
+    B.eps = S.eps.as<double>();
+    B.R1 = S.R1.as<double>();
+    B.Lener1 = nullptr;
+    B.t_old = t + N2;
+    B.t_new = tn + N2;
+    B.e_new = en + N2;
+    B.nold = ord.nold;
+    for (int k = 0; k < ord.nold; ++k) B.e_old[k] = errs + (long)ord.old_slots[k] * Nd + N2;
+    B.part = part;
+    B.part_stride = PB;
+    B.part_off = v2;
+    t1_update(ctx, B, nb1);
+    A.t1n = tn + N2;
+  }
+  t2_update<T>(ctx, A);
+  double* small = S.small.as<double>();
+  reduce_rows(ctx, part, PB, PB, 2 + ord.nold, small);
+  S.active.push_back(slot);
+  ord.m = (int)S.active.size();
+  for (int k = 0; k < ord.m; ++k) ord.slots[k] = S.active[k];
+  diis_solve(ctx, small, ord);
+  diis_extrap(ctx, vals, Nd, ord, small, Nd, t);
+  JUES_HIP(hipEventRecord(e1, ctx->stream));
+  double h[2];
+  d2h(ctx, h, small, sizeof(h));
+  float ms = 0.f;
+  JUES_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)R1extra;
+
+  S.iter += 1;
+  info->iter = S.iter;
+  info->stage = sizeof(T) == 4 ? 1 : 0;
+  info->energy = h[0];
+  info->delta_e = S.have_prev ? h[0] - S.E_prev : h[0];
+  info->rnorm = std::sqrt(std::max(0.0, h[1]));
+  info->diis_size = ord.m;
+  info->ms = ms;
+  S.E_prev = h[0];
+  S.have_prev = true;
+  if (!std::isfinite(h[0]) || !std::isfinite(h[1]))
+    throw Error(JUES_NONFINITE, "non-finite energy or residual at CC iteration " + std::to_string(S.iter));
+}
+
+template void cc_iteration<double>(jues_ctx*, CCState&, jues_iter_info*);
+template void cc_iteration<float>(jues_ctx*, CCState&, jues_iter_info*);
+
+// ---------------------------------------------------------------------------
+// setup / drivers
+// ---------------------------------------------------------------------------
+void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, const double* Cocc, int o,
+              const double* Cvir, int v, int n, const double* eps, const double* T1_init, const double* T2_init) {
+  JUES_REQUIRE(method == 0 || method == 1, "cc_setup: method must be 0 (RCCD) or 1 (RCCSD)");
+  JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED, "cc_setup: bad precision");
+  JUES_REQUIRE(Cocc && Cvir && eps && o > 0 && v > 0, "cc_setup: bad arguments");
+  JUES_REQUIRE(n == ctx->n && n > 0, "cc_setup: n does not match the AO integrals");
+  jues_cc_opts opts;
+  jues_cc_default_opts(&opts);
+  if (opts_in) opts = *opts_in;
+  JUES_REQUIRE(opts.diis_max >= 1 && opts.diis_max <= MAX_DIIS, "cc_setup: diis_max must be in 1..8");
+  JUES_REQUIRE(opts.fcn >= 0 && opts.fcn < o, "cc_setup: bad fcn");
+  JUES_REQUIRE(opts.guess >= 0 && opts.guess <= 3, "cc_setup: bad guess");
+  JUES_REQUIRE(opts.guess != JUES_GUESS_USER || T2_init, "cc_setup: guess=USER needs T2");
+  sync(ctx);
+  ctx->cc.reset();
+  ctx->cc.reset(new CCState());
+  CCState& S = *ctx->cc;
+  S.method = method;
+  S.prec = prec;
+  S.opts = opts;
+  const int fc = opts.fcn;
+  S.o = o - fc;
+  S.v = v;
+  S.n = n;
+  const int oo_ = S.o;
+  const long N2 = (long)oo_ * oo_ * v * v, N1 = (long)oo_ * v;
+  S.Nd = N2 + (method == 1 ? N1 : 0);
+  for (auto& e : S.ev) JUES_HIP(hipEventCreate(&e));
+  // orbital data (frozen core dropped)
+  DeviceArray<double> co(ctx, Cocc + (size_t)fc * n, (size_t)n * oo_), cv(ctx, Cvir, (size_t)n * v);
+  std::vector<double> e(eps + fc, eps + o + v);
+  S.eps.alloc(e.size() * sizeof(double));
+  h2d(ctx, S.eps.p, e.data(), e.size() * sizeof(double));
+  build_pairs(ctx, S);
+  S.i64.reset(new CCInts<double>());
+  cc_build_ints(ctx, S, co.p, cv.p);
+  if (method == 1) cc_build_ccsd_ints(ctx, S);
+  // amplitudes
+  S.t.alloc(S.Nd * sizeof(double));
+  double* t = S.t.as<double>();
+  fill<double>(ctx, t, S.Nd, 0.0);
+  switch (opts.guess) {
+    case JUES_GUESS_ZERO: break;
+    case JUES_GUESS_MP2:
+      JUES_HIP(hipMemcpyAsync(t, S.i64->Kpp.p, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      divide_d2(ctx, t, S.eps.as<double>(), S.o, S.v);
+      break;
+    case JUES_GUESS_OVOV:
+      JUES_HIP(hipMemcpyAsync(t, S.guess_buf.p, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      divide_d2(ctx, t, S.eps.as<double>(), S.o, S.v);
+      break;
+    case JUES_GUESS_USER:
+      h2d(ctx, t, T2_init, N2 * sizeof(double));
+      if (method == 1 && T1_init) h2d(ctx, t + N2, T1_init, N1 * sizeof(double));
+      break;
+  }
+  S.guess_buf.release();
+  const int dm = opts.diis ? opts.diis_max : 1;
+  S.vals.alloc((size_t)dm * S.Nd * sizeof(double));
+  S.errs.alloc((size_t)dm * S.Nd * sizeof(double));
+  S.small.alloc(SMALL_SIZE * sizeof(double));
+  fill<double>(ctx, S.small.as<double>(), SMALL_SIZE, 0.0);
+  if (prec == JUES_FP32_MIXED) cc_make_ints32(ctx, S);
+  S.E_prev = cc_energy(ctx, S);
+  S.have_prev = true;
+  S.iter = 0;
+  sync(ctx);
+}
+
+double cc_energy(jues_ctx* ctx, CCState& S) {
+  const long N2 = (long)S.o * S.o * S.v * S.v;
+  double E = dot<double, double>(ctx, S.Lener.as<double>(), S.t.as<double>(), N2);
+  if (S.method == 1) E += cc_energy_t1(ctx, S);
+  return E;
+}
+
+void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_conv, jues_iter_info* last,
+            bool* converged, jues_iter_cb cb, void* user) {
+  *converged = false;
+  if (stage_prec == JUES_FP32_MIXED) cc_make_ints32(ctx, S);
+  int stall = 0;
+  double best = 1e300;
+  for (int k = 0; k < nsteps; ++k) {
+    jues_iter_info info{};
+    if (stage_prec == JUES_FP32_MIXED) cc_iteration<float>(ctx, S, &info);
+    else cc_iteration<double>(ctx, S, &info);
+    if (cb) cb(user, &info);
+    if (S.opts.verbose)
+      std::fprintf(stderr, "  cc iter %3d %s E=% .15f dE=% .3e |dt|=%.3e diis=%d %.1f ms\n", info.iter,
+                   info.stage ? "fp32" : "fp64", info.energy, info.delta_e, info.rnorm, info.diis_size, info.ms);
+    *last = info;
+    if (stage_prec == JUES_FP64) {
+      if (std::fabs(info.delta_e) < S.opts.e_conv && info.rnorm < S.opts.r_conv) {
+        *converged = true;
+        if (stop_on_conv) break;
+      }
+    } else {
+      // FP32 stage: done when the step is below fp32_r_conv or stops shrinking
+      if (info.rnorm < best * 0.9) { best = info.rnorm; stall = 0; }
+      else ++stall;
+      if (info.rnorm < S.opts.fp32_r_conv || stall >= 4) {
+        *converged = true;
+        if (stop_on_conv) break;
+      }
+    }
+  }
+}
+
+void cc_reset_diis(CCState& S) { S.active.clear(); }
+
+}  // namespace jues
+
+using namespace jues;
+
+namespace {
+
+jues_status cc_solve(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts, const double* Cocc, int o,
+                     const double* Cvir, int v, int n, const double* eps, double* E, double* T1_inout,
+                     double* T2_inout, jues_iter_cb cb, void* user) {
+  JUES_API_BEGIN(ctx)
+  JUES_REQUIRE(E, "null energy pointer");
+  cc_setup(ctx, method, prec, opts, Cocc, o, Cvir, v, n, eps, T1_inout, T2_inout);
+  CCState& S = *ctx->cc;
+  jues_iter_info last{};
+  bool conv = false;
+  if (prec == JUES_FP64) {
+    cc_run(ctx, S, JUES_FP64, S.opts.maxit, true, &last, &conv, cb, user);
+  } else {
+    cc_run(ctx, S, JUES_FP32_MIXED, S.opts.maxit, true, &last, &conv, cb, user);
+    cc_reset_diis(S);
+    cc_run(ctx, S, JUES_FP64, S.opts.refine_maxit, true, &last, &conv, cb, user);
+  }
+  *E = cc_energy(ctx, S);
+  const long N2 = (long)S.o * S.o * S.v * S.v;
+  if (T2_inout) d2h(ctx, T2_inout, S.t.p, N2 * sizeof(double));
+  if (T1_inout && method == 1) d2h(ctx, T1_inout, S.t.as<double>() + N2, (size_t)S.o * S.v * sizeof(double));
+  if (!std::isfinite(*E)) throw Error(JUES_NONFINITE, "non-finite CC energy");
+  if (!conv) {
+    ctx->err = "CC equations did not converge in the iteration limit";
+    return JUES_NOT_CONVERGED;
+  }
+  JUES_API_END(ctx)
+}
+
+}  // namespace
+
+extern "C" {
+
+jues_status jues_rccd(jues_ctx* ctx, int prec, const jues_cc_opts* opts, const double* Cocc, int o, const double* Cvir,
+                      int v, int n, const double* eps, double* E, double* T2_inout, jues_iter_cb cb, void* user) {
+  return cc_solve(ctx, 0, prec, opts, Cocc, o, Cvir, v, n, eps, E, nullptr, T2_inout, cb, user);
+}
+
+jues_status jues_rccsd(jues_ctx* ctx, int prec, const jues_cc_opts* opts, const double* Cocc, int o,
+                       const double* Cvir, int v, int n, const double* eps, double* E, double* T1_inout,
+                       double* T2_inout, jues_iter_cb cb, void* user) {
+  return cc_solve(ctx, 1, prec, opts, Cocc, o, Cvir, v, n, eps, E, T1_inout, T2_inout, cb, user);
+}
+
+jues_status jues_cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts, const double* Cocc, int o,
+                          const double* Cvir, int v, int n, const double* eps, const double* T1_init,
+                          const double* T2_init) {
+  JUES_API_BEGIN(ctx)
+  cc_setup(ctx, method, prec, opts, Cocc, o, Cvir, v, n, eps, T1_init, T2_init);
+  JUES_API_END(ctx)
+}
+
+jues_status jues_cc_iterate(jues_ctx* ctx, int stage_prec, int nsteps, int stop_on_conv, jues_iter_info* info,
+                            int* converged) {
+  JUES_API_BEGIN(ctx)
+  JUES_REQUIRE(ctx->cc, "jues_cc_iterate: call jues_cc_setup first");
+  JUES_REQUIRE(stage_prec == JUES_FP64 || stage_prec == JUES_FP32_MIXED, "jues_cc_iterate: bad precision");
+  jues_iter_info last{};
+  bool conv = false;
+  cc_run(ctx, *ctx->cc, stage_prec, nsteps, stop_on_conv != 0, &last, &conv, nullptr, nullptr);
+  if (info) *info = last;
+  if (converged) *converged = conv ? 1 : 0;
+  JUES_API_END(ctx)
+}
+
+jues_status jues_cc_reset_diis(jues_ctx* ctx) {
+  JUES_API_BEGIN(ctx)
+  JUES_REQUIRE(ctx->cc, "no CC state");
+  cc_reset_diis(*ctx->cc);
+  JUES_API_END(ctx)
+}
+
+jues_status jues_cc_get(jues_ctx* ctx, double* E, double* T1_out, double* T2_out) {
+  JUES_API_BEGIN(ctx)
+  JUES_REQUIRE(ctx->cc, "no CC state");
+  CCState& S = *ctx->cc;
+  if (E) *E = cc_energy(ctx, S);
+  const long N2 = (long)S.o * S.o * S.v * S.v;
+  if (T2_out) d2h(ctx, T2_out, S.t.p, N2 * sizeof(double));
+  if (T1_out && S.method == 1) d2h(ctx, T1_out, S.t.as<double>() + N2, (size_t)S.o * S.v * sizeof(double));
+  JUES_API_END(ctx)
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// Declarations of the coupled-cluster bandwidth-bound kernels (cc_kernels.hip).
+#pragma once
+
+#include "jues_internal.h"
+
+namespace jues {
+
+constexpr int MAX_DIIS = 8;
+// layout of the small FP64 device array used by the update / DIIS kernels
+constexpr int SMALL_B = 16;                               // MAX_DIIS x MAX_DIIS Gram matrix, by slot
+constexpr int SMALL_C = SMALL_B + MAX_DIIS * MAX_DIIS;   // coefficients, by age order
+constexpr int SMALL_FLAG = SMALL_C + MAX_DIIS;          // 1 if the last solve was singular
+constexpr int SMALL_SIZE = SMALL_FLAG + 8;
+
+// (p,q) pair tables for the packed ladder: *_le lists p<=q, *_lt lists p<q, both
+// in column-major triangular order (index p + q(q+1)/2, resp. p + q(q-1)/2).
+struct PairTables {
+  int2 *o_le = nullptr, *o_lt = nullptr, *v_le = nullptr, *v_lt = nullptr;
+  long npo = 0, nmo = 0, npv = 0, nmv = 0;
+};
+
+template <typename TR>
+struct UpdateArgs {
+  int o, v;
+  const double* eps;
+  const double* Kpp;    // <ij|ab> (FP64, ijab)
+  const double* Lener;  // 2<ij|ab> - <ji|ab> (FP64, ijab)
+  const TR* S;          // ring + Fock-like terms, symmetrised, ov layout [i,a,j,b]
+  const TR* Lhh;        // hole-hole ladder (ijab)
+  const TR* Lp;         // packed pp-ladder, symmetric part
+  const TR* Lm;         // packed pp-ladder, antisymmetric part
+  const TR* extra;      // any further ijab-ordered residual terms (RCCSD), may be null
+  const double* t1n;    // new T1 (o x v) for the CCSD energy term, null for RCCD
+  long npo, nmo;
+  const double* t_old;
+  double* t_new;
+  double* e_new;
+  const double* e_old[MAX_DIIS];
+  int nold;
+  double* part;         // partial sums: part[q*part_stride + part_off + block]
+  long part_stride, part_off;
+};
+
+struct T1UpdateArgs {
+  int o, v;
+  const double* eps;
+  const double* R1;     // T1 residual numerator (FP64, o x v)
+  const double* Lener1; // may be null
+  const double* t_old;
+  double* t_new;
+  double* e_new;
+  const double* e_old[MAX_DIIS];
+  int nold;
+  double* part;
+  long part_stride, part_off;
+};
+
+struct DiisOrder {
+  int m;                    // number of active pairs
+  int slots[MAX_DIIS];      // oldest -> newest
+  int newest;               // slot just written
+  int nold;                 // number of other active slots
+  int old_slots[MAX_DIIS];  // their slots, in the order their dots were accumulated
+};
+
+template <typename T>
+void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm);
+template <typename Ts, typename Td>
+void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm);
+template <typename TR>
+void t2_update(jues_ctx* ctx, const UpdateArgs<TR>& A);
+void t1_update(jues_ctx* ctx, const T1UpdateArgs& A, int nblocks);
+void reduce_rows(jues_ctx* ctx, const double* part, long PB, long count, int nq, double* out);
+void diis_solve(jues_ctx* ctx, double* small, const DiisOrder& ord);
+void diis_extrap(jues_ctx* ctx, const double* vals, long stride, const DiisOrder& ord, const double* small, long n,
+                 double* t);
+void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v);
+
+}  // namespace jues

@@ -1,0 +1,347 @@
+// Bandwidth-bound kernels of the coupled-cluster iteration: symmetric packing of
+// the particle-particle ladder, the fused residual-assembly / Jacobi update /
+// energy / DIIS-dot kernel, the deterministic partial-sum reduction, the on-device
+// bordered DIIS solve and the extrapolation.
+#include "cc_kernels.h"
+
+namespace jues {
+namespace {
+
+// pair (p<=q) index p + q(q+1)/2 and strict pair (p<q) index p + q(q-1)/2
+__device__ inline long tri_le(int p, int q) { return (long)p + (long)q * (q + 1) / 2; }
+__device__ inline long tri_lt(int p, int q) { return (long)p + (long)q * (q - 1) / 2; }
+
+// Tp[(i<=j),(e<=f)] = t[ijef] + t[ijfe] (e<f), t[ijee] (e=f)
+// Tm[(i<j),(e<f)]   = t[ijef] - t[ijfe]
+template <typename T>
+__global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ t, int o, int v,
+                                                     const int2* __restrict__ pairs_o_le, const int2* __restrict__ pairs_v_le,
+                                                     long npo, long npv, T* __restrict__ Tp,
+                                                     const int2* __restrict__ pairs_o_lt, const int2* __restrict__ pairs_v_lt,
+                                                     long nmo, long nmv, T* __restrict__ Tm) {
+  const long oo = (long)o * o;
+  const long tot_p = npo * npv, tot_m = nmo * nmv;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < tot_p + tot_m; x += (long)gridDim.x * 256L) {
+    if (x < tot_p) {
+      const long r = x % npo, c = x / npo;
+      const int2 ij = pairs_o_le[r], ef = pairs_v_le[c];
+      const long base = ij.x + (long)o * ij.y;
+      const double a = t[base + oo * (ef.x + (long)v * ef.y)];
+      const double b = t[base + oo * (ef.y + (long)v * ef.x)];
+      Tp[x] = (T)(ef.x == ef.y ? a : a + b);
+    } else {
+      const long y = x - tot_p;
+      const long r = y % nmo, c = y / nmo;
+      const int2 ij = pairs_o_lt[r], ef = pairs_v_lt[c];
+      const long base = ij.x + (long)o * ij.y;
+      const double a = t[base + oo * (ef.x + (long)v * ef.y)];
+      const double b = t[base + oo * (ef.y + (long)v * ef.x)];
+      Tm[y] = (T)(a - b);
+    }
+  }
+}
+
+// From a chemist slab X[a,e,b,f'] (f' = f - f0, f in [f0,f1)) build
+// Vp[(e<=f),(a<=b)] = ((ae|bf) + (af|be))/2 and Vm[(e<f),(a<b)] = ((ae|bf) - (af|be))/2,
+// using (af|be) = X[b,e,a,f'].
+template <typename Ts, typename Td>
+__global__ __launch_bounds__(256) void pack_v_kernel(const Ts* __restrict__ X, int v, int f0, int f1,
+                                                     const int2* __restrict__ pairs_v_le, long npv, Td* __restrict__ Vp,
+                                                     const int2* __restrict__ pairs_v_lt, long nmv, Td* __restrict__ Vm) {
+  // rows: e<=f with f in [f0,f1): q from f0(f0+1)/2 to f1(f1+1)/2
+  const long r0p = (long)f0 * (f0 + 1) / 2, r1p = (long)f1 * (f1 + 1) / 2;
+  const long r0m = (long)f0 * (f0 - 1) / 2, r1m = (long)f1 * (f1 - 1) / 2;
+  const long nrp = r1p - r0p, nrm = r1m - r0m;
+  const long tot_p = nrp * npv, tot_m = nrm * nmv;
+  const long v1 = v, v2 = v1 * v, v3 = v2 * v;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < tot_p + tot_m; x += (long)gridDim.x * 256L) {
+    if (x < tot_p) {
+      const long rr = r0p + x % nrp, c = x / nrp;
+      const int2 ef = pairs_v_le[rr], ab = pairs_v_le[c];
+      const long fl = ef.y - f0;
+      const double p = (double)X[ab.x + v1 * ef.x + v2 * ab.y + v3 * fl];
+      const double q = (double)X[ab.y + v1 * ef.x + v2 * ab.x + v3 * fl];
+      Vp[rr + npv * c] = (Td)(0.5 * (p + q));
+    } else {
+      const long y = x - tot_p;
+      const long rr = r0m + y % nrm, c = y / nrm;
+      const int2 ef = pairs_v_lt[rr], ab = pairs_v_lt[c];
+      const long fl = ef.y - f0;
+      const double p = (double)X[ab.x + v1 * ef.x + v2 * ab.y + v3 * fl];
+      const double q = (double)X[ab.y + v1 * ef.x + v2 * ab.x + v3 * fl];
+      Vm[rr + nmv * c] = (Td)(0.5 * (p - q));
+    }
+  }
+}
+
+__device__ inline double block_sum256(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// One block per (a,b): assemble R[ijab] = K[ijab] + S[ia,jb] + Lhh[ijab] + L+ + L-,
+// t_new = R/D, e = t_new - t_old; write t_new and e into the DIIS slot; block
+// partial sums of E = sum Lener*t_new, e.e and e.e_k for the stored errors.
+template <typename TR>
+__global__ __launch_bounds__(256) void t2_update_kernel(UpdateArgs<TR> A) {
+  __shared__ double sh[4];
+  const int o = A.o, v = A.v;
+  const int ab = blockIdx.x;
+  const int a = ab % v, b = ab / v;
+  const long oo = (long)o * o, ov = (long)o * v;
+  const double dab = -A.eps[o + a] - A.eps[o + b];
+  const int sab = a < b ? 1 : (a > b ? -1 : 0);
+  const long qp = a <= b ? (long)a + (long)b * (b + 1) / 2 : (long)b + (long)a * (a + 1) / 2;
+  const long qm = a < b ? (long)a + (long)b * (b - 1) / 2 : (long)b + (long)a * (a - 1) / 2;
+  double sE = 0.0, sEE = 0.0;
+  double sd[MAX_DIIS];
+#pragma unroll
+  for (int k = 0; k < MAX_DIIS; ++k) sd[k] = 0.0;
+  for (int ij = threadIdx.x; ij < (int)oo; ij += 256) {
+    const int i = ij % o, j = ij / o;
+    const long x = (long)ij + oo * ab;
+    double R = A.Kpp[x];
+    if (A.S) R += (double)A.S[(long)i + (long)o * a + ov * ((long)j + (long)o * b)];
+    if (A.Lhh) R += (double)A.Lhh[x];
+    if (A.Lp) {
+      const long pp = i <= j ? (long)i + (long)j * (j + 1) / 2 : (long)j + (long)i * (i + 1) / 2;
+      R += (double)A.Lp[pp + A.npo * qp];
+      const int sij = i < j ? 1 : (i > j ? -1 : 0);
+      if (sij != 0 && sab != 0) {
+        const long pm = i < j ? (long)i + (long)j * (j - 1) / 2 : (long)j + (long)i * (i - 1) / 2;
+        R += (double)(sij * sab) * (double)A.Lm[pm + A.nmo * qm];
+      }
+    }
+    if (A.extra) R += (double)A.extra[x];
+    const double tn = R / (A.eps[i] + A.eps[j] + dab);
+    const double e = tn - A.t_old[x];
+    A.t_new[x] = tn;
+    A.e_new[x] = e;
+    const double tt = A.t1n ? A.t1n[i + (long)o * a] * A.t1n[j + (long)o * b] : 0.0;
+    sE += A.Lener[x] * (tn + tt);
+    sEE += e * e;
+#pragma unroll
+    for (int k = 0; k < MAX_DIIS; ++k)
+      if (k < A.nold) sd[k] += e * A.e_old[k][x];
+  }
+  const long PB = A.part_stride;
+  double r = block_sum256(sE, sh);
+  if (threadIdx.x == 0) A.part[0 * PB + A.part_off + blockIdx.x] = r;
+  r = block_sum256(sEE, sh);
+  if (threadIdx.x == 0) A.part[1 * PB + A.part_off + blockIdx.x] = r;
+  for (int k = 0; k < A.nold; ++k) {
+    r = block_sum256(sd[k], sh);
+    if (threadIdx.x == 0) A.part[(2 + k) * PB + A.part_off + blockIdx.x] = r;
+  }
+}
+
+// T1 part of the update: t1_new = R1/D_ia, error and partials (RCCSD).
+__global__ __launch_bounds__(256) void t1_update_kernel(T1UpdateArgs A) {
+  __shared__ double sh[4];
+  const long n1 = (long)A.o * A.v;
+  double sE = 0.0, sEE = 0.0, sd[MAX_DIIS];
+#pragma unroll
+  for (int k = 0; k < MAX_DIIS; ++k) sd[k] = 0.0;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < n1; x += (long)gridDim.x * 256L) {
+    const int i = (int)(x % A.o), a = (int)(x / A.o);
+    const double tn = A.R1[x] / (A.eps[i] - A.eps[A.o + a]);
+    const double e = tn - A.t_old[x];
+    A.t_new[x] = tn;
+    A.e_new[x] = e;
+    sE += A.Lener1 ? A.Lener1[x] * tn : 0.0;
+    sEE += e * e;
+#pragma unroll
+    for (int k = 0; k < MAX_DIIS; ++k)
+      if (k < A.nold) sd[k] += e * A.e_old[k][x];
+  }
+  const long PB = A.part_stride;
+  double r = block_sum256(sE, sh);
+  if (threadIdx.x == 0) A.part[0 * PB + A.part_off + blockIdx.x] = r;
+  r = block_sum256(sEE, sh);
+  if (threadIdx.x == 0) A.part[1 * PB + A.part_off + blockIdx.x] = r;
+  for (int k = 0; k < A.nold; ++k) {
+    r = block_sum256(sd[k], sh);
+    if (threadIdx.x == 0) A.part[(2 + k) * PB + A.part_off + blockIdx.x] = r;
+  }
+}
+
+// Fixed-order sums of nq partial arrays; out[q] = sum_b part[q*PB + b].
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const double* __restrict__ part, long PB, long count,
+                                                          double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int q = blockIdx.x;
+  double s = 0.0;
+  for (long b = threadIdx.x; b < count; b += 256) s += part[q * PB + b];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = sh[0];
+}
+
+// Store the new B row/column and solve the bordered Pulay system (mRCCD.jl:185-198)
+// in FP64 with partial pivoting.  small layout: [0]=E, [1]=ee, [2..2+nold)=dots,
+// B at SMALL_B (MAX_DIIS^2, by slot), coefficients at SMALL_C (MAX_DIIS, by order).
+__global__ void diis_solve_kernel(double* small, DiisOrder ord) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* B = small + SMALL_B;
+  const int s = ord.newest;
+  B[s * MAX_DIIS + s] = small[1];
+  for (int k = 0; k < ord.nold; ++k) {
+    const int sk = ord.old_slots[k];
+    B[s * MAX_DIIS + sk] = small[2 + k];
+    B[sk * MAX_DIIS + s] = small[2 + k];
+  }
+  const int m = ord.m;
+  double* c = small + SMALL_C;
+  if (m == 1) { c[0] = 1.0; small[SMALL_FLAG] = 0.0; return; }
+  double M[(MAX_DIIS + 1) * (MAX_DIIS + 1)];
+  double r[MAX_DIIS + 1];
+  const int N = m + 1;
+  double mx = 0.0;
+  for (int p = 0; p < m; ++p)
+    for (int q = 0; q < m; ++q) mx = fmax(mx, fabs(B[ord.slots[p] * MAX_DIIS + ord.slots[q]]));
+  if (!(mx > 0.0)) mx = 1.0;
+  for (int p = 0; p < N; ++p) {
+    for (int q = 0; q < N; ++q) {
+      double val;
+      if (p < m && q < m) val = B[ord.slots[p] * MAX_DIIS + ord.slots[q]] / mx;
+      else if (p == m && q == m) val = 0.0;
+      else val = -1.0;
+      M[p * N + q] = val;
+    }
+    r[p] = p == m ? -1.0 : 0.0;
+  }
+  bool ok = true;
+  for (int k = 0; k < N; ++k) {
+    int piv = k;
+    for (int p = k + 1; p < N; ++p)
+      if (fabs(M[p * N + k]) > fabs(M[piv * N + k])) piv = p;
+    if (!(fabs(M[piv * N + k]) > 1e-300)) { ok = false; break; }
+    if (piv != k) {
+      for (int q = 0; q < N; ++q) { const double tmp = M[k * N + q]; M[k * N + q] = M[piv * N + q]; M[piv * N + q] = tmp; }
+      const double tmp = r[k]; r[k] = r[piv]; r[piv] = tmp;
+    }
+    for (int p = k + 1; p < N; ++p) {
+      const double f = M[p * N + k] / M[k * N + k];
+      for (int q = k; q < N; ++q) M[p * N + q] -= f * M[k * N + q];
+      r[p] -= f * r[k];
+    }
+  }
+  if (ok) {
+    for (int k = N - 1; k >= 0; --k) {
+      double acc = r[k];
+      for (int q = k + 1; q < N; ++q) acc -= M[k * N + q] * r[q];
+      r[k] = acc / M[k * N + k];
+    }
+    for (int p = 0; p < m; ++p) ok = ok && isfinite(r[p]);
+  }
+  if (ok) {
+    for (int p = 0; p < m; ++p) c[p] = r[p];
+    small[SMALL_FLAG] = 0.0;
+  } else {  // singular subspace: take the newest vector (a plain Jacobi step)
+    for (int p = 0; p < m; ++p) c[p] = p == m - 1 ? 1.0 : 0.0;
+    small[SMALL_FLAG] = 1.0;
+  }
+}
+
+// t[x] = sum_k c_k vals[slot_k][x]
+__global__ __launch_bounds__(256) void diis_extrap_kernel(const double* __restrict__ vals, long stride, DiisOrder ord,
+                                                          const double* __restrict__ small, long n, double* __restrict__ t) {
+  double c[MAX_DIIS];
+#pragma unroll
+  for (int k = 0; k < MAX_DIIS; ++k) c[k] = k < ord.m ? small[SMALL_C + k] : 0.0;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < n; x += (long)gridDim.x * 256L) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAX_DIIS; ++k)
+      if (k < ord.m) s += c[k] * vals[(long)ord.slots[k] * stride + x];
+    t[x] = s;
+  }
+}
+
+// t[ijab] /= D[ijab]
+__global__ __launch_bounds__(256) void divide_d2_kernel(double* __restrict__ t, const double* __restrict__ eps, int o,
+                                                        int v) {
+  const long oo = (long)o * o, n = oo * v * v;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < n; x += (long)gridDim.x * 256L) {
+    const long ij = x % oo, ab = x / oo;
+    const int i = (int)(ij % o), j = (int)(ij / o), a = (int)(ab % v), b = (int)(ab / v);
+    t[x] /= eps[i] + eps[j] - eps[o + a] - eps[o + b];
+  }
+}
+
+int grid256(long n) { return (int)std::max(1L, std::min((n + 255) / 256, 131072L)); }
+
+}  // namespace
+
+template <typename T>
+void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm) {
+  const long tot = P.npo * P.npv + P.nmo * P.nmv;
+  KTimer k(ctx, "cc_pack_t", 0.0, (double)tot * (16.0 + sizeof(T)));
+  hipLaunchKernelGGL((pack_t_kernel<T>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, t, o, v, P.o_le, P.v_le,
+                     P.npo, P.npv, Tp, P.o_lt, P.v_lt, P.nmo, P.nmv, Tm);
+  JUES_HIP(hipGetLastError());
+}
+
+template <typename Ts, typename Td>
+void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm) {
+  const long nrp = (long)f1 * (f1 + 1) / 2 - (long)f0 * (f0 + 1) / 2;
+  const long nrm = (long)f1 * (f1 - 1) / 2 - (long)f0 * (f0 - 1) / 2;
+  const long tot = nrp * P.npv + nrm * P.nmv;
+  KTimer k(ctx, "cc_pack_v", 0.0, (double)tot * (2 * sizeof(Ts) + sizeof(Td)));
+  hipLaunchKernelGGL((pack_v_kernel<Ts, Td>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, X, v, f0, f1, P.v_le,
+                     P.npv, Vp, P.v_lt, P.nmv, Vm);
+  JUES_HIP(hipGetLastError());
+}
+
+template <typename TR>
+void t2_update(jues_ctx* ctx, const UpdateArgs<TR>& A) {
+  const long v2 = (long)A.v * A.v;
+  const long n2 = (long)A.o * A.o * v2;
+  KTimer k(ctx, "cc_update", 0.0, (double)n2 * (8.0 * (5 + A.nold) + 4 * sizeof(TR)));
+  hipLaunchKernelGGL((t2_update_kernel<TR>), dim3((unsigned)v2), dim3(256), 0, ctx->stream, A);
+  JUES_HIP(hipGetLastError());
+}
+
+void t1_update(jues_ctx* ctx, const T1UpdateArgs& A, int nblocks) {
+  hipLaunchKernelGGL(t1_update_kernel, dim3(nblocks), dim3(256), 0, ctx->stream, A);
+  JUES_HIP(hipGetLastError());
+}
+
+void reduce_rows(jues_ctx* ctx, const double* part, long PB, long count, int nq, double* out) {
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(nq), dim3(256), 0, ctx->stream, part, PB, count, out);
+  JUES_HIP(hipGetLastError());
+}
+
+void diis_solve(jues_ctx* ctx, double* small, const DiisOrder& ord) {
+  hipLaunchKernelGGL(diis_solve_kernel, dim3(1), dim3(64), 0, ctx->stream, small, ord);
+  JUES_HIP(hipGetLastError());
+}
+
+void diis_extrap(jues_ctx* ctx, const double* vals, long stride, const DiisOrder& ord, const double* small, long n,
+                 double* t) {
+  KTimer k(ctx, "cc_diis_extrap", 0.0, (double)n * 8.0 * (ord.m + 1));
+  hipLaunchKernelGGL(diis_extrap_kernel, dim3(grid256(n)), dim3(256), 0, ctx->stream, vals, stride, ord, small, n, t);
+  JUES_HIP(hipGetLastError());
+}
+
+void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v) {
+  const long n = (long)o * o * v * v;
+  hipLaunchKernelGGL(divide_d2_kernel, dim3(grid256(n)), dim3(256), 0, ctx->stream, t, eps, o, v);
+  JUES_HIP(hipGetLastError());
+}
+
+template void pack_t<double>(jues_ctx*, const PairTables&, const double*, int, int, double*, double*);
+template void pack_t<float>(jues_ctx*, const PairTables&, const double*, int, int, float*, float*);
+template void pack_v<double, double>(jues_ctx*, const PairTables&, const double*, int, int, int, double*, double*);
+template void pack_v<double, float>(jues_ctx*, const PairTables&, const double*, int, int, int, float*, float*);
+template void t2_update<double>(jues_ctx*, const UpdateArgs<double>&);
+template void t2_update<float>(jues_ctx*, const UpdateArgs<float>&);
+
+}  // namespace jues
