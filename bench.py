@@ -1,0 +1,134 @@
+"""RCCD iteration benchmark on one MI355X: FP32-mixed vs plain FP64.
+
+Workload: closed-shell RCCD at o=60, v=340 (n=400, naux=1200) on seeded
+synthetic B-factor integrals; every MO integral block (including the packed
+v^4 ladder block) and the amplitudes are resident in HBM before timing.  A
+"step" is one full CC iteration (all contractions, the fused update, DIIS).
+The FP32-mixed stage (FP32 MFMA contractions, FP64 update/energy/DIIS) is timed
+first, then the DIIS history is reset and the plain FP64 stage is timed from the
+same amplitudes.  `value` is the FP32-mixed seconds per iteration.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg4_rccd_n400] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "jues.jl_amd"))
+
+METRIC = "RCCD s/iter + MFMA TFLOP/s, FP32-mixed vs FP64, |ΔE|≤1e-8 Eh, o=60 v=340"
+PEAK = {"f64": 78.6, "f32": 157.3}  # dense MFMA TFLOP/s (MI355X datasheet)
+
+
+def f_rccd(o: int, v: int) -> float:
+    """Canonical RCCD flops per iteration (pp-ladder + ring + hh + Fock-like terms)."""
+    return 2 * o**2 * v**4 + 12 * o**3 * v**3 + 4 * o**4 * v**2 + 4 * o**2 * v**3 + 4 * o**3 * v**2
+
+
+def roofline(stats: dict, prec: str) -> dict:
+    """Roofline of the dominant kernel class (largest total device time)."""
+    gemms = {k: s for k, s in stats.items() if s["flops"] > 0}
+    name, s = max(gemms.items(), key=lambda kv: kv[1]["ms"])
+    per_launch_flops = s["flops"] / s["calls"]
+    avg_s = s["ms"] / s["calls"] / 1e3
+    achieved = per_launch_flops / avg_s / 1e12
+    return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[prec],
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK[prec], 4), "traffic": None,
+            "flops_per_launch": per_launch_flops, "avg_launch_ms": round(avg_s * 1e3, 3),
+            "launches": int(s["calls"])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg4_rccd_n400")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-k", type=int, default=32)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from jues import synthetic, _lib
+    from jues.CoupledCluster import Solver
+    from jues.Wavefunction import Wfn
+
+    s = synthetic.workload(args.workload)
+    o, v = s.o, s.v
+    w = Wfn.from_synthetic(s, dense=False)
+    t0 = time.perf_counter()
+    solver = Solver(w, method="rccd", prec="fp32_mixed", guess="mp2", diis=True, diis_max=5)
+    t_setup = time.perf_counter() - t0
+    ctx = solver.ctx
+    print(f"[bench] setup {t_setup:.1f} s, device bytes {ctx.lib.jues_device_bytes(ctx.h) / 2**30:.1f} GiB",
+          file=sys.stderr, flush=True)
+
+    res = {}
+    for stage, prec in (("fp32", "fp32_mixed"), ("fp64", "fp64")):
+        if stage == "fp64":
+            solver.reset_diis()
+        if args.warmup:
+            solver.iterate(args.warmup, prec=prec)
+        ctx.stats_reset(True)
+        t0 = time.perf_counter()
+        info, _ = solver.iterate(args.steps, prec=prec)   # synchronous: returns after the stream drains
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.stats_reset(False)
+        res[stage] = dict(s_per_iter=dt / args.steps, info=info, stats=st)
+        print(f"[bench] {stage}: {dt / args.steps:.3f} s/iter, E={info['energy']:.10f} "
+              f"|dt|={info['rnorm']:.3e}", file=sys.stderr, flush=True)
+
+    F = f_rccd(o, v)
+    m, d = res["fp32"], res["fp64"]
+    executed = lambda st: sum(x["flops"] for x in st.values())  # noqa: E731
+    out = {
+        "metric": METRIC,
+        "value": round(m["s_per_iter"], 4),
+        "unit": "s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(m["s_per_iter"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded B-factor ERIs, QR orbitals, sorted uniform orbital energies)",
+        "config": {"workload": f"RCCD o={o} v={v} n={s.n} naux={s.naux}, B-factor integrals, "
+                               "vvvv packed and resident in HBM",
+                   "o": o, "v": v, "n": s.n, "naux": s.naux, "diis_max": 5},
+        "fp64_s_per_iter": round(d["s_per_iter"], 4),
+        "speedup_mixed_vs_fp64": round(d["s_per_iter"] / m["s_per_iter"], 3),
+        "tflops_canonical": {"fp32_mixed": round(F / m["s_per_iter"] / 1e12, 2),
+                             "fp64": round(F / d["s_per_iter"] / 1e12, 2), "flops_per_iter": F},
+        "tflops_executed": {"fp32_mixed": round(executed(m["stats"]) / args.steps / m["s_per_iter"] / 1e12, 2),
+                            "fp64": round(executed(d["stats"]) / args.steps / d["s_per_iter"] / 1e12, 2),
+                            "flops_per_iter": executed(m["stats"]) / args.steps},
+        "roofline": roofline(m["stats"], "f32"),
+        "roofline_fp64": roofline(d["stats"], "f64"),
+        "kernel_ms_per_iter": {"fp32": {k: round(x["ms"] / args.steps, 2) for k, x in m["stats"].items()},
+                               "fp64": {k: round(x["ms"] / args.steps, 2) for k, x in d["stats"].items()}},
+        "setup_s": round(t_setup, 2),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import cpu_baseline
+        cb = cpu_baseline.rccd_iteration_sample(o, v, k=args.cpu_k)
+        out["cpu_baseline"] = {"value": round(cb["seconds_per_iter"], 2), "unit": "s/iter",
+                               "cores": cb["threads"], "kind": "port", "sample": cb["sample"],
+                               "host_cpus": os.cpu_count()}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -75,6 +75,14 @@ typedef struct {
     double fp32_r_conv;       /* default 1e-5 */
     int32_t refine_maxit;     /* default 60 */
     int32_t verbose;          /* print one line per iteration to stderr */
+    /* Coupling continuation.  Every residual term carries exactly one integral, so the
+     * equations for integrals scaled by lambda are t = lambda R(t) / D.  With
+     * continuation = N > 1 the solver converges lambda = c/N, 2c/N, ..., c (c = coupling)
+     * in turn, each warm-started from the previous amplitudes with a fresh DIIS history;
+     * the result is the lambda = c solution on the branch connected to lambda -> 0.
+     * Energies are those of the lambda-scaled integrals. */
+    double coupling;          /* c, default 1.0 */
+    int32_t continuation;     /* N, default 1 (no continuation) */
 } jues_cc_opts;
 
 /* Per-iteration record passed to the optional callback. */
@@ -86,6 +94,7 @@ typedef struct {
     double rnorm;             /* ||t_new - t_old||_2 (T1 and T2 together for CCSD) */
     int32_t diis_size;        /* number of DIIS pairs used this iteration */
     double ms;                /* device time of the iteration (HIP events on the ctx stream) */
+    double coupling;          /* lambda of the equations iterated (1 unless continuation is used) */
 } jues_iter_info;
 
 typedef void (*jues_iter_cb)(void* user, const jues_iter_info* info);

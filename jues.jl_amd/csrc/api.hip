@@ -246,6 +246,8 @@ void jues_cc_default_opts(jues_cc_opts* o) {
   o->fp32_r_conv = 1e-5;
   o->refine_maxit = 60;
   o->verbose = 0;
+  o->coupling = 1.0;
+  o->continuation = 1;
 }
 
 }  // extern "C"

@@ -50,6 +50,7 @@ struct CCState {
   std::unique_ptr<CCWork<double>> w64;
   std::unique_ptr<CCWork<float>> w32;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  double lambda = 1.0;  // coupling of the equations being iterated
   double E_prev = 0;
   bool have_prev = false;
   int iter = 0;

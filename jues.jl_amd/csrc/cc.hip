@@ -326,6 +326,7 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
   A.Lm = W.Lm.template as<T>();
   A.extra = S.method == 1 ? W.X2.template as<T>() : nullptr;
   A.t1n = nullptr;
+  A.lambda = S.lambda;
   A.npo = S.P.npo;
   A.nmo = S.P.nmo;
   A.t_old = t;
@@ -344,6 +345,7 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
     B.eps = S.eps.as<double>();
     B.R1 = S.R1.as<double>();
     B.Lener1 = nullptr;
+    B.lambda = S.lambda;
     B.t_old = t + N2;
     B.t_new = tn + N2;
     B.e_new = en + N2;
@@ -378,6 +380,7 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
   info->rnorm = std::sqrt(std::max(0.0, h[1]));
   info->diis_size = ord.m;
   info->ms = ms;
+  info->coupling = S.lambda;
   S.E_prev = h[0];
   S.have_prev = true;
   if (!std::isfinite(h[0]) || !std::isfinite(h[1]))
@@ -403,6 +406,7 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   JUES_REQUIRE(opts.fcn >= 0 && opts.fcn < o, "cc_setup: bad fcn");
   JUES_REQUIRE(opts.guess >= 0 && opts.guess <= 3, "cc_setup: bad guess");
   JUES_REQUIRE(opts.guess != JUES_GUESS_USER || T2_init, "cc_setup: guess=USER needs T2");
+  JUES_REQUIRE(opts.continuation >= 1 && std::isfinite(opts.coupling), "cc_setup: bad continuation/coupling");
   sync(ctx);
   ctx->cc.reset();
   ctx->cc.reset(new CCState());
@@ -410,6 +414,7 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   S.method = method;
   S.prec = prec;
   S.opts = opts;
+  S.lambda = opts.coupling / opts.continuation;
   const int fc = opts.fcn;
   S.o = o - fc;
   S.v = v;
@@ -463,7 +468,7 @@ double cc_energy(jues_ctx* ctx, CCState& S) {
   const long N2 = (long)S.o * S.o * S.v * S.v;
   double E = dot<double, double>(ctx, S.Lener.as<double>(), S.t.as<double>(), N2);
   if (S.method == 1) E += cc_energy_t1(ctx, S);
-  return E;
+  return S.lambda * E;
 }
 
 void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_conv, jues_iter_info* last,
@@ -515,12 +520,28 @@ jues_status cc_solve(jues_ctx* ctx, int method, int prec, const jues_cc_opts* op
   CCState& S = *ctx->cc;
   jues_iter_info last{};
   bool conv = false;
-  if (prec == JUES_FP64) {
-    cc_run(ctx, S, JUES_FP64, S.opts.maxit, true, &last, &conv, cb, user);
-  } else {
-    cc_run(ctx, S, JUES_FP32_MIXED, S.opts.maxit, true, &last, &conv, cb, user);
-    cc_reset_diis(S);
-    cc_run(ctx, S, JUES_FP64, S.opts.refine_maxit, true, &last, &conv, cb, user);
+  const int nl = S.opts.continuation;
+  for (int k = 1; k <= nl; ++k) {
+    if (k > 1) {
+      // next coupling, warm-started from the previous solution
+      S.lambda = S.opts.coupling * k / nl;
+      cc_reset_diis(S);
+      S.E_prev = cc_energy(ctx, S);
+    }
+    if (prec == JUES_FP64) {
+      cc_run(ctx, S, JUES_FP64, S.opts.maxit, true, &last, &conv, cb, user);
+    } else {
+      cc_run(ctx, S, JUES_FP32_MIXED, S.opts.maxit, true, &last, &conv, cb, user);
+      if (k == nl) {
+        cc_reset_diis(S);
+        cc_run(ctx, S, JUES_FP64, S.opts.refine_maxit, true, &last, &conv, cb, user);
+      }
+    }
+    if (!conv && k < nl) {
+      ctx->err = "CC continuation did not converge at coupling " + std::to_string(S.lambda);
+      *E = cc_energy(ctx, S);
+      return JUES_NOT_CONVERGED;
+    }
   }
   *E = cc_energy(ctx, S);
   const long N2 = (long)S.o * S.o * S.v * S.v;

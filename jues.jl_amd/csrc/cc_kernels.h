@@ -31,6 +31,7 @@ struct UpdateArgs {
   const TR* Lm;         // packed pp-ladder, antisymmetric part
   const TR* extra;      // any further ijab-ordered residual terms (RCCSD), may be null
   const double* t1n;    // new T1 (o x v) for the CCSD energy term, null for RCCD
+  double lambda;        // coupling: t_new = lambda R / D, E = lambda sum Lener tau
   long npo, nmo;
   const double* t_old;
   double* t_new;
@@ -43,6 +44,7 @@ struct UpdateArgs {
 
 struct T1UpdateArgs {
   int o, v;
+  double lambda;
   const double* eps;
   const double* R1;     // T1 residual numerator (FP64, o x v)
   const double* Lener1; // may be null

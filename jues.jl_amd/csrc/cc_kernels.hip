@@ -116,12 +116,12 @@ __global__ __launch_bounds__(256) void t2_update_kernel(UpdateArgs<TR> A) {
       }
     }
     if (A.extra) R += (double)A.extra[x];
-    const double tn = R / (A.eps[i] + A.eps[j] + dab);
+    const double tn = A.lambda * R / (A.eps[i] + A.eps[j] + dab);
     const double e = tn - A.t_old[x];
     A.t_new[x] = tn;
     A.e_new[x] = e;
     const double tt = A.t1n ? A.t1n[i + (long)o * a] * A.t1n[j + (long)o * b] : 0.0;
-    sE += A.Lener[x] * (tn + tt);
+    sE += A.lambda * A.Lener[x] * (tn + tt);
     sEE += e * e;
 #pragma unroll
     for (int k = 0; k < MAX_DIIS; ++k)
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void t1_update_kernel(T1UpdateArgs A) {
   for (int k = 0; k < MAX_DIIS; ++k) sd[k] = 0.0;
   for (long x = blockIdx.x * 256L + threadIdx.x; x < n1; x += (long)gridDim.x * 256L) {
     const int i = (int)(x % A.o), a = (int)(x / A.o);
-    const double tn = A.R1[x] / (A.eps[i] - A.eps[A.o + a]);
+    const double tn = A.lambda * A.R1[x] / (A.eps[i] - A.eps[A.o + a]);
     const double e = tn - A.t_old[x];
     A.t_new[x] = tn;
     A.e_new[x] = e;

@@ -14,6 +14,9 @@
 // tiles are walked in groups of 8 tile-rows to reuse B columns within an L2.
 #include "jues_internal.h"
 
+#include <algorithm>
+#include <cstring>
+
 namespace jues {
 namespace {
 
@@ -60,7 +63,9 @@ __device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, i
   tn = r / gsz;
 }
 
-template <typename T, bool TA, bool TB>
+// KIND only separates kernel symbols (0 generic, 1 CC ring, 2 CC pp-ladder) so that
+// profiles attribute time to the coupled-cluster hot spots.
+template <typename T, bool TA, bool TB, int KIND>
 __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alpha,
                                                       const T* __restrict__ A, long lda, long sA,
                                                       const T* __restrict__ B, long ldb, long sB,
@@ -180,18 +185,34 @@ __global__ void scale_kernel(T* C, long M, long N, long ldc, T beta) {
   }
 }
 
+// C = beta*C + sum_s W[s] (fixed order), W[s] is M x N with leading dimension M.
+template <typename T>
+__global__ void splitk_reduce_kernel(const T* __restrict__ W, int S, long M, long N, T* __restrict__ C, long ldc,
+                                     T beta) {
+  const long total = M * N;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int q = 0; q < S; ++q) acc += (double)W[(long)q * total + e];
+    const long m = e % M, n = e / M;
+    const long off = m + n * ldc;
+    if (beta != T(0)) acc += (double)beta * (double)C[off];
+    C[off] = (T)acc;
+  }
+}
+
 }  // namespace
 
 template <typename T>
-void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
-          long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch,
-          const char* tag) {
+static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
+                      long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch,
+                      const char* tag, bool allow_split) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   JUES_REQUIRE(M < (1L << 31) && N < (1L << 31) && K < (1L << 31), "gemm: dimension too large");
   JUES_REQUIRE(ldc >= M, "gemm: ldc < M");
   JUES_REQUIRE(lda >= (ta ? K : M) || K == 0, "gemm: lda too small");
   JUES_REQUIRE(ldb >= (tb ? N : K) || K == 0, "gemm: ldb too small");
   const char* name = tag ? tag : (sizeof(T) == 8 ? "gemm_f64" : "gemm_f32");
+  const int kind = !tag ? 0 : (std::strcmp(tag, "cc_ring") == 0 ? 1 : (std::strcmp(tag, "cc_pp_ladder") == 0 ? 2 : 0));
   if (K <= 0) {
     KTimer t(ctx, name, 0, 0);
     for (int b = 0; b < batch; ++b)
@@ -202,6 +223,28 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
   }
   const long tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   JUES_REQUIRE(tiles_m * tiles_n < (1L << 31), "gemm: too many tiles");
+  // Few output tiles and a long K (the Fock-like terms): split K over workgroups,
+  // write partial products to a workspace and reduce them in a fixed order.
+  if (allow_split && batch == 1 && tiles_m * tiles_n < 128 && K >= 4096) {
+    const long want = std::max(2L, std::min(K / 1024, 512L / (tiles_m * tiles_n)));
+    const int BKs = sizeof(T) == 8 ? 16 : 32;
+    long kc = (K + want - 1) / want;
+    kc = (kc + BKs - 1) / BKs * BKs;
+    const int S = (int)((K + kc - 1) / kc);
+    T* Wk = static_cast<T*>(scratch(ctx, 9, sizeof(T) * (size_t)S * M * N));
+    const long sa = ta ? kc : kc * lda, sb = tb ? kc * ldb : kc;
+    if (S > 1)
+      gemm_impl<T>(ctx, ta, tb, M, N, kc, alpha, A, lda, sa, B, ldb, sb, T(0), Wk, M, M * N, S - 1, tag, false);
+    const long klast = K - (long)(S - 1) * kc;
+    gemm_impl<T>(ctx, ta, tb, M, N, klast, alpha, A + (long)(S - 1) * sa, lda, 0, B + (long)(S - 1) * sb, ldb, 0,
+                 T(0), Wk + (long)(S - 1) * M * N, M, 0, 1, tag, false);
+    KTimer t(ctx, name, 0.0, (double)sizeof(T) * S * M * N);
+    const long tot = M * N;
+    const int nb = (int)std::max(1L, std::min((tot + 255) / 256, 65536L));
+    hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(nb), dim3(256), 0, ctx->stream, Wk, S, M, N, C, ldc, beta);
+    JUES_HIP(hipGetLastError());
+    return;
+  }
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
   KTimer timer(ctx, name, flops, bytes);
@@ -211,16 +254,25 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
     const T* Ab = A + (long)b0 * sA;
     const T* Bb = B + (long)b0 * sB;
     T* Cb = C + (long)b0 * sC;
-#define JUES_GEMM_LAUNCH(TA_, TB_)                                                                  \
-  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, \
+#define JUES_GEMM_LAUNCH(TA_, TB_, K_)                                                             \
+  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, \
                      alpha, Ab, lda, sA, Bb, ldb, sB, beta, Cb, ldc, sC, (int)tiles_m, (int)tiles_n)
-    if (!ta && !tb) JUES_GEMM_LAUNCH(false, false);
-    else if (!ta && tb) JUES_GEMM_LAUNCH(false, true);
-    else if (ta && !tb) JUES_GEMM_LAUNCH(true, false);
-    else JUES_GEMM_LAUNCH(true, true);
+    if (!ta && !tb) {
+      if (kind == 1) JUES_GEMM_LAUNCH(false, false, 1);
+      else if (kind == 2) JUES_GEMM_LAUNCH(false, false, 2);
+      else JUES_GEMM_LAUNCH(false, false, 0);
+    } else if (!ta && tb) JUES_GEMM_LAUNCH(false, true, 0);
+    else if (ta && !tb) JUES_GEMM_LAUNCH(true, false, 0);
+    else JUES_GEMM_LAUNCH(true, true, 0);
 #undef JUES_GEMM_LAUNCH
     JUES_HIP(hipGetLastError());
   }
+}
+
+template <typename T>
+void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda, long sA,
+          const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch, const char* tag) {
+  gemm_impl<T>(ctx, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, tag, true);
 }
 
 template void gemm<double>(jues_ctx*, bool, bool, long, long, long, double, const double*, long, long,

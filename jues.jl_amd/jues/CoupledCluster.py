@@ -37,6 +37,8 @@ def _opts(kw: dict, method_default_guess: str) -> _lib.CCOpts:
     o.fp32_r_conv = float(kw.pop("fp32_r_conv", 1e-5))
     o.refine_maxit = int(kw.pop("refine_maxit", 60))
     o.verbose = int(bool(kw.pop("doprint", False)))
+    o.coupling = float(kw.pop("coupling", 1.0))
+    o.continuation = int(kw.pop("continuation", 1))
     return o
 
 
@@ -67,7 +69,7 @@ def _run(method: int, refWfn: Wfn, kw: dict, default_guess: str):
     def _cb(user, info):
         rec = info.contents
         history.append(dict(iter=rec.iter, stage=rec.stage, energy=rec.energy, delta_e=rec.delta_e,
-                            rnorm=rec.rnorm, diis_size=rec.diis_size, ms=rec.ms))
+                            rnorm=rec.rnorm, diis_size=rec.diis_size, ms=rec.ms, coupling=rec.coupling))
         if callback is not None:
             callback(history[-1])
 
@@ -143,7 +145,7 @@ class Solver:
         self.ctx.check(self.ctx.lib.jues_cc_iterate(self.ctx.h, prec_code(prec), nsteps, int(stop_on_conv),
                                                     C.byref(info), C.byref(conv)))
         return dict(iter=info.iter, stage=info.stage, energy=info.energy, delta_e=info.delta_e, rnorm=info.rnorm,
-                    diis_size=info.diis_size, ms=info.ms), bool(conv.value)
+                    diis_size=info.diis_size, ms=info.ms, coupling=info.coupling), bool(conv.value)
 
     def reset_diis(self):
         self.ctx.check(self.ctx.lib.jues_cc_reset_diis(self.ctx.h))

@@ -45,13 +45,14 @@ class CCOpts(C.Structure):
         ("maxit", C.c_int32), ("e_conv", C.c_double), ("r_conv", C.c_double), ("diis", C.c_int32),
         ("diis_max", C.c_int32), ("guess", C.c_int32), ("ccsd_eqs", C.c_int32), ("fcn", C.c_int32),
         ("fp32_r_conv", C.c_double), ("refine_maxit", C.c_int32), ("verbose", C.c_int32),
+        ("coupling", C.c_double), ("continuation", C.c_int32),
     ]
 
 
 class IterInfo(C.Structure):
     _fields_ = [
         ("iter", C.c_int32), ("stage", C.c_int32), ("energy", C.c_double), ("delta_e", C.c_double),
-        ("rnorm", C.c_double), ("diis_size", C.c_int32), ("ms", C.c_double),
+        ("rnorm", C.c_double), ("diis_size", C.c_int32), ("ms", C.c_double), ("coupling", C.c_double),
     ]
 
 
