@@ -1,10 +1,209 @@
-// MP2 adjoint (placeholder until the back-transform lands).
-#include "jues_internal.h"
+// Hand-written adjoint of the AO->MO transform and the RMP2 energy.
+//
+// E = sum K_iajb (2 K_iajb - K_ibja) / D_ijab with K = (ia|jb) (RMP2.jl:33-37 on
+// the block of Transformation.jl:39-93).  With every entry of C and g
+// independent, G_iajb = dE/dK_iajb = 2 (2 K_iajb - K_ibja) / D_ijab and
+//   dE/dg[mu,nu,lam,sig] = sum G_iajb Co[mu,i] Cv[nu,a] Co[lam,j] Cv[sig,b],
+//   dE/dC = [dC1 + dC3 | dC2 + dC4] for the four slots C enters:
+//     dC1[mu,i]  = sum H12[mu,nu,j,b] Gt[i,nu,j,b],   Gt  = sum_a Cv[nu,a] G[i,a,j,b]
+//     dC2[nu,a]  = sum H12[mu,nu,j,b] Gh[mu,a,j,b],   Gh  = sum_i Co[mu,i] G[i,a,j,b]
+//     dC3[lam,j] = sum H34[i,a,lam,sig] G3[i,a,j,sig], G3 = sum_b Cv[sig,b] G[i,a,j,b]
+//     dC4[sig,b] = sum H34[i,a,lam,sig] G4[i,a,lam,b], G4 = sum_j Co[lam,j] G[i,a,j,b]
+//   with the half-transformed integrals recomputed (never stored between calls):
+//     H12[mu,nu,j,b] = sum g[mu,nu,lam,sig] Co[lam,j] Cv[sig,b]
+//     H34[i,a,lam,sig] = sum g[mu,nu,lam,sig] Co[mu,i] Cv[nu,a].
+// No symmetry of g is assumed.  With factors, H12 = B . L^T and H34 = L . B^T
+// with L[i,a,P] = (Co^T B_P Cv)[i,a].
 #include "api_util.h"
+#include "jues_internal.h"
+
+#include <algorithm>
+
+namespace jues {
+namespace {
+
+// G[i,a,j,b] = 2(2K - Kx)/D and per-column energy partials (one block per (j,b))
+template <typename T>
+__global__ __launch_bounds__(256) void mp2_g_kernel(const T* __restrict__ K, const double* __restrict__ eps, int o,
+                                                    int v, T* __restrict__ G, double* __restrict__ part) {
+  __shared__ double sh[4];
+  const int ov = o * v;
+  const int jb = blockIdx.x;
+  const int j = jb % o, b = jb / o;
+  const double ejb = eps[j] - eps[o + b];
+  const long col = (long)jb * ov;
+  double s = 0.0;
+  for (int ia = threadIdx.x; ia < ov; ia += 256) {
+    const int i = ia % o, a = ia / o;
+    const double k = (double)K[col + ia];
+    const double kx = (double)K[(long)(i + o * b) + (long)ov * (j + o * a)];
+    const double d = eps[i] - eps[o + a] + ejb;
+    s += k * (2.0 * k - kx) / d;
+    G[col + ia] = (T)(2.0 * (2.0 * k - kx) / d);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[jb] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void sum_fixed_kernel2(const double* __restrict__ part, int count,
+                                                         double* __restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += 256) s += part[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+template <typename T>
+void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const double* eps, double* E, T* dC,
+                  T* dg) {
+  const long n = ctx->n, n2 = n * n, n3 = n2 * n;
+  const long ov = (long)o * v, N = ov * ov;
+  // K = (ia|jb), G, energy
+  DevBuf Kb(N * sizeof(T)), Gb(N * sizeof(T)), part((ov + 1) * sizeof(double));
+  T* K = Kb.as<T>();
+  T* G = Gb.as<T>();
+  tei_transform_dev<T>(ctx, Co, o, Cv, v, Co, o, Cv, v, K);
+  {
+    KTimer t(ctx, "mp2_grad_g", 0.0, (double)N * sizeof(T) * 3);
+    hipLaunchKernelGGL((mp2_g_kernel<T>), dim3((unsigned)ov), dim3(256), 0, ctx->stream, K, eps, o, v, G,
+                       part.as<double>());
+    hipLaunchKernelGGL(sum_fixed_kernel2, dim3(1), dim3(256), 0, ctx->stream, part.as<double>(), (int)ov,
+                       part.as<double>() + ov);
+    JUES_HIP(hipGetLastError());
+  }
+  d2h(ctx, E, part.as<double>() + ov, sizeof(double));
+  Kb.release();
+  const T one = T(1), zero = T(0);
+  // half-transformed integrals, recomputed here
+  DevBuf H12b(n2 * ov * sizeof(T)), H34b(n2 * ov * sizeof(T));
+  T* H12 = H12b.as<T>();  // [mu,nu,j,b]
+  T* H34 = H34b.as<T>();  // [i,a,lam,sig]
+  if (ctx->gao.p) {
+    const T* g = ao_dense<T>(ctx);
+    {
+      DevBuf Y(n3 * v * sizeof(T));  // Y[mu,nu,lam,b] = sum_sig g Cv[sig,b]
+      gemm<T>(ctx, false, false, n3, v, n, one, g, n3, 0, Cv, n, 0, zero, Y.as<T>(), n3, 0, 1, "mp2grad_half");
+      gemm<T>(ctx, false, false, n2, o, n, one, Y.as<T>(), n2, n3, Co, n, 0, zero, H12, n2, n2 * o, v,
+              "mp2grad_half");
+    }
+    {
+      DevBuf Z(o * n3 * sizeof(T));  // Z[i,nu,lam,sig] = sum_mu Co[mu,i] g
+      gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z.as<T>(), o, 0, 1, "mp2grad_half");
+      gemm<T>(ctx, false, false, o, v, n, one, Z.as<T>(), o, (long)o * n, Cv, n, 0, zero, H34, o, ov, (int)n2,
+              "mp2grad_half");
+    }
+  } else {
+    const T* B = ao_factors<T>(ctx);
+    const long naux = ctx->naux;
+    DevBuf L(ov * naux * sizeof(T));
+    mo_factors_dev<T>(ctx, Co, o, Cv, v, L.as<T>());
+    gemm<T>(ctx, false, true, n2, ov, naux, one, B, n2, 0, L.as<T>(), ov, 0, zero, H12, n2, 0, 1, "mp2grad_half");
+    gemm<T>(ctx, false, true, ov, n2, naux, one, L.as<T>(), ov, 0, B, n2, 0, zero, H34, ov, 0, 1, "mp2grad_half");
+  }
+  T* dCo = dC;           // n x o  (columns 0..o-1 of dC, leading dimension n)
+  T* dCv = dC + n * o;   // n x v
+  // slot 1: Gt[i,nu,jb] = G[i,a,jb] Cv[nu,a]^T ; dC1 = H12 (n x n ov) . Gt^T
+  {
+    DevBuf Gt((long)o * n * ov * sizeof(T));
+    gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, (long)o * n, (int)ov,
+            "mp2grad");
+    gemm<T>(ctx, false, true, n, o, n * ov, one, H12, n, 0, Gt.as<T>(), o, 0, zero, dCo, n, 0, 1, "mp2grad");
+  }
+  // slot 2: Gh[mu,a,jb] = Co . G ; dC2[nu,a] = sum_{mu,jb} H12[mu,nu,jb] Gh[mu,a,jb]
+  {
+    DevBuf Gh(n * v * ov * sizeof(T)), H12p(n2 * ov * sizeof(T)), Ghp(n * v * ov * sizeof(T));
+    gemm<T>(ctx, false, false, n, v * ov, o, one, Co, n, 0, G, o, 0, zero, Gh.as<T>(), n, 0, 1, "mp2grad");
+    permute<T, T>(ctx, H12, {n, n, o, v}, {1, 0, 2, 3}, H12p.as<T>(), 1.0, 0.0, "mp2grad_perm");
+    permute<T, T>(ctx, Gh.as<T>(), {n, v, o, v}, {1, 0, 2, 3}, Ghp.as<T>(), 1.0, 0.0, "mp2grad_perm");
+    gemm<T>(ctx, false, true, n, v, n * ov, one, H12p.as<T>(), n, 0, Ghp.as<T>(), v, 0, zero, dCv, n, 0, 1,
+            "mp2grad");
+  }
+  // slot 3: G3[ia,j,sig] = G[iaj,b] Cv[sig,b] ; dC3[lam,j] = sum_{ia,sig} H34[ia,lam,sig] G3[ia,j,sig]
+  {
+    DevBuf G3(ov * o * n * sizeof(T)), H34p(n2 * ov * sizeof(T)), G3p(ov * o * n * sizeof(T));
+    gemm<T>(ctx, false, true, ov * o, n, v, one, G, ov * o, 0, Cv, n, 0, zero, G3.as<T>(), ov * o, 0, 1, "mp2grad");
+    permute<T, T>(ctx, H34, {ov, n, n}, {0, 2, 1}, H34p.as<T>(), 1.0, 0.0, "mp2grad_perm");
+    permute<T, T>(ctx, G3.as<T>(), {ov, o, n}, {0, 2, 1}, G3p.as<T>(), 1.0, 0.0, "mp2grad_perm");
+    gemm<T>(ctx, true, false, n, o, ov * n, one, H34p.as<T>(), ov * n, 0, G3p.as<T>(), ov * n, 0, one, dCo, n, 0, 1,
+            "mp2grad");
+  }
+  // slot 4: G4[ia,lam,b] = G[ia,j,b] Co[lam,j] ; dC4[sig,b] = sum_{ia,lam} H34[ia,lam,sig] G4[ia,lam,b]
+  {
+    DevBuf G4(ov * n * v * sizeof(T));
+    gemm<T>(ctx, false, true, ov, n, o, one, G, ov, ov * o, Co, n, 0, zero, G4.as<T>(), ov, ov * n, v, "mp2grad");
+    gemm<T>(ctx, true, false, n, v, ov * n, one, H34, ov * n, 0, G4.as<T>(), ov * n, 0, one, dCv, n, 0, 1,
+            "mp2grad");
+  }
+  H12b.release();
+  H34b.release();
+  if (dg) {
+    // back-transform G -> dE/dg: [i,a,j,b] -> Gh[mu,a,j,b] -> X[mu,a,j,sig] -> [a,mu,sig,j]
+    //   -> [nu,mu,sig,j] -> [nu,mu,sig,lam] -> [mu,nu,lam,sig]
+    DevBuf Gh(n * v * ov * sizeof(T));
+    gemm<T>(ctx, false, false, n, v * ov, o, one, Co, n, 0, G, o, 0, zero, Gh.as<T>(), n, 0, 1, "mp2grad_back");
+    DevBuf X(n * v * o * n * sizeof(T));
+    gemm<T>(ctx, false, true, n * v * o, n, v, one, Gh.as<T>(), n * v * o, 0, Cv, n, 0, zero, X.as<T>(), n * v * o,
+            0, 1, "mp2grad_back");
+    Gh.release();
+    DevBuf Xp(n * v * o * n * sizeof(T));
+    permute<T, T>(ctx, X.as<T>(), {n, v, o, n}, {1, 0, 3, 2}, Xp.as<T>(), 1.0, 0.0, "mp2grad_perm");  // [a,mu,sig,j]
+    X.release();
+    DevBuf Y(n3 * o * sizeof(T));
+    gemm<T>(ctx, false, false, n, n * n * o, v, one, Cv, n, 0, Xp.as<T>(), v, 0, zero, Y.as<T>(), n, 0, 1,
+            "mp2grad_back");  // [nu,mu,sig,j]
+    Xp.release();
+    DevBuf W(n3 * n * sizeof(T));
+    gemm<T>(ctx, false, true, n3, n, o, one, Y.as<T>(), n3, 0, Co, n, 0, zero, W.as<T>(), n3, 0, 1,
+            "mp2grad_back");  // [nu,mu,sig,lam]
+    Y.release();
+    permute<T, T>(ctx, W.as<T>(), {n, n, n, n}, {1, 0, 3, 2}, dg, 1.0, 0.0, "mp2grad_perm");
+  }
+}
+
+}  // namespace
+}  // namespace jues
+
 using namespace jues;
-extern "C" jues_status jues_rmp2_grad(jues_ctx* ctx, int, const double*, int, int, const double*, double*, double*,
-                                      double*) {
+
+extern "C" jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, int o, int n, const double* eps,
+                                      double* E, double* dE_dC, double* dE_dg) {
   JUES_API_BEGIN(ctx)
-  throw Error(JUES_BAD_ARG, "not implemented");
+  JUES_REQUIRE(C && eps && E && o > 0 && o < n, "jues_rmp2_grad: bad arguments");
+  JUES_REQUIRE(n == ctx->n && n > 0, "jues_rmp2_grad: n does not match the AO integrals");
+  JUES_REQUIRE(!dE_dg || ctx->gao.p, "jues_rmp2_grad: dE/dg needs dense AO integrals");
+  JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED, "jues_rmp2_grad: bad precision");
+  const int v = n - o;
+  DeviceArray<double> e(ctx, eps, (size_t)n);
+  const size_t nn = (size_t)n * n, n4 = nn * nn;
+  if (prec == JUES_FP64) {
+    DeviceArray<double> c(ctx, C, nn);
+    DevBuf dC(nn * sizeof(double));
+    DevBuf dg(dE_dg ? n4 * sizeof(double) : 0);
+    mp2_grad_dev<double>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<double>(),
+                         dE_dg ? dg.as<double>() : nullptr);
+    if (dE_dC) d2h(ctx, dE_dC, dC.p, nn * sizeof(double));
+    if (dE_dg) d2h(ctx, dE_dg, dg.p, n4 * sizeof(double));
+  } else {
+    DeviceArray<float> c(ctx, C, nn);
+    DevBuf dC(nn * sizeof(float)), dC64(nn * sizeof(double));
+    DevBuf dg(dE_dg ? n4 * sizeof(float) : 0);
+    mp2_grad_dev<float>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<float>(), dE_dg ? dg.as<float>() : nullptr);
+    convert<float, double>(ctx, dC.as<float>(), dC64.as<double>(), (long)nn);
+    if (dE_dC) d2h(ctx, dE_dC, dC64.p, nn * sizeof(double));
+    if (dE_dg) {
+      DevBuf dg64(n4 * sizeof(double));
+      convert<float, double>(ctx, dg.as<float>(), dg64.as<double>(), (long)n4);
+      d2h(ctx, dE_dg, dg64.p, n4 * sizeof(double));
+    }
+  }
+  JUES_REQUIRE(std::isfinite(*E), "jues_rmp2_grad: non-finite energy");
   JUES_API_END(ctx)
 }

@@ -73,15 +73,33 @@ def _ao_key(eri) -> tuple:
     return ("dense", id(eri), eri.shape)
 
 
-def ensure_ao(ctx: "_lib.Context", eri) -> int:
-    """Make ``eri`` the context's current AO integrals (uploads only when it changed). Returns n."""
+def ensure_ao(ctx: "_lib.Context", eri, force: bool = False) -> int:
+    """Make ``eri`` the context's current AO integrals. Returns n.
+
+    The device copy is reused while the same array object is passed again (the
+    context holds a reference, so the identity cannot be recycled).  After
+    modifying an ``ao_eri`` array in place, pass ``force=True`` (or call
+    ``invalidate``) to upload it again.
+    """
     key = _ao_key(eri)
-    if ctx.ao_key != key:
+    if force or ctx.ao_key != key or getattr(ctx, "ao_ref", None) is not _ao_obj(eri):
         if isinstance(eri, FactoredERI):
             ctx.set_ao_factors(eri.B, key)
         else:
             ctx.set_ao_eri(eri, key)
+        ctx.ao_ref = _ao_obj(eri)
     return eri.n if isinstance(eri, FactoredERI) else eri.shape[0]
+
+
+def _ao_obj(eri):
+    return eri.B if isinstance(eri, FactoredERI) else eri
+
+
+def invalidate(device: int = 0) -> None:
+    """Forget the device copy of the AO integrals (the next call uploads again)."""
+    ctx = _lib.context(device)
+    ctx.ao_key = None
+    ctx.ao_ref = None
 
 
 def prec_code(prec: str) -> int:

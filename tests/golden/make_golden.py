@@ -1,0 +1,73 @@
+"""Generate the golden fixtures in tests/golden/ from the CPU oracle.
+
+    python tests/golden/make_golden.py
+
+cfg1 (n=24, o=5, v=19, naux=72): RMP2, RCCD (mRCCD.jl equations), RCCSD with
+both reference equation sets, the first three plain-Jacobi energies from t=0,
+converged amplitudes.  cfg2 (n=120, o=20, v=100, naux=360): the full-strength
+iteration diverges for these integrals, so RCCD and RCCSD are converged for the
+integrals scaled by a coupling of 0.02 (factors scaled by sqrt(0.02)); norms and
+sampled entries of the amplitudes are stored.  Oracles converge to
+|dE| < 1e-13 and ||dt|| < 1e-12.
+"""
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path[:0] = [str(ROOT / "oracle"), str(ROOT / "jues.jl_amd")]
+import jues_oracle as O  # noqa: E402
+from jues import synthetic  # noqa: E402
+
+SAMPLE_IDX = 64
+
+
+def cfg1():
+    s = synthetic.workload("cfg1_rccd_n24")
+    g = s.ao_eri()
+    oovv = O.get_eri(g, s.Co, s.Cv, "OOVV")
+    Emp2 = O.rmp2(oovv, s.eps, s.o)
+    I = O.make_rccd_integrals(g, s.Co, s.Cv)
+    hist = []
+    O.solve_rccd(I, s.eps, s.o, guess="zero", diis=False, maxit=3, history=hist)
+    E_ccd, T_ccd, _ = O.solve_rccd(I, s.eps, s.o, guess="zero", max_vecs=8, maxit=400)
+    Ic = O.make_rccsd_integrals(g, s.Co, s.Cv)
+    E_m, T1_m, T2_m, _ = O.solve_rccsd(Ic, s.eps, s.o, max_vecs=8, maxit=400)
+    b = O.make_rccsd15_integrals(lambda a, b_, c, d: O.tei_transform(g, a, b_, c, d), s.Co, s.Cv)
+    E_r, T1_r, T2_r, _ = O.solve_rccsd(b, s.eps, s.o, max_vecs=8, maxit=400, update=O.rccsd15_update)
+    K = O.tei_transform(g, s.Co, s.Cv, s.Co, s.Cv)
+    np.savez_compressed(HERE / "cfg1.npz", E_mp2=Emp2, jacobi_E=np.array([h[0] for h in hist]),
+                        E_rccd=E_ccd, T2_rccd=T_ccd, E_mrccsd=E_m, T1_mrccsd=T1_m, T2_mrccsd=T2_m,
+                        E_rccsd=E_r, T1_rccsd=T1_r, T2_rccsd=T2_r,
+                        iajb_sum=K.sum(), iajb_sumsq=(K * K).sum())
+    print("cfg1", Emp2, E_ccd, E_m, E_r)
+
+
+def cfg2(lam=0.02):
+    s = synthetic.workload("cfg2_rccsd_n120")
+    B = s.B * np.sqrt(lam)
+    t0 = time.time()
+    I = O.rccd_integrals_from_factors(B, s.Co, s.Cv)
+    Emp2 = O.rmp2(I["oovv"], s.eps, s.o)
+    E_ccd, T_ccd, it = O.solve_rccd(I, s.eps, s.o, guess="zero", max_vecs=8, maxit=200)
+    print("cfg2 rccd", E_ccd, it, time.time() - t0, flush=True)
+    Ic = O.rccsd_integrals_from_factors(B, s.Co, s.Cv)
+    E_m, T1_m, T2_m, it2 = O.solve_rccsd(Ic, s.eps, s.o, max_vecs=8, maxit=200)
+    print("cfg2 rccsd", E_m, it2, time.time() - t0, flush=True)
+    rng = np.random.default_rng(7)
+    idx = rng.integers(0, T_ccd.size, SAMPLE_IDX)
+    np.savez_compressed(HERE / "cfg2_coupling0.02.npz", coupling=lam, E_mp2=Emp2,
+                        E_rccd=E_ccd, T2_rccd_norm=np.linalg.norm(T_ccd), T2_rccd_sample=T_ccd.ravel(order="F")[idx],
+                        E_mrccsd=E_m, T1_mrccsd=T1_m, T2_mrccsd_norm=np.linalg.norm(T2_m),
+                        T2_mrccsd_sample=T2_m.ravel(order="F")[idx], sample_idx=idx)
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["cfg1", "cfg2"]
+    if "cfg1" in which:
+        cfg1()
+    if "cfg2" in which:
+        cfg2()
