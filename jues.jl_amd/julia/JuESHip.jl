@@ -1,0 +1,278 @@
+"""
+    JuESHip
+
+Julia binding of `libjues_hip.so` (the C ABI in `include/jues.h`).  It provides
+the JuES entry points of the correlation hot path with the same names and
+arguments as the reference modules (`Transformation.tei_transform`,
+`IntegralTransformation.get_eri`, `MollerPlesset.do_rmp2`,
+`CoupledCluster.{RCCD,mRCCD}.do_rccd`, `CoupledCluster.{RCCSD,mRCCSD}.do_rccsd`)
+and a ChainRulesCore `rrule` for the RMP2 energy as a function of the MO
+coefficients and the AO integrals.
+
+Julia arrays are column-major, which is the memory order the library expects,
+so arrays are passed without copies (`GC.@preserve` keeps them alive during the
+call).  A non-zero status becomes a Julia `error`, as the reference's `error()`
+calls do.
+
+The library location comes from `ENV["JUES_HIP_LIB"]`, else the in-tree build
+`jues.jl_amd/lib/libjues_hip.so`.
+"""
+module JuESHip
+
+using ChainRulesCore
+
+const libjues = get(ENV, "JUES_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libjues_hip.so"))
+
+const JUES_FP64 = Cint(0)
+const JUES_FP32_MIXED = Cint(1)
+
+struct CCOpts
+    maxit::Int32
+    e_conv::Float64
+    r_conv::Float64
+    diis::Int32
+    diis_max::Int32
+    guess::Int32
+    ccsd_eqs::Int32
+    fcn::Int32
+    fp32_r_conv::Float64
+    refine_maxit::Int32
+    verbose::Int32
+    coupling::Float64
+    continuation::Int32
+end
+
+function CCOpts(; maxit=100, cc_max_iter=maxit, cc_e_conv=1e-10, cc_max_rms=1e-9, diis=true,
+                diis_max=5, guess=:mp2, ccsd_eqs=:mRCCSD, fcn=0, fp32_r_conv=1e-5,
+                refine_maxit=60, doprint=false, coupling=1.0, continuation=1)
+    g = Dict(:zero => 0, :mp2 => 1, :ovov => 2, :user => 3)[guess]
+    e = Dict(:mRCCSD => 0, :RCCSD => 1)[ccsd_eqs]
+    CCOpts(cc_max_iter, cc_e_conv, cc_max_rms, diis, diis_max, g, e, fcn, fp32_r_conv,
+           refine_maxit, doprint, coupling, continuation)
+end
+
+precision_code(p::Symbol) = p === :fp64 ? JUES_FP64 : p === :fp32_mixed ? JUES_FP32_MIXED :
+                             error("precision must be :fp64 or :fp32_mixed")
+
+# ---------------------------------------------------------------------------
+# context (one per Julia thread, created lazily)
+# ---------------------------------------------------------------------------
+mutable struct Context
+    handle::Ptr{Cvoid}
+    ao_key::Any
+end
+
+const CONTEXTS = Dict{Int,Context}()
+
+function context(device::Integer=0)
+    get!(CONTEXTS, Threads.threadid()) do
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:jues_create, libjues), Cint, (Ptr{Ptr{Cvoid}}, Cint), h, device)
+        st == 0 || error("jues_create failed (status $st)")
+        ctx = Context(h[], nothing)
+        finalizer(c -> ccall((:jues_destroy, libjues), Cvoid, (Ptr{Cvoid},), c.handle), ctx)
+        ctx
+    end
+end
+
+function check(ctx::Context, st::Cint; allow_not_converged=false)
+    st == 0 && return
+    (allow_not_converged && st == 1) && return
+    msg = unsafe_string(ccall((:jues_last_error, libjues), Cstring, (Ptr{Cvoid},), ctx.handle))
+    error(msg)
+end
+
+"""AO integrals carried by a Wfn (`ao_eri` dense n^4) or a factor array B[mu,nu,P]."""
+struct FactoredERI
+    B::Array{Float64,3}
+end
+
+function ensure_ao!(ctx::Context, gao::Array{Float64,4})
+    if ctx.ao_key !== gao
+        n = size(gao, 1)
+        GC.@preserve gao check(ctx, ccall((:jues_set_ao_eri, libjues), Cint,
+                                         (Ptr{Cvoid}, Ptr{Float64}, Cint), ctx.handle, gao, n))
+        ctx.ao_key = gao
+    end
+    size(gao, 1)
+end
+
+function ensure_ao!(ctx::Context, f::FactoredERI)
+    if ctx.ao_key !== f
+        n, _, naux = size(f.B)
+        GC.@preserve f check(ctx, ccall((:jues_set_ao_factors, libjues), Cint,
+                                       (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint), ctx.handle, f.B, n, naux))
+        ctx.ao_key = f
+    end
+    size(f.B, 1)
+end
+
+# ---------------------------------------------------------------------------
+# Transformation.tei_transform   (Backend/Transformation.jl:15-93)
+# ---------------------------------------------------------------------------
+module Transformation
+import ..JuESHip: context, ensure_ao!, check, libjues, precision_code, FactoredERI
+export tei_transform
+
+function tei_transform(gao, C1::Matrix{Float64}, C2::Matrix{Float64}, C3::Matrix{Float64},
+                       C4::Matrix{Float64}, name::String="default"; precision=:fp64)
+    ctx = context()
+    n = ensure_ao!(ctx, gao)
+    d = (size(C1, 2), size(C2, 2), size(C3, 2), size(C4, 2))
+    out = Array{Float64,4}(undef, d...)
+    GC.@preserve C1 C2 C3 C4 out check(ctx, ccall((:jues_tei_transform, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint,
+         Cint, Ptr{Float64}),
+        ctx.handle, precision_code(precision), C1, d[1], C2, d[2], C3, d[3], C4, d[4], n, out))
+    return out
+end
+tei_transform(gao, C::Matrix{Float64}, name::String="default"; kw...) =
+    tei_transform(gao, C, C, C, C, name; kw...)
+end
+
+# ---------------------------------------------------------------------------
+# IntegralTransformation.get_eri   (Backend/IntegralTransformation.jl:38-101)
+# ---------------------------------------------------------------------------
+module IntegralTransformation
+import ..JuESHip: context, ensure_ao!, check, libjues, precision_code
+export get_eri
+
+function get_eri(wfn, eri_string::String; notation::String="phys", fcn::Int=0, precision=:fp64)
+    sizeof(eri_string) == 4 || error("Invalid string given to JuES.IntegralTransformation.get_eri: $eri_string")
+    ctx = context()
+    n = ensure_ao!(ctx, wfn.ao_eri)
+    o, v = wfn.nalpha, wfn.nvira
+    dims = Tuple(c in ('o', 'O') ? o - fcn : v for c in eri_string)
+    out = Array{Float64,4}(undef, dims...)
+    Co, Cv = wfn.Cao, wfn.Cav
+    GC.@preserve Co Cv out check(ctx, ccall((:jues_get_eri, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Cstring, Cstring, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Cint, Ptr{Float64}),
+        ctx.handle, precision_code(precision), eri_string, notation, Co, o, Cv, v, n, fcn, out))
+    return out
+end
+end
+
+# ---------------------------------------------------------------------------
+# MollerPlesset.do_rmp2   (MollerPlesset/RMP2.jl:11-45) and its adjoint
+# ---------------------------------------------------------------------------
+module MollerPlesset
+import ..JuESHip: context, ensure_ao!, check, libjues, precision_code
+using ChainRulesCore
+export do_rmp2, rmp2_energy
+
+function do_rmp2(refWfn; precision=:fp64, kwargs...)
+    ctx = context()
+    n = ensure_ao!(ctx, refWfn.ao_eri)
+    Co, Cv, eps = refWfn.Cao, refWfn.Cav, refWfn.epsa
+    E = Ref{Float64}(0.0)
+    GC.@preserve Co Cv eps check(ctx, ccall((:jues_rmp2, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}),
+        ctx.handle, precision_code(precision), Co, refWfn.nalpha, Cv, refWfn.nvira, n, eps, E))
+    return E[]
+end
+
+"""
+    rmp2_energy(C, gao, eps, nocc)
+
+RMP2 correlation energy as a differentiable function of the full MO matrix `C`
+(n x n, occupied columns first) and the dense AO integrals `gao` (n^4); `eps`
+is held fixed.
+"""
+function rmp2_energy(C::Matrix{Float64}, gao::Array{Float64,4}, eps::Vector{Float64}, nocc::Int;
+                     precision=:fp64)
+    E, _, _ = rmp2_grad(C, gao, eps, nocc; want_dC=false, want_dg=false, precision=precision)
+    return E
+end
+
+function rmp2_grad(C, gao, eps, nocc; want_dC=true, want_dg=true, precision=:fp64)
+    ctx = context()
+    n = ensure_ao!(ctx, gao)
+    E = Ref{Float64}(0.0)
+    dC = want_dC ? zeros(n, n) : nothing
+    dg = want_dg ? zeros(n, n, n, n) : nothing
+    pC = dC === nothing ? Ptr{Float64}(C_NULL) : pointer(dC)
+    pg = dg === nothing ? Ptr{Float64}(C_NULL) : pointer(dg)
+    GC.@preserve C eps dC dg check(ctx, ccall((:jues_rmp2_grad, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}, Ptr{Float64}, Ptr{Float64}),
+        ctx.handle, precision_code(precision), C, nocc, n, eps, E, pC, pg))
+    return E[], dC, dg
+end
+
+# reverse rule: the library's hand-written adjoint (recomputes the half-transformed integrals)
+function ChainRulesCore.rrule(::typeof(rmp2_energy), C::Matrix{Float64}, gao::Array{Float64,4},
+                              eps::Vector{Float64}, nocc::Int; precision=:fp64)
+    E, dC, dg = rmp2_grad(C, gao, eps, nocc; precision=precision)
+    function rmp2_energy_pullback(Ebar)
+        eb = unthunk(Ebar)
+        return NoTangent(), eb .* dC, eb .* dg, ZeroTangent(), NoTangent()
+    end
+    return E, rmp2_energy_pullback
+end
+end
+
+# ---------------------------------------------------------------------------
+# CoupledCluster.{RCCD,mRCCD,RCCSD,mRCCSD}
+# ---------------------------------------------------------------------------
+module CoupledCluster
+import ..JuESHip: context, ensure_ao!, check, libjues, precision_code, CCOpts
+
+function _cc(method, refWfn; precision=:fp64, return_T=false, check_convergence=true,
+             default_guess=:mp2, kwargs...)
+    kw = Dict{Symbol,Any}(kwargs)
+    haskey(kw, :guess) || (kw[:guess] = default_guess)
+    opts = Ref(CCOpts(; kw...))
+    ctx = context()
+    n = ensure_ao!(ctx, refWfn.ao_eri)
+    o, v = refWfn.nalpha, refWfn.nvira
+    oa = o - opts[].fcn
+    Co, Cv, eps = refWfn.Cao, refWfn.Cav, refWfn.epsa
+    T2 = zeros(oa, oa, v, v)
+    T1 = zeros(oa, v)
+    E = Ref{Float64}(0.0)
+    st = GC.@preserve Co Cv eps T1 T2 begin
+        if method == 0
+            ccall((:jues_rccd, libjues), Cint,
+                  (Ptr{Cvoid}, Cint, Ref{CCOpts}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64},
+                   Ref{Float64}, Ptr{Float64}, Ptr{Cvoid}, Ptr{Cvoid}),
+                  ctx.handle, precision_code(precision), opts, Co, o, Cv, v, n, eps, E, T2, C_NULL, C_NULL)
+        else
+            ccall((:jues_rccsd, libjues), Cint,
+                  (Ptr{Cvoid}, Cint, Ref{CCOpts}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64},
+                   Ref{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}, Ptr{Cvoid}),
+                  ctx.handle, precision_code(precision), opts, Co, o, Cv, v, n, eps, E, T1, T2, C_NULL, C_NULL)
+        end
+    end
+    check(ctx, st; allow_not_converged=!check_convergence)
+    if return_T
+        return method == 0 ? (E[], T2) : (E[], T1, T2)
+    end
+    return E[]
+end
+
+module RCCD
+import ..CoupledCluster: _cc
+"RCCD.jl:33-83 (default guess (ij|ab)/D, RCCD.jl:145-160)"
+do_rccd(refWfn; return_T2=false, kwargs...) = _cc(0, refWfn; return_T=return_T2, default_guess=:ovov, kwargs...)
+end
+
+module mRCCD
+import ..CoupledCluster: _cc
+"mRCCD.jl:38-118 (DIIS, zero guess mRCCD.jl:245)"
+do_rccd(refWfn; maxit=100, doprint=false, return_T2=false, kwargs...) =
+    _cc(0, refWfn; return_T=return_T2, default_guess=:zero, maxit=maxit, doprint=doprint, kwargs...)
+end
+
+module RCCSD
+import ..CoupledCluster: _cc
+"RCCSD.jl:33-101 (Fae T1 term of RCCSD.jl:189)"
+do_rccsd(refWfn; return_T=false, kwargs...) = _cc(1, refWfn; return_T=return_T, ccsd_eqs=:RCCSD, kwargs...)
+end
+
+module mRCCSD
+import ..CoupledCluster: _cc
+"mRCCSD.jl:35-101 (Fae T1 term of mRCCSD.jl:140)"
+do_rccsd(refWfn; return_T=false, kwargs...) = _cc(1, refWfn; return_T=return_T, ccsd_eqs=:mRCCSD, kwargs...)
+end
+end
+
+end # module JuESHip
