@@ -43,7 +43,22 @@ template <> struct Mfma<float> {
   }
 };
 
-constexpr int BM = 128, BN = 128, NT = 256, PAD = 1, NXCD = 8, GROUP_M = 8;
+constexpr int BM = 128, BN = 128, NT = 256, NXCD = 8, GROUP_M = 8;
+
+// Raw buffer loads: the 64-bit base lives in a scalar resource descriptor, the
+// per-lane part is one 32-bit byte offset and the per-element part a scalar
+// offset, so an interior tile's loads hold no 64-bit address registers.
+constexpr int kBufferDword3 = 0x00020000;  // gfx9-family raw buffer descriptor word 3
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, kBufferDword3);
+}
+template <typename T> __device__ inline T bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ inline double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <> __device__ inline float bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
 
 __device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, int& tm, int& tn) {
   // XCD-aware remap (bijective for any nwg): consecutive hardware ids go to
@@ -63,6 +78,20 @@ __device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, i
   tn = r / gsz;
 }
 
+// Shared-memory layout of one operand slice.  When the operand's contiguous
+// global dimension is the row index (m, or n for B) the slice is stored k-major,
+// [k][m], padded so that the four k rows an MFMA fragment read touches start 16
+// (FP32) / 32 (FP64) banks apart; when k is contiguous it is stored m-major,
+// [m][k] with an odd padded row, so that both the coalesced stores and the
+// fragment reads are bank-conflict free.
+template <typename T, bool KMAJOR, int ROWS, int BK>
+struct Slice {
+  static constexpr int PADR = sizeof(T) == 8 ? 16 : 16;  // k-major row padding (elements)
+  static constexpr int PADK = 1;                          // m-major row padding (elements)
+  static constexpr int SIZE = KMAJOR ? BK * (ROWS + PADR) : ROWS * (BK + PADK);
+  __device__ static inline int idx(int r, int k) { return KMAJOR ? k * (ROWS + PADR) + r : r * (BK + PADK) + k; }
+};
+
 // KIND only separates kernel symbols (0 generic, 1 CC ring, 2 CC pp-ladder) so that
 // profiles attribute time to the coupled-cluster hot spots.
 template <typename T, bool TA, bool TB, int KIND>
@@ -70,12 +99,15 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
                                                       const T* __restrict__ A, long lda, long sA,
                                                       const T* __restrict__ B, long ldb, long sB,
                                                       T beta, T* __restrict__ C, long ldc, long sC,
-                                                      int tiles_m, int tiles_n) {
+                                                      int tiles_m, int tiles_n, int fast_ok) {
   constexpr int BK = Mfma<T>::BK;
   constexpr int LA = BM * BK / NT;  // A elements loaded per thread per slice
   constexpr int LB = BN * BK / NT;
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
+  // A is m-contiguous unless transposed; B is k-contiguous unless transposed.
+  using SA = Slice<T, !TA, BM, BK>;
+  using SB = Slice<T, TB, BN, BK>;
+  __shared__ T As[2][SA::SIZE];
+  __shared__ T Bs[2][SB::SIZE];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -87,43 +119,65 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
   tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
 
+  // element e of a thread's share: the contiguous global dimension varies fastest
+  auto a_rk = [](int e, int& m, int& k) {
+    if (TA) { k = e % BK; m = e / BK; } else { m = e % BM; k = e / BM; }
+  };
+  auto b_rk = [](int e, int& n, int& k) {
+    if (TB) { n = e % BN; k = e / BN; } else { k = e % BK; n = e / BK; }
+  };
+
   T ra[LA], rb[LB];
+  // Interior tiles and full k-slices load without any bounds logic: element r of
+  // a thread's share sits at p0 + k0*kstride + r*rstride.  Edge tiles and the
+  // last partial k-slice clamp rows/columns into the matrix (those values only
+  // reach output elements that are never stored) and zero k >= K with a select,
+  // so no load is ever predicated by a branch.
+  // per-lane byte offsets inside a tile's k-slice and per-element scalar strides
+  const int a_lane = (int)(sizeof(T) * (TA ? (tid % BK) + (long)(tid / BK) * lda : (tid % BM) + (long)(tid / BM) * lda));
+  const int b_lane = (int)(sizeof(T) * (TB ? (tid % BN) + (long)(tid / BN) * ldb : (tid % BK) + (long)(tid / BK) * ldb));
+  const int a_rs = (int)(sizeof(T) * (TA ? (long)(NT / BK) * lda : (long)(NT / BM) * lda));
+  const int b_rs = (int)(sizeof(T) * (TB ? (long)(NT / BN) * ldb : (long)(NT / BK) * ldb));
+  const bool interior = fast_ok && (m0 + BM <= M) && (n0 + BN <= N);
   auto load = [&](int k0) {
+    if (interior && k0 + BK <= K) {
+      const auto rA = make_rsrc(TA ? A + k0 + (long)m0 * lda : A + m0 + (long)k0 * lda);
+      const auto rB = make_rsrc(TB ? B + n0 + (long)k0 * ldb : B + k0 + (long)n0 * ldb);
 #pragma unroll
-    for (int r = 0; r < LA; ++r) {
-      const int e = tid + NT * r;
-      const int m = TA ? e / BK : e % BM;
-      const int k = TA ? e % BK : e / BM;
-      const int gm = m0 + m, gk = k0 + k;
-      T val = T(0);
-      if (gm < M && gk < K) val = TA ? A[(long)gk + (long)gm * lda] : A[(long)gm + (long)gk * lda];
-      ra[r] = val;
-    }
+      for (int r = 0; r < LA; ++r) ra[r] = bload<T>(rA, a_lane, r * a_rs);
 #pragma unroll
-    for (int r = 0; r < LB; ++r) {
-      const int e = tid + NT * r;
-      const int n = TB ? e % BN : e / BK;
-      const int k = TB ? e / BN : e % BK;
-      const int gn = n0 + n, gk = k0 + k;
-      T val = T(0);
-      if (gn < N && gk < K) val = TB ? B[(long)gn + (long)gk * ldb] : B[(long)gk + (long)gn * ldb];
-      rb[r] = val;
+      for (int r = 0; r < LB; ++r) rb[r] = bload<T>(rB, b_lane, r * b_rs);
+    } else {
+#pragma unroll
+      for (int r = 0; r < LA; ++r) {
+        int m, k;
+        a_rk(tid + NT * r, m, k);
+        const int gm = min(m0 + m, M - 1), gk = k0 + k, gkc = min(gk, K - 1);
+        const T val = TA ? A[(long)gkc + (long)gm * lda] : A[(long)gm + (long)gkc * lda];
+        ra[r] = gk < K ? val : T(0);
+      }
+#pragma unroll
+      for (int r = 0; r < LB; ++r) {
+        int n, k;
+        b_rk(tid + NT * r, n, k);
+        const int gn = min(n0 + n, N - 1), gk = k0 + k, gkc = min(gk, K - 1);
+        const T val = TB ? B[(long)gn + (long)gkc * ldb] : B[(long)gkc + (long)gn * ldb];
+        rb[r] = gk < K ? val : T(0);
+      }
     }
   };
-  auto store = [&]() {
+  auto store = [&](int buf) {
 #pragma unroll
     for (int r = 0; r < LA; ++r) {
-      const int e = tid + NT * r;
-      const int m = TA ? e / BK : e % BM;
-      const int k = TA ? e % BK : e / BM;
-      As[k][m] = ra[r];
+      int m, k;
+      a_rk(tid + NT * r, m, k);
+      As[buf][SA::idx(m, k)] = ra[r];
     }
 #pragma unroll
     for (int r = 0; r < LB; ++r) {
-      const int e = tid + NT * r;
-      const int n = TB ? e % BN : e / BK;
-      const int k = TB ? e / BN : e % BK;
-      Bs[k][n] = rb[r];
+      int n, k;
+      b_rk(tid + NT * r, n, k);
+      Bs[buf][SB::idx(n, k)] = rb[r];
     }
   };
 
@@ -133,26 +187,47 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = typename Mfma<T>::acc{0, 0, 0, 0};
 
+  const int fr = lane & 15, fk = lane >> 4;
+  T af[2][4], bf[2][4];
+  auto frags = [&](int buf, int kk, int slot) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) af[slot][i] = As[buf][SA::idx(wm * 64 + i * 16 + fr, kk + fk)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bf[slot][j] = Bs[buf][SB::idx(wn * 64 + j * 16 + fr, kk + fk)];
+  };
+
   const int nk = (K + BK - 1) / BK;
   load(0);
+  store(0);
+  __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
-    __syncthreads();
-    store();
-    __syncthreads();
+    const int cur = kt & 1;
     if (kt + 1 < nk) load((kt + 1) * BK);
+    if constexpr (sizeof(T) == 4) {
+      // FP32: the 32-cycle MFMAs cannot hide a fragment read, so fragments for
+      // step s+1 are fetched while the MFMAs of step s issue
+      frags(cur, 0, 0);
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      T af[4], bf[4];
+      for (int s = 0; s < BK / 4; ++s) {
+        if (s + 1 < BK / 4) frags(cur, (s + 1) * 4, (s + 1) & 1);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[kr][wm * 64 + i * 16 + (lane & 15)];
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Bs[kr][wn * 64 + j * 16 + (lane & 15)];
+          for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[s & 1][j], af[s & 1][i], acc[i][j]);
+      }
+    } else {
+      // FP64: sixteen 64-cycle MFMAs per step cover the fragment latency
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int s = 0; s < BK / 4; ++s) {
+        frags(cur, s * 4, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[j], af[i], acc[i][j]);
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[0][j], af[0][i], acc[i][j]);
+      }
     }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
   }
 
   // acc[i][j][r] holds C(m, n) with m = lane%16 of block i, n = Mfma<T>::row(lane, r) of block j.
@@ -248,6 +323,9 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
   KTimer timer(ctx, name, flops, bytes);
+  // the buffer-load path addresses a tile's k-slice with 32-bit byte offsets
+  const long span = (long)sizeof(T) * (std::max<long>(BM, BN) + 64) * (std::max(lda, ldb) + 1);
+  const int fast_ok = span < (1L << 31) ? 1 : 0;
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     const int nb = std::min(batch - b0, 65535);
     dim3 grid((unsigned)(tiles_m * tiles_n), 1, nb);
@@ -256,7 +334,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
     T* Cb = C + (long)b0 * sC;
 #define JUES_GEMM_LAUNCH(TA_, TB_, K_)                                                             \
   hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, \
-                     alpha, Ab, lda, sA, Bb, ldb, sB, beta, Cb, ldc, sC, (int)tiles_m, (int)tiles_n)
+                     alpha, Ab, lda, sA, Bb, ldb, sB, beta, Cb, ldc, sC, (int)tiles_m, (int)tiles_n, fast_ok)
     if (!ta && !tb) {
       if (kind == 1) JUES_GEMM_LAUNCH(false, false, 1);
       else if (kind == 2) JUES_GEMM_LAUNCH(false, false, 2);
