@@ -51,7 +51,8 @@ typedef enum {
     JUES_CCSD_RCCSD = 1    /* CoupledCluster/RCCSD.jl */
 } jues_ccsd_eqs;
 
-/* Initial T2 guess. */
+/* Initial T2 guess.  MP2 and OVOV are the first-order amplitudes of the equations
+ * iterated first, so under coupling continuation they carry that lambda (c/N). */
 typedef enum {
     JUES_GUESS_ZERO = 0,   /* t = 0, as mRCCD.jl:245 */
     JUES_GUESS_MP2 = 1,    /* t = <ij|ab>/D, as mRCCSD.jl:66 */

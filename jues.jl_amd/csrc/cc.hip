@@ -436,15 +436,16 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   S.t.alloc(S.Nd * sizeof(double));
   double* t = S.t.as<double>();
   fill<double>(ctx, t, S.Nd, 0.0);
+  // first-order amplitudes of the equations being iterated, whose integrals carry lambda
   switch (opts.guess) {
     case JUES_GUESS_ZERO: break;
     case JUES_GUESS_MP2:
       JUES_HIP(hipMemcpyAsync(t, S.i64->Kpp.p, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      divide_d2(ctx, t, S.eps.as<double>(), S.o, S.v);
+      divide_d2(ctx, t, S.eps.as<double>(), S.o, S.v, S.lambda);
       break;
     case JUES_GUESS_OVOV:
       JUES_HIP(hipMemcpyAsync(t, S.guess_buf.p, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      divide_d2(ctx, t, S.eps.as<double>(), S.o, S.v);
+      divide_d2(ctx, t, S.eps.as<double>(), S.o, S.v, S.lambda);
       break;
     case JUES_GUESS_USER:
       h2d(ctx, t, T2_init, N2 * sizeof(double));

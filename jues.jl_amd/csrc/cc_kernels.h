@@ -76,6 +76,6 @@ void reduce_rows(jues_ctx* ctx, const double* part, long PB, long count, int nq,
 void diis_solve(jues_ctx* ctx, double* small, const DiisOrder& ord);
 void diis_extrap(jues_ctx* ctx, const double* vals, long stride, const DiisOrder& ord, const double* small, long n,
                  double* t);
-void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v);
+void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v, double scale);
 
 }  // namespace jues

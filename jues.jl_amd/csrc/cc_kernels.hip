@@ -267,12 +267,12 @@ __global__ __launch_bounds__(256) void diis_extrap_kernel(const double* __restri
 
 // t[ijab] /= D[ijab]
 __global__ __launch_bounds__(256) void divide_d2_kernel(double* __restrict__ t, const double* __restrict__ eps, int o,
-                                                        int v) {
+                                                        int v, double scale) {
   const long oo = (long)o * o, n = oo * v * v;
   for (long x = blockIdx.x * 256L + threadIdx.x; x < n; x += (long)gridDim.x * 256L) {
     const long ij = x % oo, ab = x / oo;
     const int i = (int)(ij % o), j = (int)(ij / o), a = (int)(ab % v), b = (int)(ab / v);
-    t[x] /= eps[i] + eps[j] - eps[o + a] - eps[o + b];
+    t[x] = scale * t[x] / (eps[i] + eps[j] - eps[o + a] - eps[o + b]);
   }
 }
 
@@ -331,9 +331,9 @@ void diis_extrap(jues_ctx* ctx, const double* vals, long stride, const DiisOrder
   JUES_HIP(hipGetLastError());
 }
 
-void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v) {
+void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v, double scale) {
   const long n = (long)o * o * v * v;
-  hipLaunchKernelGGL(divide_d2_kernel, dim3(grid256(n)), dim3(256), 0, ctx->stream, t, eps, o, v);
+  hipLaunchKernelGGL(divide_d2_kernel, dim3(grid256(n)), dim3(256), 0, ctx->stream, t, eps, o, v, scale);
   JUES_HIP(hipGetLastError());
 }
 
