@@ -1,12 +1,15 @@
-"""RCCD iteration benchmark on one MI355X: FP32-mixed vs plain FP64.
+"""RCCD iteration benchmark on one MI355X: FP32-mixed and split-FP16 vs plain FP64.
 
 Workload: closed-shell RCCD at o=60, v=340 (n=400, naux=1200) on seeded
 synthetic B-factor integrals; every MO integral block (including the packed
 v^4 ladder block) and the amplitudes are resident in HBM before timing.  A
 "step" is one full CC iteration (all contractions, the fused update, DIIS).
 The FP32-mixed stage (FP32 MFMA contractions, FP64 update/energy/DIIS) is timed
-first, then the DIIS history is reset and the plain FP64 stage is timed from the
-same amplitudes.  `value` is the FP32-mixed seconds per iteration.
+first, then the split-FP16 stage (ring and pp-ladder products on FP16 MFMA with
+hi+lo operand splitting, FP32 accumulation), then the plain FP64 stage, each from
+the amplitudes the previous one left and with the DIIS history reset.  `value` is
+the FP32-mixed seconds per iteration; the split-FP16 figures are reported beside
+it under "fp16x3".
 
 The synthetic integrals at o=60, v=340 are so strongly coupled that the CC
 iteration has no convergent solution at full strength (E_MP2 ~ -1e4 Eh; DIIS and
@@ -15,7 +18,8 @@ integral, so the run uses the equations for integrals scaled by a coupling
 lambda (default 0.005, reported in `config`), where the iteration converges; all
 shapes, flops and bytes are those of the full workload and the cost of an
 iteration does not depend on the values.  `--converge` runs both forms to
-convergence and reports |dE| (FP32-mixed + FP64 refinement vs plain FP64).
+convergence and reports |dE| (FP32-mixed or split-FP16 + FP64 refinement vs
+plain FP64).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg4_rccd_n400]
                        [--coupling 0.005] [--converge] [--no-cpu]
@@ -33,7 +37,7 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "jues.jl_amd"))
 
 METRIC = "RCCD s/iter + MFMA TFLOP/s, FP32-mixed vs FP64, |ΔE|≤1e-8 Eh, o=60 v=340"
-PEAK = {"f64": 78.6, "f32": 157.3}  # dense MFMA TFLOP/s (MI355X datasheet)
+PEAK = {"f64": 78.6, "f32": 157.3, "f16": 2500.0}  # dense MFMA TFLOP/s (MI355X datasheet)
 
 
 def f_rccd(o: int, v: int) -> float:
@@ -51,6 +55,20 @@ def roofline(stats: dict, prec: str) -> dict:
     return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[prec],
             "unit": "TFLOP/s", "frac": round(achieved / PEAK[prec], 4), "traffic": None,
             "flops_per_launch": per_launch_flops, "avg_launch_ms": round(avg_s * 1e3, 3),
+            "launches": int(s["calls"])}
+
+
+def roofline_split(stats: dict) -> dict:
+    """Roofline of the dominant split-FP16 GEMM: each FP32-equivalent flop executes
+    three FP16 MFMA products, priced against the dense FP16 peak."""
+    g = {k: s for k, s in stats.items() if s["flops"] > 0 and k in ("cc_ring", "cc_pp_ladder")}
+    name, s = max(g.items(), key=lambda kv: kv[1]["ms"])
+    avg_s = s["ms"] / s["calls"] / 1e3
+    eq = s["flops"] / s["calls"] / avg_s / 1e12
+    return {"kernel": name, "bound": "mfma", "achieved": round(3 * eq, 2), "peak": PEAK["f16"], "unit": "TFLOP/s",
+            "frac": round(3 * eq / PEAK["f16"], 4), "fp32_equivalent_tflops": round(eq, 2),
+            "fp32_equivalent_frac_of_f32_peak": round(eq / PEAK["f32"], 4), "traffic": None,
+            "flops_per_launch": s["flops"] / s["calls"], "avg_launch_ms": round(avg_s * 1e3, 3),
             "launches": int(s["calls"])}
 
 
@@ -86,9 +104,8 @@ def main():
           file=sys.stderr, flush=True)
 
     res = {}
-    for stage, prec in (("fp32", "fp32_mixed"), ("fp64", "fp64")):
-        if stage == "fp64":
-            solver.reset_diis()
+    for stage, prec in (("fp32", "fp32_mixed"), ("fp16x3", "fp16x3"), ("fp64", "fp64")):
+        solver.reset_diis()
         if args.warmup:
             solver.iterate(args.warmup, prec=prec)
         ctx.stats_reset(True)
@@ -102,7 +119,7 @@ def main():
               f"|dt|={info['rnorm']:.3e}", file=sys.stderr, flush=True)
 
     F = f_rccd(o, v)
-    m, d = res["fp32"], res["fp64"]
+    m, d, h = res["fp32"], res["fp64"], res["fp16x3"]
     executed = lambda st: sum(x["flops"] for x in st.values())  # noqa: E731
     out = {
         "metric": METRIC,
@@ -127,9 +144,14 @@ def main():
         "tflops_executed": {"fp32_mixed": round(executed(m["stats"]) / args.steps / m["s_per_iter"] / 1e12, 2),
                             "fp64": round(executed(d["stats"]) / args.steps / d["s_per_iter"] / 1e12, 2),
                             "flops_per_iter": executed(m["stats"]) / args.steps},
+        "fp16x3": {"s_per_iter": round(h["s_per_iter"], 4),
+                   "speedup_vs_fp64": round(d["s_per_iter"] / h["s_per_iter"], 3),
+                   "tflops_canonical": round(F / h["s_per_iter"] / 1e12, 2),
+                   "roofline": roofline_split(h["stats"])},
         "roofline": roofline(m["stats"], "f32"),
         "roofline_fp64": roofline(d["stats"], "f64"),
         "kernel_ms_per_iter": {"fp32": {k: round(x["ms"] / args.steps, 2) for k, x in m["stats"].items()},
+                               "fp16x3": {k: round(x["ms"] / args.steps, 2) for k, x in h["stats"].items()},
                                "fp64": {k: round(x["ms"] / args.steps, 2) for k, x in d["stats"].items()}},
         "setup_s": round(t_setup, 2),
     }
@@ -170,6 +192,23 @@ def converge(w, coupling: float) -> dict:
                         time_to_solution_s=round(t32 + tref, 2), final_rnorm=i64["rnorm"], final_dE=i64["delta_e"])
     del sm
     t0 = time.perf_counter()
+    sh = Solver(w, method="rccd", prec="fp16x3", **kw)
+    t0 = time.perf_counter()
+    i16, _ = sh.iterate(200, prec="fp16x3", stop_on_conv=True)
+    t16 = time.perf_counter() - t0
+    E16 = sh.result()[0]
+    sh.reset_diis()
+    t0 = time.perf_counter()
+    j64, cj = sh.iterate(100, prec="fp64", stop_on_conv=True)
+    trh = time.perf_counter() - t0
+    Eh = sh.result()[0]
+    print(f"[bench] fp16x3: split {i16['iter']} it {t16:.1f} s E={E16:.12f}; refine {j64['iter'] - i16['iter']} it "
+          f"{trh:.1f} s E={Eh:.12f} conv={cj}", file=sys.stderr, flush=True)
+    res["fp16x3"] = dict(split_iters=i16["iter"], split_s=round(t16, 2), E_after_split=E16,
+                         refine_iters=j64["iter"] - i16["iter"], refine_s=round(trh, 2), E=Eh, converged=cj,
+                         time_to_solution_s=round(t16 + trh, 2), final_rnorm=j64["rnorm"], final_dE=j64["delta_e"])
+    del sh
+    t0 = time.perf_counter()
     sd = Solver(w, method="rccd", prec="fp64", **kw)
     t0 = time.perf_counter()
     d64, cd = sd.iterate(200, prec="fp64", stop_on_conv=True)
@@ -180,6 +219,8 @@ def converge(w, coupling: float) -> dict:
                        final_rnorm=d64["rnorm"], final_dE=d64["delta_e"])
     res["abs_dE_mixed_vs_fp64"] = abs(Em - Ed)
     res["abs_dE_fp32_stage_vs_fp64"] = abs(E32 - Ed)
+    res["abs_dE_fp16x3_vs_fp64"] = abs(Eh - Ed)
+    res["abs_dE_split_stage_vs_fp64"] = abs(E16 - Ed)
     res["coupling"] = coupling
     res["setup_s"] = round(t_setup, 2)
     return res

@@ -39,8 +39,12 @@ typedef enum {
 
 typedef enum {
     JUES_FP64 = 0,        /* plain form: every contraction in FP64 MFMA */
-    JUES_FP32_MIXED = 1   /* contractions in FP32 MFMA (FP32 accumulate); energies,
+    JUES_FP32_MIXED = 1,  /* contractions in FP32 MFMA (FP32 accumulate); energies,
                              denominators, DIIS and the final refinement in FP64 */
+    JUES_FP16X3_MIXED = 2 /* as FP32_MIXED, but the ring, particle-particle ladder and
+                             hole-hole GEMMs run on FP16 MFMA with each FP32 operand split
+                             into hi + lo halves under a power-of-two scale (three products,
+                             FP32 accumulate; ~2^-21 relative per product) */
 } jues_prec;
 
 /* Which reference RCCSD equations to follow.  They differ only in the T1 exchange
@@ -71,8 +75,8 @@ typedef struct {
     int32_t guess;            /* jues_guess, default JUES_GUESS_MP2 */
     int32_t ccsd_eqs;         /* jues_ccsd_eqs, default JUES_CCSD_MRCCSD */
     int32_t fcn;              /* frozen core: drop the first fcn occupied orbitals (get_eri's fcn) */
-    /* FP32_MIXED only: the FP32 stage stops once ||dt||_2 < fp32_r_conv (or it stalls);
-     * FP64 refinement iterations then run to e_conv / r_conv, at most refine_maxit. */
+    /* FP32_MIXED / FP16X3_MIXED: the fast stage stops once ||dt||_2 < fp32_r_conv (or it
+     * stalls); FP64 refinement iterations then run to e_conv / r_conv, at most refine_maxit. */
     double fp32_r_conv;       /* default 1e-5 */
     int32_t refine_maxit;     /* default 60 */
     int32_t verbose;          /* print one line per iteration to stderr */
@@ -89,7 +93,7 @@ typedef struct {
 /* Per-iteration record passed to the optional callback. */
 typedef struct {
     int32_t iter;             /* 1-based */
-    int32_t stage;            /* 0 = FP64, 1 = FP32 (mixed form) */
+    int32_t stage;            /* 0 = FP64, 1 = FP32 (FP32_MIXED), 2 = split-FP16 (FP16X3_MIXED) */
     double energy;            /* correlation energy of the Jacobi update t_new */
     double delta_e;           /* energy - previous energy */
     double rnorm;             /* ||t_new - t_old||_2 (T1 and T2 together for CCSD) */
@@ -185,7 +189,8 @@ int jues_kernel_stats(jues_ctx* ctx, const char** names, double* ms, double* cal
 void jues_kernel_stats_reset(jues_ctx* ctx, int enable);
 
 /* Raw MFMA GEMM (column-major): C = alpha*op(A)*op(B) + beta*C, on device copies of
- * host arrays.  Exposed for the GEMM unit tests; prec selects FP64 or FP32 MFMA. */
+ * host arrays.  Exposed for the GEMM unit tests; prec selects FP64 or FP32 MFMA, or
+ * JUES_FP16X3_MIXED for the split-FP16 GEMM (transa 'T', transb 'N' only). */
 jues_status jues_gemm_test(jues_ctx* ctx, int prec, char transa, char transb, int M, int N,
                            int K, double alpha, const double* A, int lda, const double* B,
                            int ldb, double beta, double* C, int ldc);

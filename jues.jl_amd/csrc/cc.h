@@ -28,6 +28,22 @@ struct CCWork {
   DevBuf t1, Fme, X2, Z, Q, n1, U2, Tau;
 };
 
+// Split-FP16 operand planes of the fast stage.  Slots below SPLIT_CONST hold
+// per-iteration operands (re-split on their first use in an iteration); the
+// others hold integral matrices, split once.
+enum SplitSlot {
+  SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT,
+  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SPLIT_SLOTS
+};
+struct SplitState {
+  struct Entry {
+    Split16Buf buf;
+    const void* src = nullptr;
+    long stamp = -2;
+  };
+  Entry e[SPLIT_SLOTS];
+};
+
 struct CCState {
   int method = 0;  // 0 RCCD, 1 RCCSD
   int prec = 0;
@@ -49,6 +65,8 @@ struct CCState {
   std::unique_ptr<CCInts<float>> i32;
   std::unique_ptr<CCWork<double>> w64;
   std::unique_ptr<CCWork<float>> w32;
+  std::unique_ptr<SplitState> sp;
+  bool split_on = false;  // the FP32 stage runs its ring / pp-ladder products on split-FP16 MFMA
   hipEvent_t ev[2] = {nullptr, nullptr};
   double lambda = 1.0;  // coupling of the equations being iterated
   double E_prev = 0;
@@ -69,6 +87,14 @@ T* bufp_t(DevBuf& b, size_t n) {
   if (b.bytes < n * sizeof(T)) b.alloc(n * sizeof(T));
   return b.as<T>();
 }
+
+// Ring product C = alpha A B + beta C over (ov) x (ov) matrices whose left
+// operand is symmetric; slotA/slotB name the operands for the split-FP16 stage.
+template <typename T>
+void ring_mm(jues_ctx* ctx, CCState& S, T alpha, const T* A, int slotA, const T* B, int slotB, T beta, T* C);
+// Packed pp-ladder Lp/Lm from FP64 amplitudes `amp` (ijab); Tp/Tm are workspaces.
+template <typename T>
+void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm);
 
 template <typename T>
 void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2);

@@ -161,8 +161,7 @@ void cc_make_ints32(jues_ctx* ctx, CCState& S) {
   convert_buf<float>(ctx, a.Kpp, b.Kpp);
   convert_buf<float>(ctx, a.Ka, b.Ka);
   convert_buf<float>(ctx, a.oooo, b.oooo);
-  convert_buf<float>(ctx, a.Vp, b.Vp);
-  convert_buf<float>(ctx, a.Vm, b.Vm);
+  // Vp/Vm are converted by the first FP32 (non-split) ladder product
   convert_buf<float>(ctx, a.ooov_c, b.ooov_c);
   convert_buf<float>(ctx, a.ovvv_c, b.ovvv_c);
   sync(ctx);
@@ -189,6 +188,90 @@ template <>
 CCInts<double>& ints<double>(CCState& S) { return *S.i64; }
 template <>
 CCInts<float>& ints<float>(CCState& S) { return *S.i32; }
+
+// ---------------------------------------------------------------------------
+// ring and pp-ladder products (FP64, FP32, or split-FP16 in the fast stage)
+// ---------------------------------------------------------------------------
+namespace {
+
+SplitState& split_state(CCState& S) {
+  if (!S.sp) S.sp.reset(new SplitState());
+  return *S.sp;
+}
+
+// planes of operand X (rows x cols, leading dimension rows) in `slot`; per-iteration
+// slots are split on their first use in an iteration, constant slots once
+template <typename Ti>
+const Split16& split_operand(jues_ctx* ctx, CCState& S, int slot, const Ti* X, long rows, long cols) {
+  SplitState::Entry& e = split_state(S).e[slot];
+  const long stamp = slot < SPLIT_CONST ? (long)S.iter : -1;
+  if (e.src != static_cast<const void*>(X) || e.stamp != stamp) {
+    split16<Ti>(ctx, X, rows, cols, rows, e.buf, "cc_split");
+    e.src = X;
+    e.stamp = stamp;
+  }
+  return e.buf.s;
+}
+
+}  // namespace
+
+template <>
+void ring_mm<double>(jues_ctx* ctx, CCState& S, double alpha, const double* A, int, const double* B, int, double beta,
+                     double* C) {
+  const long ov = (long)S.o * S.v;
+  gemm<double>(ctx, false, false, ov, ov, ov, alpha, A, ov, 0, B, ov, 0, beta, C, ov, 0, 1, "cc_ring");
+}
+
+template <>
+void ring_mm<float>(jues_ctx* ctx, CCState& S, float alpha, const float* A, int slotA, const float* B, int slotB,
+                    float beta, float* C) {
+  const long ov = (long)S.o * S.v;
+  if (!S.split_on) {
+    gemm<float>(ctx, false, false, ov, ov, ov, alpha, A, ov, 0, B, ov, 0, beta, C, ov, 0, 1, "cc_ring");
+    return;
+  }
+  // A is symmetric, so A B = A^T B: its planes serve as the TN kernel's left operand
+  const Split16& a = split_operand<float>(ctx, S, slotA, A, ov, ov);
+  const Split16& b = split_operand<float>(ctx, S, slotB, B, ov, ov);
+  gemm16_tn(ctx, ov, ov, alpha, a, b, beta, C, ov, "cc_ring");
+}
+
+template <typename T>
+void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm) {
+  const PairTables& P = S.P;
+  const bool minus = P.nmo > 0 && P.nmv > 0;
+  if constexpr (sizeof(T) == 4) {
+    if (S.split_on) {
+      // Lp[(ij),(ab)] = sum_(ef) Tp[(ij),(ef)] Vp[(ef),(ab)]: Tp packed transposed is the
+      // k-contiguous left operand; Vp's planes come straight from the FP64 integrals
+      pack_t<float>(ctx, P, amp, S.o, S.v, Tp, Tm, true);
+      const Split16& tp = split_operand<float>(ctx, S, SP_TPT, Tp, P.npv, P.npo);
+      const Split16& vp = split_operand<double>(ctx, S, SP_VP, S.i64->Vp.as<double>(), P.npv, P.npv);
+      gemm16_tn(ctx, P.npo, P.npv, 1.0f, tp, vp, 0.0f, Lp, P.npo, "cc_pp_ladder");
+      if (minus) {
+        const Split16& tm = split_operand<float>(ctx, S, SP_TMT, Tm, P.nmv, P.nmo);
+        const Split16& vm = split_operand<double>(ctx, S, SP_VM, S.i64->Vm.as<double>(), P.nmv, P.nmv);
+        gemm16_tn(ctx, P.nmo, P.nmv, 1.0f, tm, vm, 0.0f, Lm, P.nmo, "cc_pp_ladder");
+      }
+      return;
+    }
+    CCInts<float>& b = *S.i32;
+    if (!b.Vp.p) {
+      convert_buf<float>(ctx, S.i64->Vp, b.Vp);
+      convert_buf<float>(ctx, S.i64->Vm, b.Vm);
+    }
+  }
+  CCInts<T>& I = ints<T>(S);
+  pack_t<T>(ctx, P, amp, S.o, S.v, Tp, Tm);
+  gemm<T>(ctx, false, false, P.npo, P.npv, P.npv, T(1), Tp, P.npo, 0, I.Vp.template as<T>(), P.npv, 0, T(0), Lp,
+          P.npo, 0, 1, "cc_pp_ladder");
+  if (minus)
+    gemm<T>(ctx, false, false, P.nmo, P.nmv, P.nmv, T(1), Tm, P.nmo, 0, I.Vm.template as<T>(), P.nmv, 0, T(0), Lm,
+            P.nmo, 0, 1, "cc_pp_ladder");
+}
+
+template void ladder_mm<double>(jues_ctx*, CCState&, const double*, double*, double*, double*, double*);
+template void ladder_mm<float>(jues_ctx*, CCState&, const double*, float*, float*, float*, float*);
 
 // Doubles-residual contractions that depend only on the doubles amplitudes `tau2`
 // (t2 for RCCD; the CCSD driver passes its own amplitudes and adds T1 terms).
@@ -228,7 +311,6 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   permute<T, T>(ctx, Tpp, d4, {0, 3, 1, 2}, Ttov, 1.0, 0.0, "cc_prep");   // t[i,j,b,a] at [i,a,j,b]
   permute<T, T>(ctx, Upp, d4, {0, 2, 1, 3}, Uov, 1.0, 0.0, "cc_prep");
   permute<T, T>(ctx, Upp, d4, {2, 0, 1, 3}, Ua, 1.0, 0.0, "cc_prep");     // U[a,m,n,f]
-  pack_t<T>(ctx, S.P, t2, o, v, Tp, Tm);
 
   // Fae[a,e] = -sum U[a,(mnf)] K[e,(mnf)];  Fmi[m,i] = sum K[m,(nef)] U[i,(nef)]
   gemm<T>(ctx, false, true, v, v, oo * v, T(-1), Ua, v, 0, I.Ka.template as<T>(), v, 0, zero, Fae, v, 0, 1, "cc_fock");
@@ -238,24 +320,20 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   gemm<T>(ctx, false, true, oo, oo, vv, one, I.Kpp.template as<T>(), oo, 0, Tpp, oo, 0, one, Wp, oo, 0, 1, "cc_hh");
   gemm<T>(ctx, true, false, oo, vv, oo, one, Wp, oo, 0, Tpp, oo, 0, zero, Lhh, oo, 0, 1, "cc_hh");
   // particle-particle ladder on the packed (anti)symmetric parts
-  gemm<T>(ctx, false, false, S.P.npo, S.P.npv, S.P.npv, one, Tp, S.P.npo, 0, I.Vp.template as<T>(), S.P.npv, 0, zero,
-          Lp, S.P.npo, 0, 1, "cc_pp_ladder");
-  if (S.P.nmo > 0 && S.P.nmv > 0)
-    gemm<T>(ctx, false, false, S.P.nmo, S.P.nmv, S.P.nmv, one, Tm, S.P.nmo, 0, I.Vm.template as<T>(), S.P.nmv, 0,
-            zero, Lm, S.P.nmo, 0, 1, "cc_pp_ladder");
-  // ring terms
+  ladder_mm<T>(ctx, S, t2, Tp, Tm, Lp, Lm);
+  // ring terms (every left operand is a symmetric (ov) x (ov) matrix)
   const T* K = I.Kov.template as<T>();
   const T* Kx = I.Ktov.template as<T>();
   JUES_HIP(hipMemcpyAsync(WJ, K, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  gemm<T>(ctx, false, false, ov, ov, ov, half, K, ov, 0, Uov, ov, 0, one, WJ, ov, 0, 1, "cc_ring");
-  gemm<T>(ctx, false, false, ov, ov, ov, -half, Kx, ov, 0, Tov, ov, 0, one, WJ, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, half, K, SP_K, Uov, SP_UOV, one, WJ);
+  ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, one, WJ);
   permute<T, T>(ctx, I.Jov.template as<T>(), {ov, ov}, {0, 1}, WE, -1.0, 0.0, "cc_prep");
-  gemm<T>(ctx, false, false, ov, ov, ov, half, Kx, ov, 0, Ttov, ov, 0, one, WE, ov, 0, 1, "cc_ring");
-  gemm<T>(ctx, false, false, ov, ov, ov, one, Tov, ov, 0, WE, ov, 0, zero, Y, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, half, Kx, SP_KX, Ttov, SP_TTOV, one, WE);
+  ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, Y);
   JUES_HIP(hipMemcpyAsync(H, Y, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  gemm<T>(ctx, false, false, ov, ov, ov, one, Uov, ov, 0, WJ, ov, 0, one, H, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
-  gemm<T>(ctx, false, false, ov, ov, ov, one, Ttov, ov, 0, WE, ov, 0, zero, Y, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
   // Fock-like terms: H += t[ia j, e] Fae[b,e] ; H -= Fmi[m,i] t[m, a jb]
   gemm<T>(ctx, false, true, oo * v, v, v, one, Tov, oo * v, 0, Fae, v, 0, one, H, oo * v, 0, 1, "cc_fock");
   gemm<T>(ctx, true, false, o, ov * v, o, T(-1), Fmi, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
@@ -374,7 +452,7 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
 
   S.iter += 1;
   info->iter = S.iter;
-  info->stage = sizeof(T) == 4 ? 1 : 0;
+  info->stage = sizeof(T) == 4 ? (S.split_on ? 2 : 1) : 0;
   info->energy = h[0];
   info->delta_e = S.have_prev ? h[0] - S.E_prev : h[0];
   info->rnorm = std::sqrt(std::max(0.0, h[1]));
@@ -396,7 +474,7 @@ template void cc_iteration<float>(jues_ctx*, CCState&, jues_iter_info*);
 void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, const double* Cocc, int o,
               const double* Cvir, int v, int n, const double* eps, const double* T1_init, const double* T2_init) {
   JUES_REQUIRE(method == 0 || method == 1, "cc_setup: method must be 0 (RCCD) or 1 (RCCSD)");
-  JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED, "cc_setup: bad precision");
+  JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED || prec == JUES_FP16X3_MIXED, "cc_setup: bad precision");
   JUES_REQUIRE(Cocc && Cvir && eps && o > 0 && v > 0, "cc_setup: bad arguments");
   JUES_REQUIRE(n == ctx->n && n > 0, "cc_setup: n does not match the AO integrals");
   jues_cc_opts opts;
@@ -458,7 +536,7 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   S.errs.alloc((size_t)dm * S.Nd * sizeof(double));
   S.small.alloc(SMALL_SIZE * sizeof(double));
   fill<double>(ctx, S.small.as<double>(), SMALL_SIZE, 0.0);
-  if (prec == JUES_FP32_MIXED) cc_make_ints32(ctx, S);
+  if (prec != JUES_FP64) cc_make_ints32(ctx, S);
   S.E_prev = cc_energy(ctx, S);
   S.have_prev = true;
   S.iter = 0;
@@ -475,17 +553,19 @@ double cc_energy(jues_ctx* ctx, CCState& S) {
 void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_conv, jues_iter_info* last,
             bool* converged, jues_iter_cb cb, void* user) {
   *converged = false;
-  if (stage_prec == JUES_FP32_MIXED) cc_make_ints32(ctx, S);
+  if (stage_prec != JUES_FP64) cc_make_ints32(ctx, S);
+  S.split_on = stage_prec == JUES_FP16X3_MIXED;
   int stall = 0;
   double best = 1e300;
+  static const char* stage_name[3] = {"fp64", "fp32", "fp16x3"};
   for (int k = 0; k < nsteps; ++k) {
     jues_iter_info info{};
-    if (stage_prec == JUES_FP32_MIXED) cc_iteration<float>(ctx, S, &info);
+    if (stage_prec != JUES_FP64) cc_iteration<float>(ctx, S, &info);
     else cc_iteration<double>(ctx, S, &info);
     if (cb) cb(user, &info);
     if (S.opts.verbose)
       std::fprintf(stderr, "  cc iter %3d %s E=% .15f dE=% .3e |dt|=%.3e diis=%d %.1f ms\n", info.iter,
-                   info.stage ? "fp32" : "fp64", info.energy, info.delta_e, info.rnorm, info.diis_size, info.ms);
+                   stage_name[info.stage], info.energy, info.delta_e, info.rnorm, info.diis_size, info.ms);
     *last = info;
     if (stage_prec == JUES_FP64) {
       if (std::fabs(info.delta_e) < S.opts.e_conv && info.rnorm < S.opts.r_conv) {
@@ -532,7 +612,7 @@ jues_status cc_solve(jues_ctx* ctx, int method, int prec, const jues_cc_opts* op
     if (prec == JUES_FP64) {
       cc_run(ctx, S, JUES_FP64, S.opts.maxit, true, &last, &conv, cb, user);
     } else {
-      cc_run(ctx, S, JUES_FP32_MIXED, S.opts.maxit, true, &last, &conv, cb, user);
+      cc_run(ctx, S, prec, S.opts.maxit, true, &last, &conv, cb, user);
       if (k == nl) {
         cc_reset_diis(S);
         cc_run(ctx, S, JUES_FP64, S.opts.refine_maxit, true, &last, &conv, cb, user);
@@ -583,7 +663,8 @@ jues_status jues_cc_iterate(jues_ctx* ctx, int stage_prec, int nsteps, int stop_
                             int* converged) {
   JUES_API_BEGIN(ctx)
   JUES_REQUIRE(ctx->cc, "jues_cc_iterate: call jues_cc_setup first");
-  JUES_REQUIRE(stage_prec == JUES_FP64 || stage_prec == JUES_FP32_MIXED, "jues_cc_iterate: bad precision");
+  JUES_REQUIRE(stage_prec == JUES_FP64 || stage_prec == JUES_FP32_MIXED || stage_prec == JUES_FP16X3_MIXED,
+               "jues_cc_iterate: bad precision");
   jues_iter_info last{};
   bool conv = false;
   cc_run(ctx, *ctx->cc, stage_prec, nsteps, stop_on_conv != 0, &last, &conv, nullptr, nullptr);

@@ -118,7 +118,6 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   permute<double, T>(ctx, t2, d4, {0, 3, 1, 2}, Ttov, 1.0, 0.0, "cc_prep");
   permute<T, T>(ctx, Tov, {ov, ov}, {0, 1}, Uov, 2.0, 0.0, "cc_prep");
   permute<T, T>(ctx, Ttov, {ov, ov}, {0, 1}, Uov, -1.0, 1.0, "cc_prep");
-  pack_t<T>(ctx, S.P, tau, o, v, Tp, Tm);
 
   // ---- Fae, Fmi, Fme --------------------------------------------------------------
   gemm<T>(ctx, false, true, v, v, oo * v, T(-1), Ua, v, 0, I.Ka.template as<T>(), v, 0, zero, Fae, v, 0, 1, "cc_fock");
@@ -161,11 +160,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   gemm<T>(ctx, true, false, oo, vv, oo, one, Wp, oo, 0, Tpp, oo, 0, zero, Lhh, oo, 0, 1, "cc_hh");
 
   // ---- particle-particle ladder on tau ------------------------------------------------
-  gemm<T>(ctx, false, false, S.P.npo, S.P.npv, S.P.npv, one, Tp, S.P.npo, 0, I.Vp.template as<T>(), S.P.npv, 0, zero,
-          Lp, S.P.npo, 0, 1, "cc_pp_ladder");
-  if (S.P.nmo > 0 && S.P.nmv > 0)
-    gemm<T>(ctx, false, false, S.P.nmo, S.P.nmv, S.P.nmv, one, Tm, S.P.nmo, 0, I.Vm.template as<T>(), S.P.nmv, 0,
-            zero, Lm, S.P.nmo, 0, 1, "cc_pp_ladder");
+  ladder_mm<T>(ctx, S, tau, Tp, Tm, Lp, Lm);
   // Wabef's T1 part: X2 = Lz + Lz^P,  Lz[ijab] = -sum_m t1[mb] Z[ij,ma],  Z = tau . (mf|ae)
   contract<T>(ctx, 1.0, TN(Tpp, "ijef"), TN(ovvv, "mfae"), 0.0, TN(Z, "ijma"), "cc_ovvv");
   contract<T>(ctx, -1.0, TN(t1, "mb"), TN(Z, "ijma"), 0.0, TN(Q, "ijab"), "cc_t1");
@@ -179,19 +174,20 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   permute<T, T>(ctx, U2, {ov, ov}, {0, 1}, Q, 2.0, 1.0, "cc_prep");
   permute<T, T>(ctx, Uov, {ov, ov}, {0, 1}, U2, 1.0, -2.0, "cc_prep");
   JUES_HIP(hipMemcpyAsync(WJ, Kov, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  gemm<T>(ctx, false, false, ov, ov, ov, half, Kov, ov, 0, U2, ov, 0, one, WJ, ov, 0, 1, "cc_ring");
-  gemm<T>(ctx, false, false, ov, ov, ov, -half, Kx, ov, 0, Tov, ov, 0, one, WJ, ov, 0, 1, "cc_ring");
+  // every left operand of the ring products is a symmetric (ov) x (ov) matrix
+  ring_mm<T>(ctx, S, half, Kov, SP_K, U2, SP_U2, one, WJ);
+  ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, one, WJ);
   contract<T>(ctx, 1.0, TN(t1, "jf"), TN(ovvv, "mebf"), 1.0, TN(WJ, "mejb"), "cc_t1");
   contract<T>(ctx, -1.0, TN(t1, "nb"), TN(ooov, "njme"), 1.0, TN(WJ, "mejb"), "cc_t1");
   permute<T, T>(ctx, Jov, {ov, ov}, {0, 1}, WE, -1.0, 0.0, "cc_prep");
-  gemm<T>(ctx, false, false, ov, ov, ov, half, Kx, ov, 0, Q, ov, 0, one, WE, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, half, Kx, SP_KX, Q, SP_Q, one, WE);
   contract<T>(ctx, -1.0, TN(t1, "jf"), TN(ovvv, "mfbe"), 1.0, TN(WE, "mejb"), "cc_t1");
   contract<T>(ctx, 1.0, TN(t1, "nb"), TN(ooov, "mjne"), 1.0, TN(WE, "mejb"), "cc_t1");
-  gemm<T>(ctx, false, false, ov, ov, ov, one, Tov, ov, 0, WE, ov, 0, zero, Y, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, Y);
   JUES_HIP(hipMemcpyAsync(H, Y, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  gemm<T>(ctx, false, false, ov, ov, ov, one, Uov, ov, 0, WJ, ov, 0, one, H, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
-  gemm<T>(ctx, false, false, ov, ov, ov, one, Ttov, ov, 0, WE, ov, 0, zero, Y, ov, 0, 1, "cc_ring");
+  ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
   // Fock-like terms with Fb / Fj
   gemm<T>(ctx, false, true, oo * v, v, v, one, Tov, oo * v, 0, Fae, v, 0, one, H, oo * v, 0, 1, "cc_fock");
   gemm<T>(ctx, true, false, o, ov * v, o, T(-1), Fmi, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");

@@ -1,0 +1,349 @@
+// Split-FP16 MFMA GEMM for gfx950 ("fp16x3"): FP32-class products on the FP16
+// matrix cores.
+//
+// An FP32 operand X is stored as two FP16 planes, hi = fp16(s X) and
+// lo = fp16(s X - hi), with one power-of-two scale s per operand chosen so that
+// max |s X| < 2^15 (kept as its integer exponent and applied with ldexp, so no
+// scale or descale factor can overflow).  hi + lo carries 22 significant bits, and
+//     a b ~ (a_hi b_hi + a_hi b_lo + a_lo b_hi) / (s_a s_b)
+// drops only the lo*lo term (relative 2^-22).  The three products run on
+// v_mfma_f32_16x16x32_f16 with FP32 accumulation.  A split GEMM costs three FP16
+// MFMA GEMMs: at the 2.5 PFLOP/s dense FP16 peak that is 833 TFLOP/s of
+// FP32-equivalent work, 5.3x the 157.3 TFLOP/s FP32 MFMA peak.
+//
+// Planes are column-major and k-contiguous (element (k, c) at k + c*ld), padded
+// with zeros to a multiple of 32 rows and 128 columns, so the GEMM loads no
+// partial tiles and needs no bounds logic except on the output.  The GEMM is the
+// "TN" form C[m,n] = alpha sum_k A(k,m) B(k,n) + beta C[m,n]; a symmetric left
+// operand (every amplitude and integral matrix of the ring terms) is its own
+// transpose, so C = A B needs no data movement.
+//
+// Tile: 256x128 per 512-thread workgroup, 8 wavefronts of 64x64 (4x4 MFMA
+// blocks of 16x16), k-slices of 32 staged through LDS (double-buffered; the four
+// plane slices take 48 KB per buffer).  16-byte chunks are XOR-swizzled by row
+// so that both the global-to-LDS stores and the fragment reads are bank-conflict
+// free.  Among 128x128/4-wave, 256x128/4-wave, 128x256, 256x256 and deeper
+// register prefetch, this shape was fastest on the ring (20400^3) and ladder
+// (1830 x 57970 x 57970) products.
+#include "jues_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace jues {
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+// Workgroup tile BM x BN over WM x WN wavefronts (each (BM/WM) x (BN/WN)), k-slices of 32.
+constexpr int BM = 256, BN = 128, WM = 4, WN = 2, NT = 64 * WM * WN, BK = 32, NXCD = 8, GROUP_M = 8;
+constexpr int PAD_C = 256;  // column padding of every plane (>= BM, BN)
+constexpr int kBufferDword3 = 0x00020000;  // gfx9-family raw buffer descriptor word 3
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, kBufferDword3);
+}
+
+// byte offset of 16-byte chunk `c` (k = 8c..8c+7) of LDS row `r` in an R x 32 FP16 slice
+__device__ inline int lds_off(int r, int c) { return r * 64 + ((c ^ ((r >> 1) & 3)) << 4); }
+
+__device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, int& tm, int& tn) {
+  // XCD-aware bijective remap, then grouped ordering (as the FP32/FP64 GEMM)
+  const int per = (nwg + NXCD - 1) / NXCD;
+  int tall = nwg % NXCD;
+  if (tall == 0) tall = NXCD;
+  const int xcd = pid % NXCD, loc = pid / NXCD;
+  const int p2 = xcd < tall ? xcd * per + loc : tall * per + (xcd - tall) * (per - 1) + loc;
+  const int group_size = GROUP_M * tiles_n;
+  const int gid = p2 / group_size;
+  const int first_m = gid * GROUP_M;
+  const int gsz = min(tiles_m - first_m, GROUP_M);
+  const int r = p2 - gid * group_size;
+  tm = first_m + r % gsz;
+  tn = r / gsz;
+}
+
+// KIND separates kernel symbols for profiles (0 generic, 1 CC ring, 2 CC pp-ladder, 3 CC hole-hole).
+template <int KIND>
+__global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, float alpha,
+                                                       const _Float16* __restrict__ Ahi,
+                                                       const _Float16* __restrict__ Alo, long lda,
+                                                       const int* __restrict__ ea,
+                                                       const _Float16* __restrict__ Bhi,
+                                                       const _Float16* __restrict__ Blo, long ldb,
+                                                       const int* __restrict__ eb, float beta,
+                                                       float* __restrict__ C, long ldc, int tiles_m, int tiles_n) {
+  constexpr int PA = BM * BK * 2, PB = BN * BK * 2;  // bytes of one plane slice
+  constexpr int CA = BM * 4 / NT, CB = BN * 4 / NT;  // 16-byte chunks per thread per plane
+  constexpr int IM = BM / WM / 16, JN = BN / WN / 16;
+  static_assert(CA * NT == BM * 4 && CB * NT == BN * 4, "tile does not divide into chunks");
+  __shared__ __attribute__((aligned(16))) char lds[2][2 * PA + 2 * PB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  // chunk q = tid + NT p of a plane slice is tile column q/4, k-chunk q%4
+  int voffA[CA], loffA[CA], voffB[CB], loffB[CB];
+#pragma unroll
+  for (int p = 0; p < CA; ++p) {
+    const int q = tid + NT * p;
+    voffA[p] = (int)(2 * ((q & 3) * 8 + (long)(q >> 2) * lda));
+    loffA[p] = lds_off(q >> 2, q & 3);
+  }
+#pragma unroll
+  for (int p = 0; p < CB; ++p) {
+    const int q = tid + NT * p;
+    voffB[p] = (int)(2 * ((q & 3) * 8 + (long)(q >> 2) * ldb));
+    loffB[p] = lds_off(q >> 2, q & 3);
+  }
+  const _Float16* aH = Ahi + (long)m0 * lda;
+  const _Float16* aL = Alo + (long)m0 * lda;
+  const _Float16* bH = Bhi + (long)n0 * ldb;
+  const _Float16* bL = Blo + (long)n0 * ldb;
+
+  u4 sa[2][CA], sb[2][CB];
+  auto gload = [&](int k0) {
+    const auto r0 = make_rsrc(aH + k0), r1 = make_rsrc(aL + k0), r2 = make_rsrc(bH + k0), r3 = make_rsrc(bL + k0);
+#pragma unroll
+    for (int p = 0; p < CA; ++p) {
+      sa[0][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r0, voffA[p], 0, 0));
+      sa[1][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r1, voffA[p], 0, 0));
+    }
+#pragma unroll
+    for (int p = 0; p < CB; ++p) {
+      sb[0][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r2, voffB[p], 0, 0));
+      sb[1][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r3, voffB[p], 0, 0));
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < CA; ++p) {
+      *reinterpret_cast<u4*>(&lds[buf][loffA[p]]) = sa[0][p];
+      *reinterpret_cast<u4*>(&lds[buf][PA + loffA[p]]) = sa[1][p];
+    }
+#pragma unroll
+    for (int p = 0; p < CB; ++p) {
+      *reinterpret_cast<u4*>(&lds[buf][2 * PA + loffB[p]]) = sb[0][p];
+      *reinterpret_cast<u4*>(&lds[buf][2 * PA + PB + loffB[p]]) = sb[1][p];
+    }
+  };
+
+  fvec4 acc[IM][JN];
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
+
+  // fragment rows: lane%16 within a 16-row block, k-chunk lane/16
+  const int fr = lane & 15, fc = lane >> 4;
+  const int nk = Kp / BK;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+    h8 ah[IM], al[IM], bh[JN], bl[JN];
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
+      const int off = lds_off(wm * (BM / WM) + i * 16 + fr, fc);
+      ah[i] = *reinterpret_cast<const h8*>(&lds[cur][off]);
+      al[i] = *reinterpret_cast<const h8*>(&lds[cur][PA + off]);
+    }
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+      const int off = lds_off(wn * (BN / WN) + j * 16 + fr, fc);
+      bh[j] = *reinterpret_cast<const h8*>(&lds[cur][2 * PA + off]);
+      bl[j] = *reinterpret_cast<const h8*>(&lds[cur][2 * PA + PB + off]);
+    }
+    // operands swapped (B^T A^T) so that a lane's accumulator column is an m index
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[j], al[i], acc[i][j], 0, 0, 0);
+      }
+    if (kt + 1 < nk) lstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  const int descale = -(ea[0] + eb[0]);
+  // acc[i][j][r] is C(m, n) with m = lane%16 of block i, n = 4(lane/16) + r of block j
+#pragma unroll
+  for (int i = 0; i < IM; ++i) {
+    const int m = m0 + wm * (BM / WM) + i * 16 + fr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * (BN / WN) + j * 16 + 4 * fc + r;
+        if (n < N) {
+          const long off = (long)m + (long)n * ldc;
+          float v = alpha * ldexpf(acc[i][j][r], descale);
+          if (beta != 0.0f) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+    }
+  }
+}
+
+constexpr int MAX_PART = 1024;
+
+template <typename Ti>
+__global__ __launch_bounds__(256) void maxabs_kernel(const Ti* __restrict__ X, long rows, long cols, long ldx,
+                                                      double* __restrict__ part) {
+  const long total = rows * cols;
+  double m = 0.0;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256L) {
+    const long r = e % rows, c = e / rows;
+    m = fmax(m, fabs((double)X[r + c * ldx]));
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// s = 2^k, k = 15 - e with max = f 2^e, f in [0.5, 1): max * s < 2^15.  A zero or
+// non-finite maximum gives k = 0 (non-finite values then reach the result).
+__global__ __launch_bounds__(256) void split_scale_kernel(const double* __restrict__ part, int nb,
+                                                           int* __restrict__ sexp) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) m = fmax(m, part[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double mx = red[0];
+    int k = 0;
+    if (mx > 0.0 && isfinite(mx)) {
+      int e;
+      (void)frexp(mx, &e);
+      k = 15 - e;
+    }
+    sexp[0] = k;
+  }
+}
+
+// planes from X (rows x cols, leading dimension ldx); each thread writes 8
+// consecutive rows of one column (rows_p is a multiple of 32)
+template <typename Ti>
+__global__ __launch_bounds__(256) void split_kernel(const Ti* __restrict__ X, long rows, long cols, long ldx,
+                                                     long rows_p, long cols_p, const int* __restrict__ sexp,
+                                                     _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
+  const int k = sexp[0];
+  const long r8 = rows_p / 8, total = r8 * cols_p;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256L) {
+    const long rq = e % r8, c = e / r8;
+    h8 vh, vl;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const long r = rq * 8 + t;
+      const Ti x = (r < rows && c < cols) ? (Ti)ldexp(X[r + c * ldx], k) : Ti(0);
+      const _Float16 h = (_Float16)(float)x;
+      vh[t] = h;
+      vl[t] = (_Float16)(float)(x - (Ti)h);
+    }
+    *reinterpret_cast<h8*>(hi + rq * 8 + c * rows_p) = vh;
+    *reinterpret_cast<h8*>(lo + rq * 8 + c * rows_p) = vl;
+  }
+}
+
+int grid_for(long n, int cap) { return (int)std::max(1L, std::min((n + 255) / 256, (long)cap)); }
+
+}  // namespace
+
+void Split16Buf::ensure(long rows, long cols) {
+  JUES_REQUIRE(rows > 0 && cols > 0, "split16: empty matrix");
+  const long rp = (rows + BK - 1) / BK * BK, cp = (cols + PAD_C - 1) / PAD_C * PAD_C;
+  const size_t plane = (size_t)rp * cp * sizeof(_Float16);
+  const size_t need = 2 * plane + 256;
+  if (mem.bytes < need) {
+    mem.release();
+    mem.alloc(need);
+  }
+  char* base = mem.as<char>();
+  s.hi = reinterpret_cast<_Float16*>(base);
+  s.lo = reinterpret_cast<_Float16*>(base + plane);
+  s.sexp = reinterpret_cast<int*>(base + 2 * plane);
+  s.rows = rows;
+  s.cols = cols;
+  s.rows_p = rp;
+  s.cols_p = cp;
+}
+
+template <typename Ti>
+void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out, const char* tag) {
+  JUES_REQUIRE(ldx >= rows, "split16: ldx < rows");
+  out.ensure(rows, cols);
+  const Split16& s = out.s;
+  double* part = static_cast<double*>(scratch(ctx, 3, MAX_PART * sizeof(double)));
+  const long n = rows * cols;
+  KTimer t(ctx, tag ? tag : "split16", 0.0, 2.0 * sizeof(Ti) * n + 4.0 * s.rows_p * s.cols_p);
+  const int nb = grid_for(n, MAX_PART);
+  hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, rows, cols, ldx, part);
+  hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
+  hipLaunchKernelGGL((split_kernel<Ti>), dim3(grid_for(s.rows_p / 8 * s.cols_p, 131072)), dim3(256), 0, ctx->stream, X,
+                     rows, cols, ldx, s.rows_p, s.cols_p, s.sexp, s.hi, s.lo);
+  JUES_HIP(hipGetLastError());
+}
+
+void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, const Split16& B, float beta, float* C,
+               long ldc, const char* tag) {
+  if (M <= 0 || N <= 0) return;
+  JUES_REQUIRE(A.rows == B.rows && A.rows_p == B.rows_p, "gemm16: operands differ in K");
+  JUES_REQUIRE(M <= A.cols && N <= B.cols, "gemm16: M or N exceeds the operand");
+  JUES_REQUIRE(ldc >= M, "gemm16: ldc < M");
+  JUES_REQUIRE(A.rows_p % BK == 0 && A.cols_p % BM == 0 && B.cols_p % BN == 0, "gemm16: planes not padded");
+  JUES_REQUIRE((M + BM - 1) / BM * BM <= A.cols_p && (N + BN - 1) / BN * BN <= B.cols_p, "gemm16: tiles exceed planes");
+  // 32-bit per-lane byte offsets inside one tile's columns
+  JUES_REQUIRE(2.0 * (double)std::max(A.rows_p, B.rows_p) * std::max(BM, BN) < 2147483647.0, "gemm16: K too large");
+  const long tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  JUES_REQUIRE(tiles_m * tiles_n < (1L << 31), "gemm16: too many tiles");
+  const double flops = 2.0 * M * N * A.rows;
+  const double bytes = 4.0 * ((double)A.rows * M + (double)A.rows * N) + (beta != 0.0f ? 8.0 : 4.0) * M * N;
+  const char* name = tag ? tag : "gemm16";
+  const int kind = !tag ? 0
+                        : (std::strcmp(tag, "cc_ring") == 0          ? 1
+                           : std::strcmp(tag, "cc_pp_ladder") == 0 ? 2
+                           : std::strcmp(tag, "cc_hh") == 0        ? 3
+                                                                   : 0);
+  KTimer timer(ctx, name, flops, bytes);
+  dim3 grid((unsigned)(tiles_m * tiles_n));
+#define JUES_G16(K_)                                                                                           \
+  hipLaunchKernelGGL((gemm16_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)A.rows_p, alpha, \
+                     A.hi, A.lo, A.rows_p, A.sexp, B.hi, B.lo, B.rows_p, B.sexp, beta, C, ldc, (int)tiles_m,   \
+                     (int)tiles_n)
+  if (kind == 1) JUES_G16(1);
+  else if (kind == 2) JUES_G16(2);
+  else if (kind == 3) JUES_G16(3);
+  else JUES_G16(0);
+#undef JUES_G16
+  JUES_HIP(hipGetLastError());
+}
+
+template void split16<float>(jues_ctx*, const float*, long, long, long, Split16Buf&, const char*);
+template void split16<double>(jues_ctx*, const double*, long, long, long, Split16Buf&, const char*);
+
+}  // namespace jues

@@ -104,6 +104,29 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
           long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch = 1,
           const char* tag = nullptr);
 
+// ---- split-FP16 MFMA GEMM (gemm16.hip) ---------------------------------------
+// An FP32 matrix X (rows x cols) as two FP16 planes hi + lo = s X, column-major,
+// zero-padded to rows_p (multiple of 32) x cols_p (multiple of 256), ld = rows_p;
+// `sexp` is a device int holding k, s = 2^k.
+struct Split16 {
+  _Float16* hi = nullptr;
+  _Float16* lo = nullptr;
+  int* sexp = nullptr;
+  long rows = 0, cols = 0, rows_p = 0, cols_p = 0;
+};
+struct Split16Buf {
+  DevBuf mem;
+  Split16 s;
+  void ensure(long rows, long cols);  // (re)allocates when the padded size grows
+};
+// planes of X; the scale is found on the device (no host synchronisation)
+template <typename Ti>
+void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out,
+             const char* tag = nullptr);
+// C[m,n] = alpha sum_k A(k,m) B(k,n) + beta C[m,n], K = A.rows = B.rows, FP32 C
+void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, const Split16& B, float beta,
+               float* C, long ldc, const char* tag = nullptr);
+
 // ---- elementwise / data movement (kernels.hip) -------------------------------
 // out = alpha * permutedims(in, perm) + beta * out   (perm 0-based; out dim k = in dim perm[k])
 template <typename Ti, typename To>

@@ -108,4 +108,6 @@ def prec_code(prec: str) -> int:
         return _lib.FP64
     if p in ("fp32_mixed", "fp32", "mixed"):
         return _lib.FP32_MIXED
+    if p in ("fp16x3_mixed", "fp16x3"):
+        return _lib.FP16X3_MIXED
     raise ValueError(f"unknown precision {prec!r}")

@@ -17,7 +17,7 @@ LIB_PATH = pathlib.Path(os.environ.get("JUES_HIP_LIB", _HERE.parent / "lib" / "l
 
 JUES_OK, JUES_NOT_CONVERGED, JUES_NONFINITE, JUES_OOM, JUES_BAD_ARG, JUES_HIP_ERROR = range(6)
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "NONFINITE", 3: "OOM", 4: "BAD_ARG", 5: "HIP_ERROR"}
-FP64, FP32_MIXED = 0, 1
+FP64, FP32_MIXED, FP16X3_MIXED = 0, 1, 2
 GUESS = {"zero": 0, "mp2": 1, "ovov": 2, "user": 3}
 CCSD_EQS = {"mRCCSD": 0, "RCCSD": 1}
 

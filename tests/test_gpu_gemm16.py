@@ -1,0 +1,85 @@
+"""Split-FP16 MFMA GEMM (C = alpha A^T B + beta C) against FP64 products.
+
+Each FP32 operand becomes hi + lo FP16 planes under a power-of-two scale; the
+three products a_hi b_hi + a_hi b_lo + a_lo b_hi are accumulated in FP32.  The
+error bound is normwise: |C - C_ref| <= tol * |alpha| * (|A|^T |B|), with tol a
+few times 2^-21 (the dropped lo*lo term and the hi/lo rounding) plus the FP32
+accumulation.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP16X3 = 2
+
+
+def _normwise(out, ref, A, B, alpha, beta, C0):
+    bound = abs(alpha) * (np.abs(A).T @ np.abs(B)) + abs(beta) * np.abs(C0)
+    return float(np.max(np.abs(out - ref) / np.maximum(bound, 1e-300)))
+
+
+@pytest.mark.parametrize("M,N,K", [(3, 5, 7), (37, 150, 45), (128, 128, 32), (129, 257, 33),
+                                   (256, 256, 64), (1000, 777, 1300)])
+def test_gemm16_shapes(ctx, M, N, K):
+    rng = np.random.default_rng(M + 10 * N + 100 * K)
+    A, B, C0 = rng.normal(size=(K, M)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    alpha, beta = 0.7, -1.3
+    ref = alpha * (A.T @ B) + beta * C0
+    out = ctx.gemm_test(FP16X3, "T", "N", alpha, A, B, beta, C0)
+    assert _normwise(out, ref, A, B, alpha, beta, C0) < 4e-6
+
+
+def test_gemm16_accuracy_close_to_fp32(ctx):
+    rng = np.random.default_rng(5)
+    M, N, K = 300, 200, 2000
+    A, B = rng.normal(size=(K, M)), rng.normal(size=(K, N))
+    ref = A.T @ B
+    z = np.zeros((M, N))
+    e16 = np.max(np.abs(ctx.gemm_test(FP16X3, "T", "N", 1.0, A, B, 0.0, z) - ref))
+    e32 = np.max(np.abs(ctx.gemm_test(1, "T", "N", 1.0, A, B, 0.0, z) - ref))
+    scale = np.max(np.abs(A).T @ np.abs(B))
+    assert e16 / scale < 2e-6
+    assert e16 < 30 * e32
+
+
+@pytest.mark.parametrize("sa,sb", [(1e-20, 1e20), (1e15, 1e-12), (3e-30, 7e-3)])
+def test_gemm16_scale_invariance(ctx, sa, sb):
+    """Per-operand power-of-two scales keep the relative accuracy at any magnitude."""
+    rng = np.random.default_rng(11)
+    A, B = rng.normal(size=(64, 96)), rng.normal(size=(64, 80))
+    ref = A.T @ B
+    out = ctx.gemm_test(FP16X3, "T", "N", 1.0 / (sa * sb), A * sa, B * sb, 0.0, np.zeros((96, 80)))
+    assert _normwise(out, ref, A, B, 1.0, 0.0, np.zeros((96, 80))) < 4e-6
+
+
+def test_gemm16_wide_dynamic_range(ctx):
+    """Columns spanning 12 orders of magnitude: small entries lose precision only
+    relative to the largest, which the normwise bound allows."""
+    rng = np.random.default_rng(3)
+    K, M, N = 200, 64, 64
+    A = rng.normal(size=(K, M)) * np.logspace(-12, 0, M)[None, :]
+    B = rng.normal(size=(K, N))
+    ref = A.T @ B
+    out = ctx.gemm_test(FP16X3, "T", "N", 1.0, A, B, 0.0, np.zeros((M, N)))
+    # column-wise relative error stays small for columns within ~1e-4 of the largest
+    big = np.abs(A).max(axis=0) > 1e-4
+    err = np.abs(out - ref)[big] / (np.abs(A).T @ np.abs(B))[big]
+    assert np.max(err) < 4e-6
+
+
+def test_gemm16_zero_operand_and_beta_zero_nan(ctx):
+    rng = np.random.default_rng(2)
+    A, B = np.zeros((40, 70)), rng.normal(size=(40, 90))
+    C0 = rng.normal(size=(70, 90))
+    out = ctx.gemm_test(FP16X3, "T", "N", 1.0, A, B, 2.0, C0)
+    assert np.max(np.abs(out - 2.0 * C0)) <= 1e-6 * np.max(np.abs(C0))
+    A = rng.normal(size=(40, 70))
+    out = ctx.gemm_test(FP16X3, "T", "N", 1.0, A, B, 0.0, np.full((70, 90), np.nan))
+    assert np.all(np.isfinite(out))
+
+
+def test_gemm16_rejects_other_transposes(ctx):
+    from jues import _lib
+    with pytest.raises(_lib.JuesError):
+        ctx.gemm_test(FP16X3, "N", "N", 1.0, np.ones((4, 4)), np.ones((4, 4)), 0.0, np.zeros((4, 4)))
