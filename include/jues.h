@@ -190,7 +190,7 @@ void jues_kernel_stats_reset(jues_ctx* ctx, int enable);
 
 /* Raw MFMA GEMM (column-major): C = alpha*op(A)*op(B) + beta*C, on device copies of
  * host arrays.  Exposed for the GEMM unit tests; prec selects FP64 or FP32 MFMA, or
- * JUES_FP16X3_MIXED for the split-FP16 GEMM (transa 'T', transb 'N' only). */
+ * JUES_FP16X3_MIXED for the split-FP16 GEMM. */
 jues_status jues_gemm_test(jues_ctx* ctx, int prec, char transa, char transb, int M, int N,
                            int K, double alpha, const double* A, int lda, const double* B,
                            int ldb, double beta, double* C, int ldc);

@@ -191,12 +191,14 @@ jues_status jues_gemm_test(jues_ctx* ctx, int prec, char transa, char transb, in
     gemm<double>(ctx, ta, tb, M, N, K, alpha, a.p, lda, 0, b.p, ldb, 0, beta, c.p, ldc, 0, 1);
     d2h(ctx, C, c.p, nc * sizeof(double));
   } else if (prec == JUES_FP16X3_MIXED) {
-    JUES_REQUIRE(ta && !tb, "jues_gemm_test: the split-FP16 GEMM takes transa='T', transb='N'");
     JUES_REQUIRE(K > 0 && M > 0 && N > 0, "jues_gemm_test: split-FP16 GEMM needs M, N, K > 0");
     DeviceArray<float> a(ctx, A, na), b(ctx, B, nb), c(ctx, C, nc);
+    // the kernel reads k-contiguous planes: op(A)^T and op(B), transposed by the split where needed
     Split16Buf sa, sb;
-    split16(ctx, a.p, K, M, lda, sa);
-    split16(ctx, b.p, K, N, ldb, sb);
+    if (ta) split16<float>(ctx, a.p, K, M, lda, sa);
+    else split16t<float>(ctx, a.p, K, M, lda, sa);
+    if (!tb) split16<float>(ctx, b.p, K, N, ldb, sb);
+    else split16t<float>(ctx, b.p, K, N, ldb, sb);
     gemm16_tn(ctx, M, N, (float)alpha, sa.s, sb.s, (float)beta, c.p, ldc);
     DevBuf c64(nc * sizeof(double));
     convert<float, double>(ctx, c.p, c64.as<double>(), (long)nc);

@@ -32,8 +32,8 @@ struct CCWork {
 // per-iteration operands (re-split on their first use in an iteration); the
 // others hold integral matrices, split once.
 enum SplitSlot {
-  SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT,
-  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SPLIT_SLOTS
+  SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT, SP_TPPT, SP_TPP, SP_WP,
+  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPPT, SPLIT_SLOTS
 };
 struct SplitState {
   struct Entry {
@@ -92,6 +92,10 @@ T* bufp_t(DevBuf& b, size_t n) {
 // operand is symmetric; slotA/slotB name the operands for the split-FP16 stage.
 template <typename T>
 void ring_mm(jues_ctx* ctx, CCState& S, T alpha, const T* A, int slotA, const T* B, int slotB, T beta, T* C);
+// Hole-hole ladder: Wp[ij,kl] += sum_ab Kpp[ij,ab] Tpp[kl,ab]; Lhh = Wp^T Tpp
+// (Kpp, Tpp: oo x vv; Wp: oo x oo, holding oooo plus any T1 terms on entry).
+template <typename T>
+void hh_mm(jues_ctx* ctx, CCState& S, const T* Kpp, const T* Tpp, T* Wp, T* Lhh);
 // Packed pp-ladder Lp/Lm from FP64 amplitudes `amp` (ijab); Tp/Tm are workspaces.
 template <typename T>
 void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm);

@@ -199,14 +199,17 @@ SplitState& split_state(CCState& S) {
   return *S.sp;
 }
 
-// planes of operand X (rows x cols, leading dimension rows) in `slot`; per-iteration
-// slots are split on their first use in an iteration, constant slots once
+// planes of operand X (rows x cols, leading dimension rows; with `trans`, of X^T
+// for X stored cols x rows) in `slot`.  Per-iteration slots are split on their
+// first use in an iteration, constant slots once.
 template <typename Ti>
-const Split16& split_operand(jues_ctx* ctx, CCState& S, int slot, const Ti* X, long rows, long cols) {
+const Split16& split_operand(jues_ctx* ctx, CCState& S, int slot, const Ti* X, long rows, long cols,
+                             bool trans = false) {
   SplitState::Entry& e = split_state(S).e[slot];
   const long stamp = slot < SPLIT_CONST ? (long)S.iter : -1;
   if (e.src != static_cast<const void*>(X) || e.stamp != stamp) {
-    split16<Ti>(ctx, X, rows, cols, rows, e.buf, "cc_split");
+    if (trans) split16t<Ti>(ctx, X, rows, cols, cols, e.buf, "cc_split");
+    else split16<Ti>(ctx, X, rows, cols, rows, e.buf, "cc_split");
     e.src = X;
     e.stamp = stamp;
   }
@@ -242,14 +245,14 @@ void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp
   const bool minus = P.nmo > 0 && P.nmv > 0;
   if constexpr (sizeof(T) == 4) {
     if (S.split_on) {
-      // Lp[(ij),(ab)] = sum_(ef) Tp[(ij),(ef)] Vp[(ef),(ab)]: Tp packed transposed is the
+      // Lp[(ij),(ab)] = sum_(ef) Tp[(ij),(ef)] Vp[(ef),(ab)]: the planes of Tp^T are the
       // k-contiguous left operand; Vp's planes come straight from the FP64 integrals
-      pack_t<float>(ctx, P, amp, S.o, S.v, Tp, Tm, true);
-      const Split16& tp = split_operand<float>(ctx, S, SP_TPT, Tp, P.npv, P.npo);
+      pack_t<float>(ctx, P, amp, S.o, S.v, Tp, Tm);
+      const Split16& tp = split_operand<float>(ctx, S, SP_TPT, Tp, P.npv, P.npo, true);
       const Split16& vp = split_operand<double>(ctx, S, SP_VP, S.i64->Vp.as<double>(), P.npv, P.npv);
       gemm16_tn(ctx, P.npo, P.npv, 1.0f, tp, vp, 0.0f, Lp, P.npo, "cc_pp_ladder");
       if (minus) {
-        const Split16& tm = split_operand<float>(ctx, S, SP_TMT, Tm, P.nmv, P.nmo);
+        const Split16& tm = split_operand<float>(ctx, S, SP_TMT, Tm, P.nmv, P.nmo, true);
         const Split16& vm = split_operand<double>(ctx, S, SP_VM, S.i64->Vm.as<double>(), P.nmv, P.nmv);
         gemm16_tn(ctx, P.nmo, P.nmv, 1.0f, tm, vm, 0.0f, Lm, P.nmo, "cc_pp_ladder");
       }
@@ -269,6 +272,29 @@ void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp
     gemm<T>(ctx, false, false, P.nmo, P.nmv, P.nmv, T(1), Tm, P.nmo, 0, I.Vm.template as<T>(), P.nmv, 0, T(0), Lm,
             P.nmo, 0, 1, "cc_pp_ladder");
 }
+
+template <typename T>
+void hh_mm(jues_ctx* ctx, CCState& S, const T* Kpp, const T* Tpp, T* Wp, T* Lhh) {
+  const long oo = (long)S.o * S.o, vv = (long)S.v * S.v;
+  if constexpr (sizeof(T) == 4) {
+    if (S.split_on) {
+      // Wp += Kpp Tpp^T: both operands enter as planes of their transposes (k = ab)
+      const Split16& kt = split_operand<float>(ctx, S, SP_KPPT, Kpp, vv, oo, true);
+      const Split16& tt = split_operand<float>(ctx, S, SP_TPPT, Tpp, vv, oo, true);
+      gemm16_tn(ctx, oo, oo, 1.0f, kt, tt, 1.0f, Wp, oo, "cc_hh");
+      // Lhh = Wp^T Tpp: k = kl is already contiguous in both
+      const Split16& w = split_operand<float>(ctx, S, SP_WP, Wp, oo, oo);
+      const Split16& t = split_operand<float>(ctx, S, SP_TPP, Tpp, oo, vv);
+      gemm16_tn(ctx, oo, vv, 1.0f, w, t, 0.0f, Lhh, oo, "cc_hh");
+      return;
+    }
+  }
+  gemm<T>(ctx, false, true, oo, oo, vv, T(1), Kpp, oo, 0, Tpp, oo, 0, T(1), Wp, oo, 0, 1, "cc_hh");
+  gemm<T>(ctx, true, false, oo, vv, oo, T(1), Wp, oo, 0, Tpp, oo, 0, T(0), Lhh, oo, 0, 1, "cc_hh");
+}
+
+template void hh_mm<double>(jues_ctx*, CCState&, const double*, const double*, double*, double*);
+template void hh_mm<float>(jues_ctx*, CCState&, const float*, const float*, float*, float*);
 
 template void ladder_mm<double>(jues_ctx*, CCState&, const double*, double*, double*, double*, double*);
 template void ladder_mm<float>(jues_ctx*, CCState&, const double*, float*, float*, float*, float*);
@@ -317,8 +343,7 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   gemm<T>(ctx, false, true, o, o, ov * v, one, I.Kpp.template as<T>(), o, 0, Upp, o, 0, zero, Fmi, o, 0, 1, "cc_fock");
   // hole-hole ladder: W' = oooo + K t^T ; Lhh = W'^T t
   JUES_HIP(hipMemcpyAsync(Wp, I.oooo.p, oo * oo * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  gemm<T>(ctx, false, true, oo, oo, vv, one, I.Kpp.template as<T>(), oo, 0, Tpp, oo, 0, one, Wp, oo, 0, 1, "cc_hh");
-  gemm<T>(ctx, true, false, oo, vv, oo, one, Wp, oo, 0, Tpp, oo, 0, zero, Lhh, oo, 0, 1, "cc_hh");
+  hh_mm<T>(ctx, S, I.Kpp.template as<T>(), Tpp, Wp, Lhh);
   // particle-particle ladder on the packed (anti)symmetric parts
   ladder_mm<T>(ctx, S, t2, Tp, Tm, Lp, Lm);
   // ring terms (every left operand is a symmetric (ov) x (ov) matrix)

@@ -65,9 +65,8 @@ struct DiisOrder {
   int old_slots[MAX_DIIS];  // their slots, in the order their dots were accumulated
 };
 
-// packed T+/T- (trans: (ef) fastest, for the split-FP16 ladder)
 template <typename T>
-void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm, bool trans = false);
+void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm);
 template <typename Ts, typename Td>
 void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm);
 template <typename TR>

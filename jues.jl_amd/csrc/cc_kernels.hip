@@ -13,9 +13,7 @@ __device__ inline long tri_lt(int p, int q) { return (long)p + (long)q * (q - 1)
 
 // Tp[(i<=j),(e<=f)] = t[ijef] + t[ijfe] (e<f), t[ijee] (e=f)
 // Tm[(i<j),(e<f)]   = t[ijef] - t[ijfe]
-// TRANS = false: Tp[(ij),(ef)] column-major (npo x npv); TRANS = true: the
-// transpose, (ef) fastest (npv x npo), the k-contiguous layout of the split GEMM.
-template <typename T, bool TRANS>
+template <typename T>
 __global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ t, int o, int v,
                                                      const int2* __restrict__ pairs_o_le, const int2* __restrict__ pairs_v_le,
                                                      long npo, long npv, T* __restrict__ Tp,
@@ -25,7 +23,7 @@ __global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ 
   const long tot_p = npo * npv, tot_m = nmo * nmv;
   for (long x = blockIdx.x * 256L + threadIdx.x; x < tot_p + tot_m; x += (long)gridDim.x * 256L) {
     if (x < tot_p) {
-      const long r = TRANS ? x / npv : x % npo, c = TRANS ? x % npv : x / npo;
+      const long r = x % npo, c = x / npo;
       const int2 ij = pairs_o_le[r], ef = pairs_v_le[c];
       const long base = ij.x + (long)o * ij.y;
       const double a = t[base + oo * (ef.x + (long)v * ef.y)];
@@ -33,7 +31,7 @@ __global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ 
       Tp[x] = (T)(ef.x == ef.y ? a : a + b);
     } else {
       const long y = x - tot_p;
-      const long r = TRANS ? y / nmv : y % nmo, c = TRANS ? y % nmv : y / nmo;
+      const long r = y % nmo, c = y / nmo;
       const int2 ij = pairs_o_lt[r], ef = pairs_v_lt[c];
       const long base = ij.x + (long)o * ij.y;
       const double a = t[base + oo * (ef.x + (long)v * ef.y)];
@@ -283,15 +281,11 @@ int grid256(long n) { return (int)std::max(1L, std::min((n + 255) / 256, 131072L
 }  // namespace
 
 template <typename T>
-void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm, bool trans) {
+void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm) {
   const long tot = P.npo * P.npv + P.nmo * P.nmv;
   KTimer k(ctx, "cc_pack_t", 0.0, (double)tot * (16.0 + sizeof(T)));
-  if (trans)
-    hipLaunchKernelGGL((pack_t_kernel<T, true>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, t, o, v, P.o_le,
-                       P.v_le, P.npo, P.npv, Tp, P.o_lt, P.v_lt, P.nmo, P.nmv, Tm);
-  else
-    hipLaunchKernelGGL((pack_t_kernel<T, false>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, t, o, v, P.o_le,
-                       P.v_le, P.npo, P.npv, Tp, P.o_lt, P.v_lt, P.nmo, P.nmv, Tm);
+  hipLaunchKernelGGL((pack_t_kernel<T>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, t, o, v, P.o_le, P.v_le,
+                     P.npo, P.npv, Tp, P.o_lt, P.v_lt, P.nmo, P.nmv, Tm);
   JUES_HIP(hipGetLastError());
 }
 
@@ -343,8 +337,8 @@ void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v, double
   JUES_HIP(hipGetLastError());
 }
 
-template void pack_t<double>(jues_ctx*, const PairTables&, const double*, int, int, double*, double*, bool);
-template void pack_t<float>(jues_ctx*, const PairTables&, const double*, int, int, float*, float*, bool);
+template void pack_t<double>(jues_ctx*, const PairTables&, const double*, int, int, double*, double*);
+template void pack_t<float>(jues_ctx*, const PairTables&, const double*, int, int, float*, float*);
 template void pack_v<double, double>(jues_ctx*, const PairTables&, const double*, int, int, int, double*, double*);
 template void pack_v<double, float>(jues_ctx*, const PairTables&, const double*, int, int, int, float*, float*);
 template void t2_update<double>(jues_ctx*, const UpdateArgs<double>&);

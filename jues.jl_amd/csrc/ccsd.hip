@@ -156,8 +156,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   JUES_HIP(hipMemcpyAsync(Wp, I.oooo.p, oo * oo * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
   contract<T>(ctx, 1.0, TN(t1, "je"), TN(ooov, "mine"), 1.0, TN(Wp, "mnij"), "cc_t1");
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(ooov, "njme"), 1.0, TN(Wp, "mnij"), "cc_t1");
-  gemm<T>(ctx, false, true, oo, oo, vv, one, I.Kpp.template as<T>(), oo, 0, Tpp, oo, 0, one, Wp, oo, 0, 1, "cc_hh");
-  gemm<T>(ctx, true, false, oo, vv, oo, one, Wp, oo, 0, Tpp, oo, 0, zero, Lhh, oo, 0, 1, "cc_hh");
+  hh_mm<T>(ctx, S, I.Kpp.template as<T>(), Tpp, Wp, Lhh);
 
   // ---- particle-particle ladder on tau ------------------------------------------------
   ladder_mm<T>(ctx, S, tau, Tp, Tm, Lp, Lm);

@@ -270,6 +270,44 @@ __global__ __launch_bounds__(256) void split_kernel(const Ti* __restrict__ X, lo
   }
 }
 
+// planes of X^T: X is stored cols x rows (leading dimension ldx), the planes are
+// rows x cols.  64 x 64 tiles pass through LDS so that both the reads of X (along
+// its columns' contiguous index) and the 16-byte plane stores are coalesced.
+template <typename Ti>
+__global__ __launch_bounds__(256) void split_t_kernel(const Ti* __restrict__ X, long rows, long cols, long ldx,
+                                                       long rows_p, const int* __restrict__ sexp,
+                                                       _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
+  __shared__ Ti tile[64][65];  // [r][c]
+  const long r0 = blockIdx.x * 64L, c0 = blockIdx.y * 64L;
+  const int t = threadIdx.x;
+#pragma unroll 4
+  for (int i = 0; i < 16; ++i) {
+    const int rr = t / 64 + 4 * i, cc = t % 64;
+    const long r = r0 + rr, c = c0 + cc;
+    tile[rr][cc] = (r < rows && c < cols) ? X[c + r * ldx] : Ti(0);
+  }
+  __syncthreads();
+  const int k = sexp[0];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int q = t + 256 * p;
+    const int rq = q % 8, cc = q / 8;
+    const long r = r0 + 8 * rq;
+    if (r >= rows_p) continue;
+    h8 vh, vl;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const Ti x = (Ti)ldexp(tile[8 * rq + e][cc], k);
+      const _Float16 h = (_Float16)(float)x;
+      vh[e] = h;
+      vl[e] = (_Float16)(float)(x - (Ti)h);
+    }
+    const long off = r + (c0 + cc) * rows_p;
+    *reinterpret_cast<h8*>(hi + off) = vh;
+    *reinterpret_cast<h8*>(lo + off) = vl;
+  }
+}
+
 int grid_for(long n, int cap) { return (int)std::max(1L, std::min((n + 255) / 256, (long)cap)); }
 
 }  // namespace
@@ -309,6 +347,24 @@ void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16
   JUES_HIP(hipGetLastError());
 }
 
+template <typename Ti>
+void split16t(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out, const char* tag) {
+  JUES_REQUIRE(ldx >= cols, "split16t: ldx < cols");
+  out.ensure(rows, cols);
+  const Split16& s = out.s;
+  double* part = static_cast<double*>(scratch(ctx, 3, MAX_PART * sizeof(double)));
+  const long n = rows * cols;
+  KTimer t(ctx, tag ? tag : "split16", 0.0, 2.0 * sizeof(Ti) * n + 4.0 * s.rows_p * s.cols_p);
+  const int nb = grid_for(n, MAX_PART);
+  // the stored matrix is cols x rows
+  hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, cols, rows, ldx, part);
+  hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
+  const dim3 grid((unsigned)((s.rows_p + 63) / 64), (unsigned)(s.cols_p / 64));
+  hipLaunchKernelGGL((split_t_kernel<Ti>), grid, dim3(256), 0, ctx->stream, X, rows, cols, ldx, s.rows_p, s.sexp,
+                     s.hi, s.lo);
+  JUES_HIP(hipGetLastError());
+}
+
 void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, const Split16& B, float beta, float* C,
                long ldc, const char* tag) {
   if (M <= 0 || N <= 0) return;
@@ -345,5 +401,7 @@ void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, con
 
 template void split16<float>(jues_ctx*, const float*, long, long, long, Split16Buf&, const char*);
 template void split16<double>(jues_ctx*, const double*, long, long, long, Split16Buf&, const char*);
+template void split16t<float>(jues_ctx*, const float*, long, long, long, Split16Buf&, const char*);
+template void split16t<double>(jues_ctx*, const double*, long, long, long, Split16Buf&, const char*);
 
 }  // namespace jues

@@ -123,6 +123,10 @@ struct Split16Buf {
 template <typename Ti>
 void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out,
              const char* tag = nullptr);
+// planes (rows x cols) of X^T, X stored cols x rows with leading dimension ldx
+template <typename Ti>
+void split16t(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out,
+              const char* tag = nullptr);
 // C[m,n] = alpha sum_k A(k,m) B(k,n) + beta C[m,n], K = A.rows = B.rows, FP32 C
 void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, const Split16& B, float beta,
                float* C, long ldc, const char* tag = nullptr);

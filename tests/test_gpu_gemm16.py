@@ -79,7 +79,16 @@ def test_gemm16_zero_operand_and_beta_zero_nan(ctx):
     assert np.all(np.isfinite(out))
 
 
-def test_gemm16_rejects_other_transposes(ctx):
-    from jues import _lib
-    with pytest.raises(_lib.JuesError):
-        ctx.gemm_test(FP16X3, "N", "N", 1.0, np.ones((4, 4)), np.ones((4, 4)), 0.0, np.zeros((4, 4)))
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("M,N,K", [(3, 5, 7), (70, 130, 65), (300, 257, 129)])
+def test_gemm16_transposed_split(ctx, ta, tb, M, N, K):
+    """Operands that are not k-contiguous pass through the LDS-tiled transposing split."""
+    rng = np.random.default_rng(M + N + K)
+    A = rng.normal(size=(K, M) if ta == "T" else (M, K))
+    B = rng.normal(size=(N, K) if tb == "T" else (K, N))
+    C0 = rng.normal(size=(M, N))
+    opA = A.T if ta == "T" else A
+    opB = B.T if tb == "T" else B
+    ref = 0.5 * (opA @ opB) + 0.25 * C0
+    out = ctx.gemm_test(FP16X3, ta, tb, 0.5, A, B, 0.25, C0)
+    assert _normwise(out, ref, opA.T, opB, 0.5, 0.25, C0) < 4e-6
