@@ -11,20 +11,23 @@
 // MFMA GEMMs: at the 2.5 PFLOP/s dense FP16 peak that is 833 TFLOP/s of
 // FP32-equivalent work, 5.3x the 157.3 TFLOP/s FP32 MFMA peak.
 //
-// Planes are column-major and k-contiguous (element (k, c) at k + c*ld), padded
-// with zeros to a multiple of 32 rows and 128 columns, so the GEMM loads no
-// partial tiles and needs no bounds logic except on the output.  The GEMM is the
-// "TN" form C[m,n] = alpha sum_k A(k,m) B(k,n) + beta C[m,n]; a symmetric left
-// operand (every amplitude and integral matrix of the ring terms) is its own
-// transpose, so C = A B needs no data movement.
+// Planes are column-major and k-contiguous, stored interleaved: for column c and
+// k-block b (32 consecutive k), 32 hi halves then 32 lo halves, so that one
+// k-slice of one column is a single 128-byte line.  They are padded with zeros to
+// a multiple of 32 rows and 256 columns, so the GEMM loads no partial tiles and
+// needs no bounds logic except on the output.  The GEMM is the "TN" form
+// C[m,n] = alpha sum_k A(k,m) B(k,n) + beta C[m,n]; a symmetric left operand
+// (every amplitude and integral matrix of the ring terms) is its own transpose,
+// so C = A B needs no data movement.
 //
-// Tile: 256x128 per 512-thread workgroup, 8 wavefronts of 64x64 (4x4 MFMA
-// blocks of 16x16), k-slices of 32 staged through LDS (double-buffered; the four
-// plane slices take 48 KB per buffer).  16-byte chunks are XOR-swizzled by row
-// so that both the global-to-LDS stores and the fragment reads are bank-conflict
-// free.  Among 128x128/4-wave, 256x128/4-wave, 128x256, 256x256 and deeper
-// register prefetch, this shape was fastest on the ring (20400^3) and ladder
-// (1830 x 57970 x 57970) products.
+// Tile: 256x128 per 512-thread workgroup, 8 wavefronts of 64x64 (4x4 MFMA blocks
+// of 16x16), k-slices of 32 staged through LDS (double-buffered, 48 KB per
+// buffer) with two further slices in flight in registers.  LDS rows are 128
+// bytes (hi then lo) with 16-byte chunks XOR-swizzled by row, so both the
+// global-to-LDS stores and the fragment reads are bank-conflict free.  Among
+// 128x128/4-wave, 256x128/4-wave, 128x256, 256x256, 32x32x16 MFMA, LDS-DMA with
+// three LDS stages and split (non-interleaved) planes, this was fastest on the
+// ring (20400^3) and ladder (1830 x 57970 x 57970) products.
 #include "jues_internal.h"
 
 #include <algorithm>
@@ -47,8 +50,9 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, kBufferDword3);
 }
 
-// byte offset of 16-byte chunk `c` (k = 8c..8c+7) of LDS row `r` in an R x 32 FP16 slice
-__device__ inline int lds_off(int r, int c) { return r * 64 + ((c ^ ((r >> 1) & 3)) << 4); }
+// byte offset of 16-byte chunk q (0-3 hi, 4-7 lo; k = 8(q%4)..+7) of LDS row r in a
+// slice of 128-byte rows
+__device__ inline int lds_off(int r, int q) { return r * 128 + ((q ^ (r & 7)) << 4); }
 
 __device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, int& tm, int& tn) {
   // XCD-aware bijective remap, then grouped ordering (as the FP32/FP64 GEMM)
@@ -69,18 +73,16 @@ __device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, i
 // KIND separates kernel symbols for profiles (0 generic, 1 CC ring, 2 CC pp-ladder, 3 CC hole-hole).
 template <int KIND>
 __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, float alpha,
-                                                       const _Float16* __restrict__ Ahi,
-                                                       const _Float16* __restrict__ Alo, long lda,
+                                                       const _Float16* __restrict__ A, long lda2,
                                                        const int* __restrict__ ea,
-                                                       const _Float16* __restrict__ Bhi,
-                                                       const _Float16* __restrict__ Blo, long ldb,
+                                                       const _Float16* __restrict__ B, long ldb2,
                                                        const int* __restrict__ eb, float beta,
                                                        float* __restrict__ C, long ldc, int tiles_m, int tiles_n) {
-  constexpr int PA = BM * BK * 2, PB = BN * BK * 2;  // bytes of one plane slice
-  constexpr int CA = BM * 4 / NT, CB = BN * 4 / NT;  // 16-byte chunks per thread per plane
+  constexpr int PA = BM * 128, PB = BN * 128;         // bytes of one slice of each operand
+  constexpr int CA = BM * 8 / NT, CB = BN * 8 / NT;   // 16-byte chunks per thread per slice
   constexpr int IM = BM / WM / 16, JN = BN / WN / 16;
-  static_assert(CA * NT == BM * 4 && CB * NT == BN * 4, "tile does not divide into chunks");
-  __shared__ __attribute__((aligned(16))) char lds[2][2 * PA + 2 * PB];
+  static_assert(CA * NT == BM * 8 && CB * NT == BN * 8, "tile does not divide into chunks");
+  __shared__ __attribute__((aligned(16))) char lds[2][PA + PB];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -88,51 +90,44 @@ __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, flo
   tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
 
-  // chunk q = tid + NT p of a plane slice is tile column q/4, k-chunk q%4
+  // chunk q = tid + NT p of a slice is tile column q/8, chunk q%8 of its 128-byte line
   int voffA[CA], loffA[CA], voffB[CB], loffB[CB];
 #pragma unroll
   for (int p = 0; p < CA; ++p) {
     const int q = tid + NT * p;
-    voffA[p] = (int)(2 * ((q & 3) * 8 + (long)(q >> 2) * lda));
-    loffA[p] = lds_off(q >> 2, q & 3);
+    voffA[p] = (int)(2 * ((q & 7) * 8 + (long)(q >> 3) * lda2));
+    loffA[p] = lds_off(q >> 3, q & 7);
   }
 #pragma unroll
   for (int p = 0; p < CB; ++p) {
     const int q = tid + NT * p;
-    voffB[p] = (int)(2 * ((q & 3) * 8 + (long)(q >> 2) * ldb));
-    loffB[p] = lds_off(q >> 2, q & 3);
+    voffB[p] = (int)(2 * ((q & 7) * 8 + (long)(q >> 3) * ldb2));
+    loffB[p] = PA + lds_off(q >> 3, q & 7);
   }
-  const _Float16* aH = Ahi + (long)m0 * lda;
-  const _Float16* aL = Alo + (long)m0 * lda;
-  const _Float16* bH = Bhi + (long)n0 * ldb;
-  const _Float16* bL = Blo + (long)n0 * ldb;
+  const _Float16* aT = A + (long)m0 * lda2;
+  const _Float16* bT = B + (long)n0 * ldb2;
 
-  u4 sa[2][CA], sb[2][CB];
-  auto gload = [&](int k0) {
-    const auto r0 = make_rsrc(aH + k0), r1 = make_rsrc(aL + k0), r2 = make_rsrc(bH + k0), r3 = make_rsrc(bL + k0);
-#pragma unroll
-    for (int p = 0; p < CA; ++p) {
-      sa[0][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r0, voffA[p], 0, 0));
-      sa[1][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r1, voffA[p], 0, 0));
-    }
-#pragma unroll
-    for (int p = 0; p < CB; ++p) {
-      sb[0][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r2, voffB[p], 0, 0));
-      sb[1][p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r3, voffB[p], 0, 0));
-    }
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int p = 0; p < CA; ++p) {
-      *reinterpret_cast<u4*>(&lds[buf][loffA[p]]) = sa[0][p];
-      *reinterpret_cast<u4*>(&lds[buf][PA + loffA[p]]) = sa[1][p];
-    }
-#pragma unroll
-    for (int p = 0; p < CB; ++p) {
-      *reinterpret_cast<u4*>(&lds[buf][2 * PA + loffB[p]]) = sb[0][p];
-      *reinterpret_cast<u4*>(&lds[buf][2 * PA + PB + loffB[p]]) = sb[1][p];
-    }
-  };
+  // Two register slots hold the next two k-slices in flight; the loop is unrolled by
+  // two so that slot indices are compile-time constants.  Loads past the last slice
+  // re-read the last slice (in bounds, never used), which keeps the body free of
+  // branches so the scheduler can interleave loads, LDS traffic and MFMAs.  (Macros,
+  // not lambdas: lambdas over the register arrays cost spills.)
+  const int nk = Kp / BK;
+  u4 sa0[CA], sb0[CB], sa1[CA], sb1[CB];
+#define GLOAD(SA, SB, KT)                                                                                 \
+  {                                                                                                       \
+    const int k0_ = min((KT), nk - 1) * 2 * BK;                                                           \
+    const auto rA_ = make_rsrc(aT + k0_), rB_ = make_rsrc(bT + k0_);                                      \
+    _Pragma("unroll") for (int p = 0; p < CA; ++p)                                                        \
+        SA[p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(rA_, voffA[p], 0, 0));      \
+    _Pragma("unroll") for (int p = 0; p < CB; ++p)                                                        \
+        SB[p] = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(rB_, voffB[p], 0, 0));      \
+  }
+#define LSTORE(BUF, SA, SB)                                                                               \
+  {                                                                                                       \
+    _Pragma("unroll") for (int p = 0; p < CA; ++p) *reinterpret_cast<u4*>(&lds[BUF][loffA[p]]) = SA[p];  \
+    _Pragma("unroll") for (int p = 0; p < CB; ++p) *reinterpret_cast<u4*>(&lds[BUF][loffB[p]]) = SB[p];  \
+  }
 
   fvec4 acc[IM][JN];
 #pragma unroll
@@ -140,41 +135,64 @@ __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, flo
 #pragma unroll
     for (int j = 0; j < JN; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
 
-  // fragment rows: lane%16 within a 16-row block, k-chunk lane/16
+  // fragment rows: lane%16 within a 16-row block, k-chunk lane/16 (hi), lane/16 + 4 (lo)
   const int fr = lane & 15, fc = lane >> 4;
-  const int nk = Kp / BK;
-  gload(0);
-  lstore(0);
+  int aoff[IM], boff[JN];
+#pragma unroll
+  for (int i = 0; i < IM; ++i) aoff[i] = lds_off(wm * (BM / WM) + i * 16 + fr, fc);
+#pragma unroll
+  for (int j = 0; j < JN; ++j) boff[j] = PA + lds_off(wn * (BN / WN) + j * 16 + fr, fc);
+  // the lo chunk of a fragment is its hi chunk's slot with bit 2 flipped: offset ^ 64
+#define COMPUTE(BUF)                                                                                      \
+  {                                                                                                       \
+    h8 ah[IM], al[IM], bh[JN], bl[JN];                                                                    \
+    _Pragma("unroll") for (int i = 0; i < IM; ++i) ah[i] = *reinterpret_cast<const h8*>(&lds[BUF][aoff[i]]); \
+    _Pragma("unroll") for (int j = 0; j < JN; ++j) bh[j] = *reinterpret_cast<const h8*>(&lds[BUF][boff[j]]); \
+    _Pragma("unroll") for (int i = 0; i < IM; ++i) al[i] = *reinterpret_cast<const h8*>(&lds[BUF][aoff[i] ^ 64]); \
+    _Pragma("unroll") for (int j = 0; j < JN; ++j) bl[j] = *reinterpret_cast<const h8*>(&lds[BUF][boff[j] ^ 64]); \
+    /* operands swapped (B^T A^T) so that a lane's accumulator column is an m index */                  \
+    _Pragma("unroll") for (int i = 0; i < IM; ++i) _Pragma("unroll") for (int j = 0; j < JN; ++j)        \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[j], ah[i], acc[i][j], 0, 0, 0);             \
+    _Pragma("unroll") for (int i = 0; i < IM; ++i) _Pragma("unroll") for (int j = 0; j < JN; ++j) {      \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bl[j], ah[i], acc[i][j], 0, 0, 0);               \
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[j], al[i], acc[i][j], 0, 0, 0);               \
+    }                                                                                                     \
+  }
+  // Schedule of one half-iteration: all fragment reads first, then the MFMAs with the
+  // next slice's global loads and LDS stores spread between them.
+#define SCHEDULE()                                                                                        \
+  {                                                                                                       \
+    constexpr int NR = 2 * (IM + JN), NM = 3 * IM * JN, NS = CA + CB, per = NM / (2 * NS);               \
+    __builtin_amdgcn_sched_group_barrier(0x100, NR, 0); /* DS reads */                                   \
+    _Pragma("unroll") for (int r = 0; r < NS; ++r) {                                                      \
+      __builtin_amdgcn_sched_group_barrier(0x008, per, 0); /* MFMA */                                    \
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   /* VMEM read */                               \
+    }                                                                                                     \
+    _Pragma("unroll") for (int r = 0; r < NS; ++r) {                                                      \
+      __builtin_amdgcn_sched_group_barrier(0x008, per, 0);                                               \
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0); /* DS write */                                  \
+    }                                                                                                     \
+    __builtin_amdgcn_sched_group_barrier(0x008, NM - per * 2 * NS, 0);                                   \
+  }
+
+  // prologue: slice 0 -> LDS buffer 0, slice 1 -> slot 1, slice 2 -> slot 0
+  GLOAD(sa0, sb0, 0);
+  LSTORE(0, sa0, sb0);
+  GLOAD(sa1, sb1, 1);
+  GLOAD(sa0, sb0, 2);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
-    h8 ah[IM], al[IM], bh[JN], bl[JN];
-#pragma unroll
-    for (int i = 0; i < IM; ++i) {
-      const int off = lds_off(wm * (BM / WM) + i * 16 + fr, fc);
-      ah[i] = *reinterpret_cast<const h8*>(&lds[cur][off]);
-      al[i] = *reinterpret_cast<const h8*>(&lds[cur][PA + off]);
-    }
-#pragma unroll
-    for (int j = 0; j < JN; ++j) {
-      const int off = lds_off(wn * (BN / WN) + j * 16 + fr, fc);
-      bh[j] = *reinterpret_cast<const h8*>(&lds[cur][2 * PA + off]);
-      bl[j] = *reinterpret_cast<const h8*>(&lds[cur][2 * PA + PB + off]);
-    }
-    // operands swapped (B^T A^T) so that a lane's accumulator column is an m index
-#pragma unroll
-    for (int i = 0; i < IM; ++i)
-#pragma unroll
-      for (int j = 0; j < JN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[j], ah[i], acc[i][j], 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < IM; ++i)
-#pragma unroll
-      for (int j = 0; j < JN; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bl[j], ah[i], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[j], al[i], acc[i][j], 0, 0, 0);
-      }
-    if (kt + 1 < nk) lstore(cur ^ 1);
+  for (int kt = 0; kt < nk; kt += 2) {
+    // buffer 0 holds slice kt; slot 1 holds kt+1, slot 0 holds kt+2
+    COMPUTE(0);
+    LSTORE(1, sa1, sb1);
+    GLOAD(sa1, sb1, kt + 3);
+    SCHEDULE();
+    __syncthreads();
+    if (kt + 1 >= nk) break;
+    COMPUTE(1);
+    LSTORE(0, sa0, sb0);
+    GLOAD(sa0, sb0, kt + 4);
+    SCHEDULE();
     __syncthreads();
   }
 
@@ -199,6 +217,11 @@ __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, flo
     }
   }
 }
+
+#undef GLOAD
+#undef LSTORE
+#undef COMPUTE
+#undef SCHEDULE
 
 constexpr int MAX_PART = 1024;
 
@@ -246,12 +269,12 @@ __global__ __launch_bounds__(256) void split_scale_kernel(const double* __restri
   }
 }
 
-// planes from X (rows x cols, leading dimension ldx); each thread writes 8
-// consecutive rows of one column (rows_p is a multiple of 32)
+// interleaved planes from X (rows x cols, leading dimension ldx); each thread writes
+// 8 consecutive rows of one column (rows_p is a multiple of 32, so they share a k-block)
 template <typename Ti>
 __global__ __launch_bounds__(256) void split_kernel(const Ti* __restrict__ X, long rows, long cols, long ldx,
                                                      long rows_p, long cols_p, const int* __restrict__ sexp,
-                                                     _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
+                                                     _Float16* __restrict__ P) {
   const int k = sexp[0];
   const long r8 = rows_p / 8, total = r8 * cols_p;
   for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256L) {
@@ -265,8 +288,9 @@ __global__ __launch_bounds__(256) void split_kernel(const Ti* __restrict__ X, lo
       vh[t] = h;
       vl[t] = (_Float16)(float)(x - (Ti)h);
     }
-    *reinterpret_cast<h8*>(hi + rq * 8 + c * rows_p) = vh;
-    *reinterpret_cast<h8*>(lo + rq * 8 + c * rows_p) = vl;
+    const long off = c * 2 * rows_p + (rq / 4) * 64 + (rq % 4) * 8;
+    *reinterpret_cast<h8*>(P + off) = vh;
+    *reinterpret_cast<h8*>(P + off + 32) = vl;
   }
 }
 
@@ -276,7 +300,7 @@ __global__ __launch_bounds__(256) void split_kernel(const Ti* __restrict__ X, lo
 template <typename Ti>
 __global__ __launch_bounds__(256) void split_t_kernel(const Ti* __restrict__ X, long rows, long cols, long ldx,
                                                        long rows_p, const int* __restrict__ sexp,
-                                                       _Float16* __restrict__ hi, _Float16* __restrict__ lo) {
+                                                       _Float16* __restrict__ P) {
   __shared__ Ti tile[64][65];  // [r][c]
   const long r0 = blockIdx.x * 64L, c0 = blockIdx.y * 64L;
   const int t = threadIdx.x;
@@ -302,9 +326,9 @@ __global__ __launch_bounds__(256) void split_t_kernel(const Ti* __restrict__ X, 
       vh[e] = h;
       vl[e] = (_Float16)(float)(x - (Ti)h);
     }
-    const long off = r + (c0 + cc) * rows_p;
-    *reinterpret_cast<h8*>(hi + off) = vh;
-    *reinterpret_cast<h8*>(lo + off) = vl;
+    const long off = (c0 + cc) * 2 * rows_p + (r / 32) * 64 + (r % 32);
+    *reinterpret_cast<h8*>(P + off) = vh;
+    *reinterpret_cast<h8*>(P + off + 32) = vl;
   }
 }
 
@@ -315,16 +339,15 @@ int grid_for(long n, int cap) { return (int)std::max(1L, std::min((n + 255) / 25
 void Split16Buf::ensure(long rows, long cols) {
   JUES_REQUIRE(rows > 0 && cols > 0, "split16: empty matrix");
   const long rp = (rows + BK - 1) / BK * BK, cp = (cols + PAD_C - 1) / PAD_C * PAD_C;
-  const size_t plane = (size_t)rp * cp * sizeof(_Float16);
-  const size_t need = 2 * plane + 256;
+  const size_t planes = 2 * (size_t)rp * cp * sizeof(_Float16);
+  const size_t need = planes + 256;
   if (mem.bytes < need) {
     mem.release();
     mem.alloc(need);
   }
   char* base = mem.as<char>();
-  s.hi = reinterpret_cast<_Float16*>(base);
-  s.lo = reinterpret_cast<_Float16*>(base + plane);
-  s.sexp = reinterpret_cast<int*>(base + 2 * plane);
+  s.p = reinterpret_cast<_Float16*>(base);
+  s.sexp = reinterpret_cast<int*>(base + planes);
   s.rows = rows;
   s.cols = cols;
   s.rows_p = rp;
@@ -343,7 +366,7 @@ void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16
   hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, rows, cols, ldx, part);
   hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
   hipLaunchKernelGGL((split_kernel<Ti>), dim3(grid_for(s.rows_p / 8 * s.cols_p, 131072)), dim3(256), 0, ctx->stream, X,
-                     rows, cols, ldx, s.rows_p, s.cols_p, s.sexp, s.hi, s.lo);
+                     rows, cols, ldx, s.rows_p, s.cols_p, s.sexp, s.p);
   JUES_HIP(hipGetLastError());
 }
 
@@ -361,7 +384,7 @@ void split16t(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split1
   hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
   const dim3 grid((unsigned)((s.rows_p + 63) / 64), (unsigned)(s.cols_p / 64));
   hipLaunchKernelGGL((split_t_kernel<Ti>), grid, dim3(256), 0, ctx->stream, X, rows, cols, ldx, s.rows_p, s.sexp,
-                     s.hi, s.lo);
+                     s.p);
   JUES_HIP(hipGetLastError());
 }
 
@@ -374,7 +397,7 @@ void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, con
   JUES_REQUIRE(A.rows_p % BK == 0 && A.cols_p % BM == 0 && B.cols_p % BN == 0, "gemm16: planes not padded");
   JUES_REQUIRE((M + BM - 1) / BM * BM <= A.cols_p && (N + BN - 1) / BN * BN <= B.cols_p, "gemm16: tiles exceed planes");
   // 32-bit per-lane byte offsets inside one tile's columns
-  JUES_REQUIRE(2.0 * (double)std::max(A.rows_p, B.rows_p) * std::max(BM, BN) < 2147483647.0, "gemm16: K too large");
+  JUES_REQUIRE(4.0 * (double)std::max(A.rows_p, B.rows_p) * std::max(BM, BN) < 2147483647.0, "gemm16: K too large");
   const long tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   JUES_REQUIRE(tiles_m * tiles_n < (1L << 31), "gemm16: too many tiles");
   const double flops = 2.0 * M * N * A.rows;
@@ -389,7 +412,7 @@ void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, con
   dim3 grid((unsigned)(tiles_m * tiles_n));
 #define JUES_G16(K_)                                                                                           \
   hipLaunchKernelGGL((gemm16_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)A.rows_p, alpha, \
-                     A.hi, A.lo, A.rows_p, A.sexp, B.hi, B.lo, B.rows_p, B.sexp, beta, C, ldc, (int)tiles_m,   \
+                     A.p, 2 * A.rows_p, A.sexp, B.p, 2 * B.rows_p, B.sexp, beta, C, ldc, (int)tiles_m,           \
                      (int)tiles_n)
   if (kind == 1) JUES_G16(1);
   else if (kind == 2) JUES_G16(2);

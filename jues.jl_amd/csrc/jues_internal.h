@@ -105,12 +105,12 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
           const char* tag = nullptr);
 
 // ---- split-FP16 MFMA GEMM (gemm16.hip) ---------------------------------------
-// An FP32 matrix X (rows x cols) as two FP16 planes hi + lo = s X, column-major,
-// zero-padded to rows_p (multiple of 32) x cols_p (multiple of 256), ld = rows_p;
+// An FP32 matrix X (rows x cols) as two FP16 planes hi + lo = s X, column-major and
+// zero-padded to rows_p (multiple of 32) x cols_p (multiple of 256), interleaved per
+// 32-row block: column c, block b holds 32 hi then 32 lo halves at p + c*2*rows_p + 64b.
 // `sexp` is a device int holding k, s = 2^k.
 struct Split16 {
-  _Float16* hi = nullptr;
-  _Float16* lo = nullptr;
+  _Float16* p = nullptr;
   int* sexp = nullptr;
   long rows = 0, cols = 0, rows_p = 0, cols_p = 0;
 };
