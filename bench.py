@@ -38,6 +38,16 @@ sys.path.insert(0, str(ROOT / "jues.jl_amd"))
 
 METRIC = "RCCD s/iter + MFMA TFLOP/s, FP32-mixed vs FP64, |ΔE|≤1e-8 Eh, o=60 v=340"
 PEAK = {"f64": 78.6, "f32": 157.3, "f16": 2500.0}  # dense MFMA TFLOP/s (MI355X datasheet)
+TRAFFIC = ROOT / "profiles" / "rccd_o60_v340_hbm_traffic.json"
+
+
+def traffic(stage: str, tag: str, workload: str):
+    """HBM bytes per launch of a kernel, from the committed FETCH_SIZE / WRITE_SIZE
+    profile of this workload (None when the profile does not cover it)."""
+    if workload != "cfg4_rccd_n400" or not TRAFFIC.exists():
+        return None
+    rec = json.loads(TRAFFIC.read_text())["per_launch"].get(f"{stage}:{tag}")
+    return None if rec is None else rec["total_bytes"]
 
 
 def f_rccd(o: int, v: int) -> float:
@@ -45,7 +55,7 @@ def f_rccd(o: int, v: int) -> float:
     return 2 * o**2 * v**4 + 12 * o**3 * v**3 + 4 * o**4 * v**2 + 4 * o**2 * v**3 + 4 * o**3 * v**2
 
 
-def roofline(stats: dict, prec: str) -> dict:
+def roofline(stats: dict, prec: str, workload: str) -> dict:
     """Roofline of the dominant kernel class (largest total device time)."""
     gemms = {k: s for k, s in stats.items() if s["flops"] > 0}
     name, s = max(gemms.items(), key=lambda kv: kv[1]["ms"])
@@ -53,12 +63,12 @@ def roofline(stats: dict, prec: str) -> dict:
     avg_s = s["ms"] / s["calls"] / 1e3
     achieved = per_launch_flops / avg_s / 1e12
     return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK[prec],
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK[prec], 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK[prec], 4), "traffic": traffic(prec, name, workload),
             "flops_per_launch": per_launch_flops, "avg_launch_ms": round(avg_s * 1e3, 3),
             "launches": int(s["calls"])}
 
 
-def roofline_split(stats: dict) -> dict:
+def roofline_split(stats: dict, workload: str) -> dict:
     """Roofline of the dominant split-FP16 GEMM: each FP32-equivalent flop executes
     three FP16 MFMA products, priced against the dense FP16 peak."""
     g = {k: s for k, s in stats.items() if s["flops"] > 0 and k in ("cc_ring", "cc_pp_ladder")}
@@ -67,7 +77,7 @@ def roofline_split(stats: dict) -> dict:
     eq = s["flops"] / s["calls"] / avg_s / 1e12
     return {"kernel": name, "bound": "mfma", "achieved": round(3 * eq, 2), "peak": PEAK["f16"], "unit": "TFLOP/s",
             "frac": round(3 * eq / PEAK["f16"], 4), "fp32_equivalent_tflops": round(eq, 2),
-            "fp32_equivalent_frac_of_f32_peak": round(eq / PEAK["f32"], 4), "traffic": None,
+            "fp32_equivalent_frac_of_f32_peak": round(eq / PEAK["f32"], 4), "traffic": traffic("f16x3", name, workload),
             "flops_per_launch": s["flops"] / s["calls"], "avg_launch_ms": round(avg_s * 1e3, 3),
             "launches": int(s["calls"])}
 
@@ -147,9 +157,9 @@ def main():
         "fp16x3": {"s_per_iter": round(h["s_per_iter"], 4),
                    "speedup_vs_fp64": round(d["s_per_iter"] / h["s_per_iter"], 3),
                    "tflops_canonical": round(F / h["s_per_iter"] / 1e12, 2),
-                   "roofline": roofline_split(h["stats"])},
-        "roofline": roofline(m["stats"], "f32"),
-        "roofline_fp64": roofline(d["stats"], "f64"),
+                   "roofline": roofline_split(h["stats"], args.workload)},
+        "roofline": roofline(m["stats"], "f32", args.workload),
+        "roofline_fp64": roofline(d["stats"], "f64", args.workload),
         "kernel_ms_per_iter": {"fp32": {k: round(x["ms"] / args.steps, 2) for k, x in m["stats"].items()},
                                "fp16x3": {k: round(x["ms"] / args.steps, 2) for k, x in h["stats"].items()},
                                "fp64": {k: round(x["ms"] / args.steps, 2) for k, x in d["stats"].items()}},
@@ -172,7 +182,7 @@ def converge(w, coupling: float) -> dict:
     from jues.CoupledCluster import Solver
     res = {}
     kw = dict(guess="zero", diis=True, diis_max=8, coupling=coupling, cc_e_conv=1e-10, cc_max_rms=1e-9,
-              fp32_r_conv=1e-5)
+              fp32_r_conv=1e-5, doprint=True)
     t0 = time.perf_counter()
     sm = Solver(w, method="rccd", prec="fp32_mixed", **kw)
     t_setup = time.perf_counter() - t0

@@ -25,6 +25,7 @@ const libjues = get(ENV, "JUES_HIP_LIB", joinpath(@__DIR__, "..", "lib", "libjue
 
 const JUES_FP64 = Cint(0)
 const JUES_FP32_MIXED = Cint(1)
+const JUES_FP16X3_MIXED = Cint(2)
 
 struct CCOpts
     maxit::Int32
@@ -52,7 +53,8 @@ function CCOpts(; maxit=100, cc_max_iter=maxit, cc_e_conv=1e-10, cc_max_rms=1e-9
 end
 
 precision_code(p::Symbol) = p === :fp64 ? JUES_FP64 : p === :fp32_mixed ? JUES_FP32_MIXED :
-                             error("precision must be :fp64 or :fp32_mixed")
+                             p === :fp16x3_mixed ? JUES_FP16X3_MIXED :
+                             error("precision must be :fp64, :fp32_mixed or :fp16x3_mixed")
 
 # ---------------------------------------------------------------------------
 # context (one per Julia thread, created lazily)
