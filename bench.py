@@ -92,11 +92,21 @@ def main():
     ap.add_argument("--cpu-k", type=int, default=32)
     ap.add_argument("--coupling", type=float, default=0.005)
     ap.add_argument("--converge", action="store_true")
+    ap.add_argument("--fast-r-conv", type=float, default=1e-5,
+                    help="--converge: the fast stage stops once ||dt|| is below this (or it stalls)")
+    ap.add_argument("--keep-diis", action="store_true",
+                    help="--converge: keep the fast stage's DIIS history for the FP64 refinement")
+    ap.add_argument("--forms", default="fp64,fp32_mixed,fp16x3", help="--converge: forms to run")
+    ap.add_argument("--mp2", action="store_true",
+                    help="instead of RCCD: AO->MO transform, RMP2 and the MP2 adjoint at n=256 (dense AO)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.mp2:
+        print(json.dumps(mp2_bench()), flush=True)
+        return
 
     from jues import synthetic, _lib
     from jues.CoupledCluster import Solver
@@ -173,66 +183,109 @@ def main():
                                "cores": cb["threads"], "kind": "port", "sample": cb["sample"],
                                "host_cpus": os.cpu_count()}
     if args.converge:
-        out["convergence"] = converge(w, args.coupling)
+        out["convergence"] = converge(w, args.coupling, args.fast_r_conv, args.keep_diis, args.forms.split(","))
     print(json.dumps(out), flush=True)
 
 
-def converge(w, coupling: float) -> dict:
-    """Both forms to convergence: FP32-mixed (FP32 stage, DIIS reset, FP64 refinement) and plain FP64."""
+def mp2_bench() -> dict:
+    """Transform, RMP2 energy and MP2 adjoint at n=256, o=40, v=216 on dense AO integrals
+    (34.4 GB FP64 resident in HBM), FP64 and FP32-mixed; device times from HIP events."""
+    import numpy as np
+    from jues import synthetic, MollerPlesset, Transformation, _lib
+    from jues.Wavefunction import Wfn, ensure_ao
+    s = synthetic.workload("cfg3_mp2grad_n256")
+    n, o, v = s.n, s.o, s.v
+    t0 = time.perf_counter()
+    Bm = s.B.reshape(n * n, s.naux, order="F")
+    # symmetric n^2 x n^2 product (its transpose is the F-ordered tensor), in row blocks: one BLAS
+    # call with a 4.3e9-element output crashes the host BLAS
+    G2 = np.empty((n * n, n * n))
+    for r0 in range(0, n * n, 4096):
+        G2[r0:r0 + 4096] = Bm[r0:r0 + 4096] @ Bm.T
+    g = G2.T.reshape((n, n, n, n), order="F")
+    print(f"[bench] dense AO integrals built on the host in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    ctx = _lib.context(0)
+    w = Wfn.from_orbitals(s.C, s.eps, s.o, g)
+    t0 = time.perf_counter()
+    ensure_ao(ctx, g)
+    t_up = time.perf_counter() - t0
+    F_T = 2 * n**4 * o + 2 * n**3 * o * v + 2 * n**2 * o**2 * v + 2 * n * o**2 * v**2
+    out = {"workload": f"RMP2 + adjoint, n={n} o={o} v={v}, dense AO integrals resident in HBM "
+                       f"({n**4 * 8 / 1e9:.1f} GB FP64)", "upload_s": round(t_up, 2), "stages": {}}
+
+    def timed(label, fn, flops, peak):
+        fn()                                          # first call: one-time allocations / FP32 copies
+        ctx.stats_reset(True)
+        t0 = time.perf_counter()
+        r = fn()
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.stats_reset(False)
+        dev = sum(x["ms"] for x in st.values())
+        out["stages"][label] = {"device_ms": round(dev, 2), "wall_s_incl_host_copies": round(wall, 3),
+                                "flops": flops, "TFLOP/s": round(flops / dev / 1e9, 1),
+                                "frac_of_peak": round(flops / dev / 1e9 / peak, 3), "peak": peak,
+                                "kernels_ms": {k: round(x["ms"], 2) for k, x in st.items()}}
+        print(f"[bench] {label}: device {dev:.1f} ms, {flops / dev / 1e9:.1f} TFLOP/s", file=sys.stderr, flush=True)
+        return r
+
+    Co, Cv = s.Co, s.Cv
+    timed("transform_iajb_fp64", lambda: Transformation.tei_transform(g, Co, Cv, Co, Cv), F_T, 78.6)
+    timed("transform_iajb_fp32", lambda: Transformation.tei_transform(g, Co, Cv, Co, Cv, prec="fp32"), F_T, 157.3)
+    E64 = timed("rmp2_fp64", lambda: MollerPlesset.do_rmp2(w), F_T, 78.6)
+    E32 = timed("rmp2_fp32_mixed", lambda: MollerPlesset.do_rmp2(w, prec="fp32_mixed"), F_T, 157.3)
+    # adjoint: back-transform of G (F_T) + recomputed half-transforms (2n^4 v + 2n^4 o) + the C contractions
+    F_G = F_T + 2 * n**4 * (o + v) + 4 * n**2 * o * v * n
+    res = timed("rmp2_grad_fp64_with_dEdg", lambda: MollerPlesset.rmp2_grad(w), F_G, 78.6)
+    out["E_mp2_fp64"] = E64
+    out["E_mp2_fp32_mixed_abs_diff"] = abs(E32 - E64)
+    out["grad_E"] = res[0]
+    return out
+
+
+def converge(w, coupling: float, fast_r_conv: float = 1e-5, keep_diis: bool = False,
+             forms=("fp64", "fp32_mixed", "fp16x3")) -> dict:
+    """Each form to convergence: plain FP64, and each fast form (FP32-mixed, split-FP16) followed
+    by FP64 refinement (DIIS history reset unless keep_diis)."""
     from jues.CoupledCluster import Solver
-    res = {}
+    res = {"coupling": coupling, "fast_r_conv": fast_r_conv, "keep_diis": keep_diis}
     kw = dict(guess="zero", diis=True, diis_max=8, coupling=coupling, cc_e_conv=1e-10, cc_max_rms=1e-9,
-              fp32_r_conv=1e-5, doprint=True)
-    t0 = time.perf_counter()
-    sm = Solver(w, method="rccd", prec="fp32_mixed", **kw)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    i32, _ = sm.iterate(200, prec="fp32_mixed", stop_on_conv=True)
-    t32 = time.perf_counter() - t0
-    E32 = sm.result()[0]
-    sm.reset_diis()
-    t0 = time.perf_counter()
-    i64, c64 = sm.iterate(100, prec="fp64", stop_on_conv=True)
-    tref = time.perf_counter() - t0
-    Em = sm.result()[0]
-    print(f"[bench] mixed: fp32 {i32['iter']} it {t32:.1f} s E={E32:.12f}; refine {i64['iter'] - i32['iter']} it "
-          f"{tref:.1f} s E={Em:.12f} conv={c64}", file=sys.stderr, flush=True)
-    res["mixed"] = dict(fp32_iters=i32["iter"], fp32_s=round(t32, 2), E_after_fp32=E32,
-                        refine_iters=i64["iter"] - i32["iter"], refine_s=round(tref, 2), E=Em, converged=c64,
-                        time_to_solution_s=round(t32 + tref, 2), final_rnorm=i64["rnorm"], final_dE=i64["delta_e"])
-    del sm
-    t0 = time.perf_counter()
-    sh = Solver(w, method="rccd", prec="fp16x3", **kw)
-    t0 = time.perf_counter()
-    i16, _ = sh.iterate(200, prec="fp16x3", stop_on_conv=True)
-    t16 = time.perf_counter() - t0
-    E16 = sh.result()[0]
-    sh.reset_diis()
-    t0 = time.perf_counter()
-    j64, cj = sh.iterate(100, prec="fp64", stop_on_conv=True)
-    trh = time.perf_counter() - t0
-    Eh = sh.result()[0]
-    print(f"[bench] fp16x3: split {i16['iter']} it {t16:.1f} s E={E16:.12f}; refine {j64['iter'] - i16['iter']} it "
-          f"{trh:.1f} s E={Eh:.12f} conv={cj}", file=sys.stderr, flush=True)
-    res["fp16x3"] = dict(split_iters=i16["iter"], split_s=round(t16, 2), E_after_split=E16,
-                         refine_iters=j64["iter"] - i16["iter"], refine_s=round(trh, 2), E=Eh, converged=cj,
-                         time_to_solution_s=round(t16 + trh, 2), final_rnorm=j64["rnorm"], final_dE=j64["delta_e"])
-    del sh
-    t0 = time.perf_counter()
-    sd = Solver(w, method="rccd", prec="fp64", **kw)
-    t0 = time.perf_counter()
-    d64, cd = sd.iterate(200, prec="fp64", stop_on_conv=True)
-    td = time.perf_counter() - t0
-    Ed = sd.result()[0]
-    print(f"[bench] fp64: {d64['iter']} it {td:.1f} s E={Ed:.12f} conv={cd}", file=sys.stderr, flush=True)
-    res["fp64"] = dict(iters=d64["iter"], time_to_solution_s=round(td, 2), E=Ed, converged=cd,
-                       final_rnorm=d64["rnorm"], final_dE=d64["delta_e"])
-    res["abs_dE_mixed_vs_fp64"] = abs(Em - Ed)
-    res["abs_dE_fp32_stage_vs_fp64"] = abs(E32 - Ed)
-    res["abs_dE_fp16x3_vs_fp64"] = abs(Eh - Ed)
-    res["abs_dE_split_stage_vs_fp64"] = abs(E16 - Ed)
-    res["coupling"] = coupling
-    res["setup_s"] = round(t_setup, 2)
+              fp32_r_conv=fast_r_conv, doprint=True)
+    if "fp64" in forms:
+        sd = Solver(w, method="rccd", prec="fp64", **kw)
+        t0 = time.perf_counter()
+        d64, cd = sd.iterate(200, prec="fp64", stop_on_conv=True)
+        td = time.perf_counter() - t0
+        Ed = sd.result()[0]
+        print(f"[bench] fp64: {d64['iter']} it {td:.1f} s E={Ed:.12f} conv={cd}", file=sys.stderr, flush=True)
+        res["fp64"] = dict(iters=d64["iter"], time_to_solution_s=round(td, 2), E=Ed, converged=cd,
+                           final_rnorm=d64["rnorm"], final_dE=d64["delta_e"])
+        del sd
+    for form in ("fp32_mixed", "fp16x3"):
+        if form not in forms:
+            continue
+        sm = Solver(w, method="rccd", prec=form, **kw)
+        t0 = time.perf_counter()
+        i1, _ = sm.iterate(200, prec=form, stop_on_conv=True)
+        t1 = time.perf_counter() - t0
+        E1 = sm.result()[0]
+        if not keep_diis:
+            sm.reset_diis()
+        t0 = time.perf_counter()
+        i2, c2 = sm.iterate(100, prec="fp64", stop_on_conv=True)
+        t2 = time.perf_counter() - t0
+        E2 = sm.result()[0]
+        print(f"[bench] {form}: fast {i1['iter']} it {t1:.1f} s E={E1:.12f}; refine {i2['iter'] - i1['iter']} it "
+              f"{t2:.1f} s E={E2:.12f} conv={c2}", file=sys.stderr, flush=True)
+        rec = dict(fast_iters=i1["iter"], fast_s=round(t1, 2), E_after_fast=E1, refine_iters=i2["iter"] - i1["iter"],
+                   refine_s=round(t2, 2), E=E2, converged=c2, time_to_solution_s=round(t1 + t2, 2),
+                   final_rnorm=i2["rnorm"], final_dE=i2["delta_e"])
+        if "fp64" in res:
+            rec["abs_dE_vs_fp64"] = abs(E2 - res["fp64"]["E"])
+            rec["abs_dE_fast_stage_vs_fp64"] = abs(E1 - res["fp64"]["E"])
+        res[form] = rec
+        del sm
     return res
 
 

@@ -180,6 +180,47 @@ def rmp2_grad(gao, C, eps, o):
     return E, dC, dg
 
 
+def mo_ov_factors(B, Co, Cv):
+    """L[i,a,P] = sum_mu,nu Co[mu,i] B[mu,nu,P] Cv[nu,a]: (ia|jb) = sum_P L[ia,P] L[jb,P]."""
+    n, _, naux = B.shape
+    X = np.tensordot(Co, B, axes=([0], [0]))          # [i, nu, P]
+    return np.einsum("inP,na->iaP", X, Cv, optimize=True)
+
+
+def rmp2_factors(B, C, eps, o):
+    """RMP2.jl:33-37 on (ia|jb) built from 3-index factors (no n^4 tensor)."""
+    Co, Cv = C[:, :o], C[:, o:]
+    v = Cv.shape[1]
+    L = mo_ov_factors(B, Co, Cv)
+    naux = L.shape[2]
+    K = (L.reshape(o * v, naux, order="F") @ L.reshape(o * v, naux, order="F").T).reshape(o, v, o, v, order="F")
+    D = form_Dijab(eps, o, v).transpose(0, 2, 1, 3)
+    return float(np.sum(K * (2 * K - K.transpose(0, 3, 2, 1)) / D))
+
+
+def rmp2_grad_factors(B, C, eps, o):
+    """dE/dC of the RMP2 energy through the factor route: with K = L L^T and
+    G = dE/dK (as in ``rmp2_grad``), dE/dL = (G + G^T) L and L = Co^T B_P Cv gives
+    dE/dCo[mu,i] = sum_aP dL[i,a,P] (B_P Cv)[mu,a], dE/dCv[nu,a] = sum_iP dL[i,a,P] (Co^T B_P)[i,nu].
+    No symmetry of B in (mu,nu) is assumed.  Returns (E, dE/dC)."""
+    Co, Cv = C[:, :o], C[:, o:]
+    v = Cv.shape[1]
+    L = mo_ov_factors(B, Co, Cv)
+    naux = L.shape[2]
+    Lm = L.reshape(o * v, naux, order="F")
+    K = (Lm @ Lm.T).reshape(o, v, o, v, order="F")
+    D = form_Dijab(eps, o, v).transpose(0, 2, 1, 3)
+    Kx = K.transpose(0, 3, 2, 1)
+    E = float(np.sum(K * (2 * K - Kx) / D))
+    G = (2.0 * (2.0 * K - Kx) / D).reshape(o * v, o * v, order="F")
+    dL = ((G + G.T) @ Lm).reshape(o, v, naux, order="F")
+    Z = np.einsum("mnP,na->maP", B, Cv, optimize=True)    # (B_P Cv)[mu,a]
+    W = np.tensordot(Co, B, axes=([0], [0]))             # (Co^T B_P)[i,nu]
+    dCo = np.einsum("iaP,maP->mi", dL, Z, optimize=True)
+    dCv = np.einsum("iaP,inP->na", dL, W, optimize=True)
+    return E, np.concatenate([dCo, dCv], axis=1)
+
+
 def rmp2_energy_from_C(gao, C, eps, o):
     Co, Cv = C[:, :o], C[:, o:]
     K = tei_transform(gao, Co, Cv, Co, Cv)

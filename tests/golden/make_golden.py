@@ -4,7 +4,9 @@
 
 cfg1 (n=24, o=5, v=19, naux=72): RMP2, RCCD (mRCCD.jl equations), RCCSD with
 both reference equation sets, the first three plain-Jacobi energies from t=0,
-converged amplitudes.  cfg2 (n=120, o=20, v=100, naux=360): the full-strength
+converged amplitudes.  cfg3 (n=256, o=40, v=216, naux=768): RMP2 energy and dE/dC at 64 seeded (mu, p)
+entries, analytic (factor route) and by central differences with h = 1e-4.
+cfg2 (n=120, o=20, v=100, naux=360): the full-strength
 iteration diverges for these integrals, so RCCD and RCCSD are converged for the
 integrals scaled by a coupling of 0.02 (factors scaled by sqrt(0.02)); norms and
 sampled entries of the amplitudes are stored.  Oracles converge to
@@ -65,9 +67,29 @@ def cfg2(lam=0.02):
                         T2_mrccsd_sample=T2_m.ravel(order="F")[idx], sample_idx=idx)
 
 
+def cfg3(h=1e-4, count=64):
+    s = synthetic.workload("cfg3_mp2grad_n256")
+    t0 = time.time()
+    E, dC = O.rmp2_grad_factors(s.B, s.C, s.eps, s.o)
+    rng = np.random.default_rng(3)
+    idx = rng.integers(0, s.n, (count, 2))
+    an = np.array([dC[mu, p] for mu, p in idx])
+    fd = np.empty(count)
+    for k, (mu, p) in enumerate(idx):
+        Cp, Cm = s.C.copy(), s.C.copy()
+        Cp[mu, p] += h
+        Cm[mu, p] -= h
+        fd[k] = (O.rmp2_factors(s.B, Cp, s.eps, s.o) - O.rmp2_factors(s.B, Cm, s.eps, s.o)) / (2 * h)
+        print(f"cfg3 fd {k} ({mu},{p}) an {an[k]:.10f} fd {fd[k]:.10f} ({time.time() - t0:.0f} s)", flush=True)
+    np.savez_compressed(HERE / "cfg3.npz", E_mp2=E, idx=idx, grad=an, fd=fd, fd_h=h)
+    print("cfg3", E, np.max(np.abs(an - fd)))
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["cfg1", "cfg2"]
+    which = sys.argv[1:] or ["cfg1", "cfg2", "cfg3"]
     if "cfg1" in which:
         cfg1()
     if "cfg2" in which:
         cfg2()
+    if "cfg3" in which:
+        cfg3()
