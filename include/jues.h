@@ -64,6 +64,14 @@ typedef enum {
     JUES_GUESS_USER = 3    /* t = T2_inout on entry (warm start) */
 } jues_guess;
 
+/* Particle-particle ladder integrals <ab|ef>. */
+typedef enum {
+    JUES_LADDER_AUTO = 0,       /* stored when they fit in device memory, else recomputed (needs factors) */
+    JUES_LADDER_STORED = 1,     /* packed V+/V- built once and kept (o^0 v^4 / 2 elements) */
+    JUES_LADDER_RECOMPUTE = 2   /* rebuilt from the 3-index factors every iteration in slabs, as
+                                   DF-RCCD.jl:198-215 contracts bvv . bvv; never stored */
+} jues_ladder;
+
 /* Options; jues_cc_default_opts() fills the defaults noted here.
  * Names mirror CoupledCluster.jl:36-43 (cc_max_iter, cc_e_conv, cc_max_rms, diis). */
 typedef struct {
@@ -88,6 +96,7 @@ typedef struct {
      * Energies are those of the lambda-scaled integrals. */
     double coupling;          /* c, default 1.0 */
     int32_t continuation;     /* N, default 1 (no continuation) */
+    int32_t ladder;           /* jues_ladder, default JUES_LADDER_AUTO */
 } jues_cc_opts;
 
 /* Per-iteration record passed to the optional callback. */

@@ -261,6 +261,7 @@ void jues_cc_default_opts(jues_cc_opts* o) {
   o->verbose = 0;
   o->coupling = 1.0;
   o->continuation = 1;
+  o->ladder = JUES_LADDER_AUTO;
 }
 
 }  // extern "C"

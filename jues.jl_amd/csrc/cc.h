@@ -19,11 +19,14 @@ struct CCInts {
   DevBuf Vp, Vm;  // packed pp-ladder integrals
   DevBuf ooov_c;  // (ij|ka) at [i,j,k,a]      (RCCSD)
   DevBuf ovvv_c;  // (ia|bc) at [i,a,b,c]      (RCCSD)
+  DevBuf Lvv;     // MO factors L[ae,P] (v^2 x naux) when the ladder is recomputed
+  DevBuf Lvvt;    // the same with a and e exchanged: L[ea,P]
 };
 
 template <typename T>
 struct CCWork {
   DevBuf Tpp, Upp, Tov, Ttov, Uov, Ua, Tp, Tm, WJ, WE, Y, H, Lhh, Wp, Fae, Fmi, Lp, Lm;
+  DevBuf Xs, Vps, Vms;  // recomputed ladder: (ae|bf) slab and its packed V+/V- rows
   // RCCSD
   DevBuf t1, Fme, X2, Z, Q, n1, U2, Tau;
 };
@@ -33,7 +36,7 @@ struct CCWork {
 // others hold integral matrices, split once.
 enum SplitSlot {
   SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT, SP_TPPT, SP_TPP, SP_WP,
-  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPPT, SPLIT_SLOTS
+  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPPT, SP_LVVT, SP_LVVTT, SPLIT_SLOTS
 };
 struct SplitState {
   struct Entry {
@@ -42,6 +45,7 @@ struct SplitState {
     long stamp = -2;
   };
   Entry e[SPLIT_SLOTS];
+  Split16Buf vps, vms, tpc, tmc;  // per-slab planes of the recomputed ladder (never cached)
 };
 
 struct CCState {
@@ -67,6 +71,7 @@ struct CCState {
   std::unique_ptr<CCWork<float>> w32;
   std::unique_ptr<SplitState> sp;
   bool split_on = false;  // the FP32 stage runs its ring / pp-ladder products on split-FP16 MFMA
+  bool ladder_recompute = false;  // <ab|ef> rebuilt from factors every iteration instead of stored
   hipEvent_t ev[2] = {nullptr, nullptr};
   double lambda = 1.0;  // coupling of the equations being iterated
   double E_prev = 0;

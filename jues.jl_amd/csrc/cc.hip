@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 
 namespace jues {
 
@@ -80,6 +81,38 @@ void build_vvvv(jues_ctx* ctx, CCState& S, CCInts<double>& I, const double* Cv) 
   }
 }
 
+// MO factors of the (vv|vv) block for the recomputed ladder: L[ae,P] = (Cv^T B_P Cv)[a,e],
+// and L[ea,P] (no symmetry of B_P in the AO pair is assumed)
+void build_lvv(jues_ctx* ctx, CCInts<double>& I, const double* Cv, int v) {
+  const long naux = ctx->naux;
+  double* L = bufp<double>(I.Lvv, (size_t)v * v * naux);
+  mo_factors_dev<double>(ctx, Cv, v, Cv, v, L);
+  double* Lt = bufp<double>(I.Lvvt, (size_t)v * v * naux);
+  permute<double, double>(ctx, L, {v, v, naux}, {1, 0, 2}, Lt, 1.0, 0.0, "setup_vvvv");
+}
+
+// Whether <ab|ef> is rebuilt every iteration.  Auto: when the packed V+/V- (with its FP32
+// or split copy) and the rest of the iteration state would not fit in free device memory.
+bool ladder_recompute_mode(jues_ctx* ctx, const CCState& S, int mode, int prec) {
+  JUES_REQUIRE(mode >= JUES_LADDER_AUTO && mode <= JUES_LADDER_RECOMPUTE, "cc_setup: bad ladder mode");
+  const bool factors = ctx->Bfac.p != nullptr;
+  if (mode == JUES_LADDER_STORED) return false;
+  if (mode == JUES_LADDER_RECOMPUTE) {
+    JUES_REQUIRE(factors, "cc_setup: a recomputed ladder needs factored AO integrals (jues_set_ao_factors)");
+    return true;
+  }
+  if (!factors) return false;
+  size_t free_b = 0, total_b = 0;
+  JUES_HIP(hipMemGetInfo(&free_b, &total_b));
+  const double N2 = (double)S.o * S.o * S.v * S.v;
+  const double vv = (double)S.P.npv * S.P.npv + (double)S.P.nmv * S.P.nmv;
+  const int dm = S.opts.diis ? S.opts.diis_max : 1;
+  const double stored = 8.0 * vv * (prec == JUES_FP64 ? 1.0 : 1.5);
+  const double other = 8.0 * N2 * (10 + 2 * dm) + (prec == JUES_FP64 ? 8.0 : 4.0) * N2 * 16 +
+                       (prec == JUES_FP16X3_MIXED ? 4.0 * N2 * 9 : 0.0);
+  return stored + other > 0.85 * (double)free_b;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -137,7 +170,8 @@ void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv
     double* g = bufp<double>(S.guess_buf, N2);
     JUES_HIP(hipMemcpyAsync(g, Jc, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   }
-  build_vvvv(ctx, S, I, Cv);
+  if (S.ladder_recompute) build_lvv(ctx, I, Cv, v);
+  else build_vvvv(ctx, S, I, Cv);
   sync(ctx);
 }
 
@@ -164,6 +198,8 @@ void cc_make_ints32(jues_ctx* ctx, CCState& S) {
   // Vp/Vm are converted by the first FP32 (non-split) ladder product
   convert_buf<float>(ctx, a.ooov_c, b.ooov_c);
   convert_buf<float>(ctx, a.ovvv_c, b.ovvv_c);
+  convert_buf<float>(ctx, a.Lvv, b.Lvv);
+  convert_buf<float>(ctx, a.Lvvt, b.Lvvt);
   sync(ctx);
 }
 
@@ -239,8 +275,75 @@ void ring_mm<float>(jues_ctx* ctx, CCState& S, float alpha, const float* A, int 
   gemm16_tn(ctx, ov, ov, alpha, a, b, beta, C, ov, "cc_ring");
 }
 
+// The ladder with <ab|ef> rebuilt from the factors every iteration (DF-RCCD.jl:198-215
+// contracts bvv[a,e,Q] bvv[b,f,Q] the same way): for a slab of f, X[e,a,b,f] =
+// sum_P L[ea,P] L[bf,P] = (ae|bf); its rows of V+/V- (pairs e<=f / e<f with f in the slab)
+// are a contiguous K-range of the ladder product, accumulated into Lp / Lm.  V+/V- never exist.
+template <typename T>
+static void ladder_recompute_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm) {
+  const PairTables& P = S.P;
+  const int v = S.v;
+  const long v2 = (long)v * v, naux = ctx->naux;
+  const bool minus = P.nmo > 0 && P.nmv > 0;
+  CCWork<T>& W = work<T>(S);
+  pack_t<T>(ctx, P, amp, S.o, v, Tp, Tm);
+  // slabs of about 4 GiB of (ae|bf) (JUES_LADDER_SLAB_BYTES overrides, e.g. to test many slabs)
+  long slab_bytes = 4L << 30;
+  if (const char* env = std::getenv("JUES_LADDER_SLAB_BYTES")) slab_bytes = std::max(1L, std::atol(env));
+  const long nf = std::max(1L, std::min<long>(v, slab_bytes / (long)(v2 * v * sizeof(T))));
+  const long rows_p = (long)std::min<long>(v, nf) * v;  // bound on the packed rows of one slab
+  T* X = bufp<T>(W.Xs, (size_t)(v2 * v * nf));
+  T* Vps = bufp<T>(W.Vps, (size_t)(rows_p * P.npv));
+  T* Vms = bufp<T>(W.Vms, (size_t)std::max(1L, rows_p * P.nmv));
+  const T* L = ints<T>(S).Lvv.template as<T>();
+  const T* Lt = ints<T>(S).Lvvt.template as<T>();
+  bool split = false;
+  if constexpr (sizeof(T) == 4) split = S.split_on;
+  bool first_m = true;
+  for (int f0 = 0; f0 < v; f0 += (int)nf) {
+    const int f1 = (int)std::min<long>(v, f0 + nf);
+    const long r0p = (long)f0 * (f0 + 1) / 2, nrp = (long)f1 * (f1 + 1) / 2 - r0p;
+    const long r0m = (long)f0 * (f0 - 1) / 2, nrm = (long)f1 * (f1 - 1) / 2 - r0m;
+    const long ncol = (long)v * (f1 - f0);
+    if constexpr (sizeof(T) == 4) {
+      if (split) {
+        SplitState& Z = split_state(S);
+        // X[(ea),(bf)] from the planes (k = P) of L[ea,P] and of the slab's columns (bf) of L[bf,P]
+        const Split16& lt = split_operand<double>(ctx, S, SP_LVVT, S.i64->Lvv.as<double>(), naux, v2, true);
+        const Split16& ltt = split_operand<double>(ctx, S, SP_LVVTT, S.i64->Lvvt.as<double>(), naux, v2, true);
+        gemm16_tn(ctx, v2, ncol, 1.0f, ltt, split16_cols(lt, (long)v * f0, ncol), 0.0f, X, v2, "cc_vvvv_gen");
+        pack_v_ea<float>(ctx, P, X, v, f0, f1, Vps, Vms);
+        split16t<float>(ctx, Tp + r0p * P.npo, nrp, P.npo, P.npo, Z.tpc, "cc_split");
+        split16<float>(ctx, Vps, nrp, P.npv, nrp, Z.vps, "cc_split");
+        gemm16_tn(ctx, P.npo, P.npv, 1.0f, Z.tpc.s, Z.vps.s, f0 == 0 ? 0.0f : 1.0f, Lp, P.npo, "cc_pp_ladder");
+        if (minus && nrm > 0) {
+          split16t<float>(ctx, Tm + r0m * P.nmo, nrm, P.nmo, P.nmo, Z.tmc, "cc_split");
+          split16<float>(ctx, Vms, nrm, P.nmv, nrm, Z.vms, "cc_split");
+          gemm16_tn(ctx, P.nmo, P.nmv, 1.0f, Z.tmc.s, Z.vms.s, first_m ? 0.0f : 1.0f, Lm, P.nmo, "cc_pp_ladder");
+          first_m = false;
+        }
+        continue;
+      }
+    }
+    gemm<T>(ctx, false, true, v2, ncol, naux, T(1), Lt, v2, 0, L + (long)v * f0, v2, 0, T(0), X, v2, 0, 1,
+            "cc_vvvv_gen");
+    pack_v_ea<T>(ctx, P, X, v, f0, f1, Vps, Vms);
+    gemm<T>(ctx, false, false, P.npo, P.npv, nrp, T(1), Tp + r0p * P.npo, P.npo, 0, Vps, nrp, 0,
+            f0 == 0 ? T(0) : T(1), Lp, P.npo, 0, 1, "cc_pp_ladder");
+    if (minus && nrm > 0) {
+      gemm<T>(ctx, false, false, P.nmo, P.nmv, nrm, T(1), Tm + r0m * P.nmo, P.nmo, 0, Vms, nrm, 0,
+              first_m ? T(0) : T(1), Lm, P.nmo, 0, 1, "cc_pp_ladder");
+      first_m = false;
+    }
+  }
+}
+
 template <typename T>
 void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm) {
+  if (S.ladder_recompute) {
+    ladder_recompute_mm<T>(ctx, S, amp, Tp, Tm, Lp, Lm);
+    return;
+  }
   const PairTables& P = S.P;
   const bool minus = P.nmo > 0 && P.nmv > 0;
   if constexpr (sizeof(T) == 4) {
@@ -532,6 +635,7 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   S.eps.alloc(e.size() * sizeof(double));
   h2d(ctx, S.eps.p, e.data(), e.size() * sizeof(double));
   build_pairs(ctx, S);
+  S.ladder_recompute = ladder_recompute_mode(ctx, S, opts.ladder, prec);
   S.i64.reset(new CCInts<double>());
   cc_build_ints(ctx, S, co.p, cv.p);
   if (method == 1) cc_build_ccsd_ints(ctx, S);

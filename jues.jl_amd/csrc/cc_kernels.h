@@ -68,7 +68,11 @@ struct DiisOrder {
 template <typename T>
 void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm);
 template <typename Ts, typename Td>
-void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm);
+void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm,
+            bool slab = false);
+// slab-local V+/V- rows (pairs with f in [f0,f1)) from X[e,a,b,f-f0] = (ae|bf)
+template <typename T>
+void pack_v_ea(jues_ctx* ctx, const PairTables& P, const T* X, int v, int f0, int f1, T* Vp, T* Vm);
 template <typename TR>
 void t2_update(jues_ctx* ctx, const UpdateArgs<TR>& A);
 void t1_update(jues_ctx* ctx, const T1UpdateArgs& A, int nblocks);

@@ -44,10 +44,13 @@ __global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ 
 // From a chemist slab X[a,e,b,f'] (f' = f - f0, f in [f0,f1)) build
 // Vp[(e<=f),(a<=b)] = ((ae|bf) + (af|be))/2 and Vm[(e<f),(a<b)] = ((ae|bf) - (af|be))/2,
 // using (af|be) = X[b,e,a,f'].
+// Row rr of the packed matrix goes to Vp[(rr - rowp0) + ldp * c] (Vm likewise): the full
+// matrix with rowp0 = 0, ldp = npv, or a slab holding only this call's rows.
 template <typename Ts, typename Td>
 __global__ __launch_bounds__(256) void pack_v_kernel(const Ts* __restrict__ X, int v, int f0, int f1,
                                                      const int2* __restrict__ pairs_v_le, long npv, Td* __restrict__ Vp,
-                                                     const int2* __restrict__ pairs_v_lt, long nmv, Td* __restrict__ Vm) {
+                                                     long rowp0, long ldp, const int2* __restrict__ pairs_v_lt,
+                                                     long nmv, Td* __restrict__ Vm, long rowm0, long ldm) {
   // rows: e<=f with f in [f0,f1): q from f0(f0+1)/2 to f1(f1+1)/2
   const long r0p = (long)f0 * (f0 + 1) / 2, r1p = (long)f1 * (f1 + 1) / 2;
   const long r0m = (long)f0 * (f0 - 1) / 2, r1m = (long)f1 * (f1 - 1) / 2;
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(256) void pack_v_kernel(const Ts* __restrict__ X, i
       const long fl = ef.y - f0;
       const double p = (double)X[ab.x + v1 * ef.x + v2 * ab.y + v3 * fl];
       const double q = (double)X[ab.y + v1 * ef.x + v2 * ab.x + v3 * fl];
-      Vp[rr + npv * c] = (Td)(0.5 * (p + q));
+      Vp[(rr - rowp0) + ldp * c] = (Td)(0.5 * (p + q));
     } else {
       const long y = x - tot_p;
       const long rr = r0m + y % nrm, c = y / nrm;
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(256) void pack_v_kernel(const Ts* __restrict__ X, i
       const long fl = ef.y - f0;
       const double p = (double)X[ab.x + v1 * ef.x + v2 * ab.y + v3 * fl];
       const double q = (double)X[ab.y + v1 * ef.x + v2 * ab.x + v3 * fl];
-      Vm[rr + nmv * c] = (Td)(0.5 * (p - q));
+      Vm[(rr - rowm0) + ldm * c] = (Td)(0.5 * (p - q));
     }
   }
 }
@@ -276,6 +279,39 @@ __global__ __launch_bounds__(256) void divide_d2_kernel(double* __restrict__ t, 
   }
 }
 
+// As pack_v_kernel, from a slab generated with rows in (e,a) order: X[e,a,b,f'] = (ae|bf),
+// so that (ae|bf) = X[e,a,b,f'] and (af|be) = (be|af) = X[e,b,a,f'] are both contiguous in e
+// and threads that run over e also write consecutive packed rows (pairs e<=f of one f).
+template <typename Ts, typename Td>
+__global__ __launch_bounds__(256) void pack_v_ea_kernel(const Ts* __restrict__ X, int v, int f0, int f1,
+                                                        const int2* __restrict__ pairs_v_le, long npv,
+                                                        Td* __restrict__ Vp, const int2* __restrict__ pairs_v_lt,
+                                                        long nmv, Td* __restrict__ Vm) {
+  const long r0p = (long)f0 * (f0 + 1) / 2, r1p = (long)f1 * (f1 + 1) / 2;
+  const long r0m = (long)f0 * (f0 - 1) / 2, r1m = (long)f1 * (f1 - 1) / 2;
+  const long nrp = r1p - r0p, nrm = r1m - r0m;
+  const long tot_p = nrp * npv, tot_m = nrm * nmv;
+  const long v1 = v, v2 = v1 * v, v3 = v2 * v;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < tot_p + tot_m; x += (long)gridDim.x * 256L) {
+    if (x < tot_p) {
+      const long rl = x % nrp, c = x / nrp;
+      const int2 ef = pairs_v_le[r0p + rl], ab = pairs_v_le[c];
+      const long base = ef.x + v3 * (ef.y - f0);
+      const double p = (double)X[base + v1 * ab.x + v2 * ab.y];
+      const double q = (double)X[base + v1 * ab.y + v2 * ab.x];
+      Vp[rl + nrp * c] = (Td)(0.5 * (p + q));
+    } else {
+      const long y = x - tot_p;
+      const long rl = y % nrm, c = y / nrm;
+      const int2 ef = pairs_v_lt[r0m + rl], ab = pairs_v_lt[c];
+      const long base = ef.x + v3 * (ef.y - f0);
+      const double p = (double)X[base + v1 * ab.x + v2 * ab.y];
+      const double q = (double)X[base + v1 * ab.y + v2 * ab.x];
+      Vm[rl + nrm * c] = (Td)(0.5 * (p - q));
+    }
+  }
+}
+
 int grid256(long n) { return (int)std::max(1L, std::min((n + 255) / 256, 131072L)); }
 
 }  // namespace
@@ -290,13 +326,15 @@ void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T
 }
 
 template <typename Ts, typename Td>
-void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm) {
-  const long nrp = (long)f1 * (f1 + 1) / 2 - (long)f0 * (f0 + 1) / 2;
-  const long nrm = (long)f1 * (f1 - 1) / 2 - (long)f0 * (f0 - 1) / 2;
+void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm, bool slab) {
+  const long r0p = (long)f0 * (f0 + 1) / 2, r0m = (long)f0 * (f0 - 1) / 2;
+  const long nrp = (long)f1 * (f1 + 1) / 2 - r0p;
+  const long nrm = (long)f1 * (f1 - 1) / 2 - r0m;
   const long tot = nrp * P.npv + nrm * P.nmv;
   KTimer k(ctx, "cc_pack_v", 0.0, (double)tot * (2 * sizeof(Ts) + sizeof(Td)));
   hipLaunchKernelGGL((pack_v_kernel<Ts, Td>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, X, v, f0, f1, P.v_le,
-                     P.npv, Vp, P.v_lt, P.nmv, Vm);
+                     P.npv, Vp, slab ? r0p : 0L, slab ? std::max(1L, nrp) : P.npv, P.v_lt, P.nmv, Vm,
+                     slab ? r0m : 0L, slab ? std::max(1L, nrm) : P.nmv);
   JUES_HIP(hipGetLastError());
 }
 
@@ -339,8 +377,23 @@ void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v, double
 
 template void pack_t<double>(jues_ctx*, const PairTables&, const double*, int, int, double*, double*);
 template void pack_t<float>(jues_ctx*, const PairTables&, const double*, int, int, float*, float*);
-template void pack_v<double, double>(jues_ctx*, const PairTables&, const double*, int, int, int, double*, double*);
-template void pack_v<double, float>(jues_ctx*, const PairTables&, const double*, int, int, int, float*, float*);
+template <typename T>
+void pack_v_ea(jues_ctx* ctx, const PairTables& P, const T* X, int v, int f0, int f1, T* Vp, T* Vm) {
+  const long nrp = (long)f1 * (f1 + 1) / 2 - (long)f0 * (f0 + 1) / 2;
+  const long nrm = (long)f1 * (f1 - 1) / 2 - (long)f0 * (f0 - 1) / 2;
+  const long tot = nrp * P.npv + nrm * P.nmv;
+  KTimer k(ctx, "cc_pack_v", 0.0, (double)tot * 3 * sizeof(T));
+  hipLaunchKernelGGL((pack_v_ea_kernel<T, T>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, X, v, f0, f1, P.v_le,
+                     P.npv, Vp, P.v_lt, P.nmv, Vm);
+  JUES_HIP(hipGetLastError());
+}
+
+template void pack_v_ea<double>(jues_ctx*, const PairTables&, const double*, int, int, int, double*, double*);
+template void pack_v_ea<float>(jues_ctx*, const PairTables&, const float*, int, int, int, float*, float*);
+template void pack_v<double, double>(jues_ctx*, const PairTables&, const double*, int, int, int, double*, double*,
+                                     bool);
+template void pack_v<double, float>(jues_ctx*, const PairTables&, const double*, int, int, int, float*, float*, bool);
+template void pack_v<float, float>(jues_ctx*, const PairTables&, const float*, int, int, int, float*, float*, bool);
 template void t2_update<double>(jues_ctx*, const UpdateArgs<double>&);
 template void t2_update<float>(jues_ctx*, const UpdateArgs<float>&);
 

@@ -338,7 +338,8 @@ int grid_for(long n, int cap) { return (int)std::max(1L, std::min((n + 255) / 25
 
 void Split16Buf::ensure(long rows, long cols) {
   JUES_REQUIRE(rows > 0 && cols > 0, "split16: empty matrix");
-  const long rp = (rows + BK - 1) / BK * BK, cp = (cols + PAD_C - 1) / PAD_C * PAD_C;
+  // 256 spare zero columns: a column view (split16_cols) near the end still has whole tiles
+  const long rp = (rows + BK - 1) / BK * BK, cp = (cols + PAD_C - 1) / PAD_C * PAD_C + PAD_C;
   const size_t planes = 2 * (size_t)rp * cp * sizeof(_Float16);
   const size_t need = planes + 256;
   if (mem.bytes < need) {
@@ -394,7 +395,7 @@ void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, con
   JUES_REQUIRE(A.rows == B.rows && A.rows_p == B.rows_p, "gemm16: operands differ in K");
   JUES_REQUIRE(M <= A.cols && N <= B.cols, "gemm16: M or N exceeds the operand");
   JUES_REQUIRE(ldc >= M, "gemm16: ldc < M");
-  JUES_REQUIRE(A.rows_p % BK == 0 && A.cols_p % BM == 0 && B.cols_p % BN == 0, "gemm16: planes not padded");
+  JUES_REQUIRE(A.rows_p % BK == 0, "gemm16: planes not padded");
   JUES_REQUIRE((M + BM - 1) / BM * BM <= A.cols_p && (N + BN - 1) / BN * BN <= B.cols_p, "gemm16: tiles exceed planes");
   // 32-bit per-lane byte offsets inside one tile's columns
   JUES_REQUIRE(4.0 * (double)std::max(A.rows_p, B.rows_p) * std::max(BM, BN) < 2147483647.0, "gemm16: K too large");

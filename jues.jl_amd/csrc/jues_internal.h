@@ -106,7 +106,7 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
 
 // ---- split-FP16 MFMA GEMM (gemm16.hip) ---------------------------------------
 // An FP32 matrix X (rows x cols) as two FP16 planes hi + lo = s X, column-major and
-// zero-padded to rows_p (multiple of 32) x cols_p (multiple of 256), interleaved per
+// zero-padded to rows_p (multiple of 32) x cols_p (>= cols + 256), interleaved per
 // 32-row block: column c, block b holds 32 hi then 32 lo halves at p + c*2*rows_p + 64b.
 // `sexp` is a device int holding k, s = 2^k.
 struct Split16 {
@@ -119,6 +119,14 @@ struct Split16Buf {
   Split16 s;
   void ensure(long rows, long cols);  // (re)allocates when the padded size grows
 };
+// columns [c0, c0 + nc) of a split operand (its planes, padding beyond them included)
+inline Split16 split16_cols(const Split16& s, long c0, long nc) {
+  Split16 r = s;
+  r.p = s.p + c0 * 2 * s.rows_p;
+  r.cols = nc;
+  r.cols_p = s.cols_p - c0;
+  return r;
+}
 // planes of X; the scale is found on the device (no host synchronisation)
 template <typename Ti>
 void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out,

@@ -39,6 +39,8 @@ def _opts(kw: dict, method_default_guess: str) -> _lib.CCOpts:
     o.verbose = int(bool(kw.pop("doprint", False)))
     o.coupling = float(kw.pop("coupling", 1.0))
     o.continuation = int(kw.pop("continuation", 1))
+    lad = kw.pop("ladder", "auto")
+    o.ladder = _lib.LADDER[lad] if isinstance(lad, str) else int(lad)
     return o
 
 

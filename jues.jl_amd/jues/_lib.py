@@ -18,6 +18,7 @@ LIB_PATH = pathlib.Path(os.environ.get("JUES_HIP_LIB", _HERE.parent / "lib" / "l
 JUES_OK, JUES_NOT_CONVERGED, JUES_NONFINITE, JUES_OOM, JUES_BAD_ARG, JUES_HIP_ERROR = range(6)
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "NONFINITE", 3: "OOM", 4: "BAD_ARG", 5: "HIP_ERROR"}
 FP64, FP32_MIXED, FP16X3_MIXED = 0, 1, 2
+LADDER = {"auto": 0, "stored": 1, "recompute": 2}
 GUESS = {"zero": 0, "mp2": 1, "ovov": 2, "user": 3}
 CCSD_EQS = {"mRCCSD": 0, "RCCSD": 1}
 
@@ -45,7 +46,7 @@ class CCOpts(C.Structure):
         ("maxit", C.c_int32), ("e_conv", C.c_double), ("r_conv", C.c_double), ("diis", C.c_int32),
         ("diis_max", C.c_int32), ("guess", C.c_int32), ("ccsd_eqs", C.c_int32), ("fcn", C.c_int32),
         ("fp32_r_conv", C.c_double), ("refine_maxit", C.c_int32), ("verbose", C.c_int32),
-        ("coupling", C.c_double), ("continuation", C.c_int32),
+        ("coupling", C.c_double), ("continuation", C.c_int32), ("ladder", C.c_int32),
     ]
 
 
