@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--keep-diis", action="store_true",
                     help="--converge: keep the fast stage's DIIS history for the FP64 refinement")
     ap.add_argument("--forms", default="fp64,fp32_mixed,fp16x3", help="--converge: forms to run")
+    ap.add_argument("--diis-max", type=int, default=8)
     ap.add_argument("--ladder", default="auto", choices=["auto", "stored", "recompute"],
                     help="pp-ladder integrals stored (packed V+/V-) or rebuilt from the factors every iteration")
     ap.add_argument("--mp2", action="store_true",
@@ -118,7 +119,7 @@ def main():
     o, v = s.o, s.v
     w = Wfn.from_synthetic(s, dense=False)
     t0 = time.perf_counter()
-    solver = Solver(w, method="rccd", prec="fp32_mixed", guess="zero", diis=True, diis_max=8,
+    solver = Solver(w, method="rccd", prec="fp32_mixed", guess="zero", diis=True, diis_max=args.diis_max,
                     coupling=args.coupling, ladder=args.ladder)
     t_setup = time.perf_counter() - t0
     ctx = solver.ctx
@@ -158,7 +159,7 @@ def main():
         "data": "synthetic (seeded B-factor ERIs, QR orbitals, sorted uniform orbital energies)",
         "config": {"workload": f"RCCD o={o} v={v} n={s.n} naux={s.naux}, B-factor integrals, "
                                "vvvv packed and resident in HBM",
-                   "o": o, "v": v, "n": s.n, "naux": s.naux, "diis_max": 8, "coupling": args.coupling,
+                   "o": o, "v": v, "n": s.n, "naux": s.naux, "diis_max": args.diis_max, "coupling": args.coupling,
                    "ladder": args.ladder},
         "fp64_s_per_iter": round(d["s_per_iter"], 4),
         "speedup_mixed_vs_fp64": round(d["s_per_iter"] / m["s_per_iter"], 3),

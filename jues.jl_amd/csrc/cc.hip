@@ -682,6 +682,19 @@ double cc_energy(jues_ctx* ctx, CCState& S) {
 void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_conv, jues_iter_info* last,
             bool* converged, jues_iter_cb cb, void* user) {
   *converged = false;
+  // Only one stage's buffers are held at a time (at o=80, v=420 the FP64 and FP32 work sets
+  // together do not fit in one GPU); a later switch rebuilds what it needs.
+  if (stage_prec == JUES_FP64) {
+    if (S.w32 || S.sp || S.i32) {
+      sync(ctx);
+      S.w32.reset();
+      S.sp.reset();
+      S.i32.reset();
+    }
+  } else if (S.w64) {
+    sync(ctx);
+    S.w64.reset();
+  }
   if (stage_prec != JUES_FP64) cc_make_ints32(ctx, S);
   S.split_on = stage_prec == JUES_FP16X3_MIXED;
   int stall = 0;
