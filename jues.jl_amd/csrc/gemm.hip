@@ -43,7 +43,7 @@ template <> struct Mfma<float> {
   }
 };
 
-constexpr int BM = 128, BN = 128, NT = 256, NXCD = 8, GROUP_M = 8;
+constexpr int NT = 256, NXCD = 8, GROUP_M = 8;
 
 // Raw buffer loads: the 64-bit base lives in a scalar resource descriptor, the
 // per-lane part is one 32-bit byte offset and the per-element part a scalar
@@ -94,8 +94,11 @@ struct Slice {
 
 // KIND only separates kernel symbols (0 generic, 1 CC ring, 2 CC pp-ladder) so that
 // profiles attribute time to the coupled-cluster hot spots.
-template <typename T, bool TA, bool TB, int KIND>
-__global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alpha,
+// BM x BN workgroup tile over WM x (4/WM) wavefronts of 64x64: 128x128 (2x2) in general; 64x256
+// (1x4) and 256x64 (4x1) for skinny products (M or N <= 64 against a long other side), where a
+// 128-wide tile would compute mostly padding.
+template <typename T, bool TA, bool TB, int KIND, int BM = 128, int BN = 128, int WM = 2>
+__global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_kernel(int M, int N, int K, T alpha,
                                                       const T* __restrict__ A, long lda, long sA,
                                                       const T* __restrict__ B, long ldb, long sB,
                                                       T beta, T* __restrict__ C, long ldc, long sC,
@@ -110,7 +113,8 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, T alp
   __shared__ T Bs[2][SB::SIZE];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  static_assert(BM == 64 * WM && BN == 64 * (4 / WM), "tile does not match the wave layout");
+  const int wm = wave % WM, wn = wave / WM;
   A += (long)blockIdx.z * sA;
   B += (long)blockIdx.z * sB;
   C += (long)blockIdx.z * sC;
@@ -296,6 +300,10 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
     JUES_HIP(hipGetLastError());
     return;
   }
+  // tile shape: 0 = 128x128, 1 = 64x256 (skinny M), 2 = 256x64 (skinny N)
+  // (the ring and ladder kernels, KIND 1 and 2, are instantiated in the 128x128 shape only)
+  const int shape = kind != 0 ? 0 : (M <= 64 && N >= 8192) ? 1 : (N <= 64 && M >= 8192) ? 2 : 0;
+  const long BM = shape == 1 ? 64 : shape == 2 ? 256 : 128, BN = shape == 1 ? 256 : shape == 2 ? 64 : 128;
   const long tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   JUES_REQUIRE(tiles_m * tiles_n < (1L << 31), "gemm: too many tiles");
   // Few output tiles and a long K (the Fock-like terms): split K over workgroups,
@@ -332,17 +340,24 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
     const T* Ab = A + (long)b0 * sA;
     const T* Bb = B + (long)b0 * sB;
     T* Cb = C + (long)b0 * sC;
-#define JUES_GEMM_LAUNCH(TA_, TB_, K_)                                                             \
-  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, \
-                     alpha, Ab, lda, sA, Bb, ldb, sB, beta, Cb, ldc, sC, (int)tiles_m, (int)tiles_n, fast_ok)
+#define JUES_GEMM_LAUNCH_S(TA_, TB_, K_, BM_, BN_, WM_)                                                  \
+  hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, K_, BM_, BN_, WM_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, \
+                     (int)K, alpha, Ab, lda, sA, Bb, ldb, sB, beta, Cb, ldc, sC, (int)tiles_m, (int)tiles_n, fast_ok)
+#define JUES_GEMM_LAUNCH(TA_, TB_, K_)                                          \
+  do {                                                                          \
+    if (shape == 1) JUES_GEMM_LAUNCH_S(TA_, TB_, K_, 64, 256, 1);               \
+    else if (shape == 2) JUES_GEMM_LAUNCH_S(TA_, TB_, K_, 256, 64, 4);          \
+    else JUES_GEMM_LAUNCH_S(TA_, TB_, K_, 128, 128, 2);                         \
+  } while (0)
     if (!ta && !tb) {
-      if (kind == 1) JUES_GEMM_LAUNCH(false, false, 1);
-      else if (kind == 2) JUES_GEMM_LAUNCH(false, false, 2);
+      if (kind == 1) JUES_GEMM_LAUNCH_S(false, false, 1, 128, 128, 2);
+      else if (kind == 2) JUES_GEMM_LAUNCH_S(false, false, 2, 128, 128, 2);
       else JUES_GEMM_LAUNCH(false, false, 0);
     } else if (!ta && tb) JUES_GEMM_LAUNCH(false, true, 0);
     else if (ta && !tb) JUES_GEMM_LAUNCH(true, false, 0);
     else JUES_GEMM_LAUNCH(true, true, 0);
 #undef JUES_GEMM_LAUNCH
+#undef JUES_GEMM_LAUNCH_S
     JUES_HIP(hipGetLastError());
   }
 }
