@@ -102,6 +102,9 @@ def main():
                     help="pp-ladder integrals stored (packed V+/V-) or rebuilt from the factors every iteration")
     ap.add_argument("--mp2", action="store_true",
                     help="instead of RCCD: AO->MO transform, RMP2 and the MP2 adjoint at n=256 (dense AO)")
+    ap.add_argument("--df", action="store_true",
+                    help="instead of RCCD: density fitting at o=60 v=340 naux=1200 (make_bmunu, DF-RMP2, "
+                         "DF-RCCD sweeps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -109,6 +112,9 @@ def main():
         sys.exit(2)
     if args.mp2:
         print(json.dumps(mp2_bench()), flush=True)
+        return
+    if args.df:
+        print(json.dumps(df_bench(args.coupling)), flush=True)
         return
 
     from jues import synthetic, _lib
@@ -245,6 +251,59 @@ def mp2_bench() -> dict:
     out["E_mp2_fp64"] = E64
     out["E_mp2_fp32_mixed_abs_diff"] = abs(E32 - E64)
     out["grad_E"] = res[0]
+    return out
+
+
+def df_bench(coupling: float) -> dict:
+    """Density fitting on the o=60, v=340 factors (n=400, naux=1200): b = pqP.Jpqh on the
+    device, DF-RMP2 in FP64 and FP32-mixed, and DF-RCCD sweeps (recomputed bvv.bvv ladder).
+    pqP = b0.J^(1/2) for a seeded metric with eigenvalues in [0.3, 3]."""
+    import numpy as np
+    from jues import synthetic, DF, MollerPlesset, _lib
+    from jues import CoupledCluster as CC
+    from jues.Wavefunction import Wfn, DFIntegrals
+    s = synthetic.workload("cfg4_rccd_n400")
+    n, o, v, naux = s.n, s.o, s.v, s.naux
+    rng = np.random.default_rng(11)
+    Q, _ = np.linalg.qr(rng.standard_normal((naux, naux)))
+    wv = np.linspace(0.3, 3.0, naux)
+    Jh = (Q * np.sqrt(wv)) @ Q.T
+    Jpqh = np.asfortranarray((Q / np.sqrt(wv)) @ Q.T)
+    pqP = np.asfortranarray((s.B.reshape(n * n, naux, order="F") @ Jh).reshape(n, n, naux, order="F"))
+    ctx = _lib.context(0)
+    out = {"workload": f"density fitting, n={n} o={o} v={v} naux={naux}", "stages": {}}
+
+    def timed(label, fn, flops, peak, reps=2):
+        for _ in range(reps - 1):
+            fn()
+        ctx.stats_reset(True)
+        r = fn()
+        st = ctx.stats()
+        ctx.stats_reset(False)
+        dev = sum(x["ms"] for x in st.values())
+        out["stages"][label] = {"device_ms": round(dev, 2), "flops": flops, "TFLOP/s": round(flops / dev / 1e9, 1),
+                                "frac_of_peak": round(flops / dev / 1e9 / peak, 3), "peak": peak,
+                                "kernels_ms": {k: round(x["ms"], 2) for k, x in st.items()}}
+        print(f"[bench] {label}: device {dev:.1f} ms, {flops / dev / 1e9:.1f} TFLOP/s", file=sys.stderr, flush=True)
+        return r
+
+    b = timed("make_bmunu_fp64", lambda: DF.make_bmunu(pqP, Jpqh), 2 * n * n * naux * naux, 78.6)
+    out["bmunu_max_abs_err"] = float(np.abs(b - s.B).max())
+    del b
+    w = Wfn.from_orbitals(s.C, s.eps, o, DFIntegrals(pqP, Jpqh))
+    F_mp2 = 2 * n * n * o * naux + 2 * n * o * v * naux + 2 * (o * v) ** 2 * naux
+    E64 = timed("df_rmp2_fp64", lambda: MollerPlesset.do_df_rmp2(w), F_mp2, 78.6)
+    E32 = timed("df_rmp2_fp32_mixed", lambda: MollerPlesset.do_df_rmp2(w, prec="fp32_mixed"), F_mp2, 157.3)
+    out["E_df_mp2_fp64"] = E64
+    out["E_df_mp2_fp32_mixed_rel_diff"] = abs(E32 - E64) / abs(E64)
+    for prec in ("fp64", "fp32_mixed", "fp16x3_mixed"):
+        hist = []
+        t0 = time.perf_counter()
+        CC.DFRCCD.do_df_rccd(w, maxit=3, prec=prec, coupling=coupling, callback=hist.append)
+        wall = time.perf_counter() - t0
+        out["stages"][f"df_rccd_sweep_{prec}"] = {"ms_per_sweep": [round(h["ms"], 1) for h in hist],
+                                                  "wall_s_incl_setup": round(wall, 1)}
+        print(f"[bench] df_rccd {prec}: {[round(h['ms']) for h in hist]} ms/sweep", file=sys.stderr, flush=True)
     return out
 
 

@@ -131,6 +131,19 @@ jues_status jues_release(jues_ctx* ctx);
 jues_status jues_set_ao_eri(jues_ctx* ctx, const double* gao, int n);
 jues_status jues_set_ao_factors(jues_ctx* ctx, const double* B, int n, int naux);
 
+/* ---- Density fitting (Backend/DF.jl:30-58) ------------------------------------
+ * Takes the pair setup_df returns (DF.jl:48-50): the three-index integrals
+ * pqP[mu,nu,P] (n x n x naux) and the metric's inverse square root
+ * Jpqh[P,Q] = (P|Q)^(-1/2) (naux x naux), and forms
+ * b[mu,nu,Q] = sum_P pqP[mu,nu,P] Jpqh[P,Q] (make_bmunu, DF.jl:52-58) on the
+ * device; b becomes the current AO factors, so every entry point below then runs
+ * the DF route: jues_rmp2 is do_df_rmp2 (MollerPlesset/DF-RMP2.jl:1-46) and
+ * jues_rccd with the recomputed ladder is do_df_rccd's bvv.bvv contraction
+ * (CoupledCluster/DF-RCCD.jl:207).  jues_get_ao_factors copies the current
+ * factors back (n and naux must match). */
+jues_status jues_set_df_integrals(jues_ctx* ctx, const double* pqP, const double* Jpqh, int n, int naux);
+jues_status jues_get_ao_factors(jues_ctx* ctx, double* B, int n, int naux);
+
 /* ---- Transformation.tei_transform (Backend/Transformation.jl:39-93) ----------
  * out[p,q,r,s] = sum C1[mu,p] C2[nu,q] C3[lam,r] C4[sig,s] g[mu,nu,lam,sig]
  * (chemist order, out is d1 x d2 x d3 x d4).  C_k is n x d_k.  Uses the dense AO

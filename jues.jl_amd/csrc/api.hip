@@ -86,6 +86,36 @@ jues_status jues_set_ao_factors(jues_ctx* ctx, const double* B, int n, int naux)
   JUES_API_END(ctx)
 }
 
+// b[mu,nu,Q] = sum_P pqP[mu,nu,P] Jpqh[P,Q] (DF.jl:52-58) as one FP64 MFMA GEMM of
+// (n^2 x naux) by (naux x naux); the result becomes the context's AO factors.
+jues_status jues_set_df_integrals(jues_ctx* ctx, const double* pqP, const double* Jpqh, int n, int naux) {
+  JUES_API_BEGIN(ctx)
+  JUES_REQUIRE(pqP && Jpqh && n > 0 && naux > 0, "jues_set_df_integrals: bad arguments");
+  sync(ctx);
+  ctx->cc.reset();
+  ctx->gao.release();
+  ctx->gao32.release();
+  ctx->Bfac32.release();
+  ctx->Bfac.release();
+  const long nn = (long)n * n;
+  DeviceArray<double> three(ctx, pqP, (size_t)nn * naux), metric(ctx, Jpqh, (size_t)naux * naux);
+  ctx->Bfac.alloc((size_t)nn * naux * sizeof(double));
+  gemm<double>(ctx, false, false, nn, naux, naux, 1.0, three.p, nn, 0, metric.p, naux, 0, 0.0,
+               ctx->Bfac.as<double>(), nn, 0, 1, "df_bmunu");
+  sync(ctx);
+  ctx->n = n;
+  ctx->naux = naux;
+  JUES_API_END(ctx)
+}
+
+jues_status jues_get_ao_factors(jues_ctx* ctx, double* B, int n, int naux) {
+  JUES_API_BEGIN(ctx)
+  JUES_REQUIRE(B, "jues_get_ao_factors: null pointer");
+  JUES_REQUIRE(ctx->Bfac.p && n == ctx->n && naux == ctx->naux, "jues_get_ao_factors: no factors of that shape");
+  d2h(ctx, B, ctx->Bfac.p, (size_t)n * n * naux * sizeof(double));
+  JUES_API_END(ctx)
+}
+
 jues_status jues_tei_transform(jues_ctx* ctx, int prec, const double* C1, int d1, const double* C2, int d2,
                                const double* C3, int d3, const double* C4, int d4, int n, double* out) {
   JUES_API_BEGIN(ctx)

@@ -48,11 +48,22 @@ template <typename T>
 void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* L) {
   const long n = ctx->n, naux = ctx->naux;
   const T* B = ao_factors<T>(ctx);
-  T* Y = static_cast<T*>(scratch(ctx, 4, sizeof(T) * (size_t)(n * d2 * naux)));
-  // Y[mu, q, P] = B_P C2 ; L[p, q, P] = C1^T Y_P
-  gemm<T>(ctx, false, false, n, d2, n, T(1), B, n, n * n, C2, n, 0, T(0), Y, n, n * d2, (int)naux, "xform_factor");
-  gemm<T>(ctx, true, false, d1, d2, n, T(1), C1, n, 0, Y, n, n * d2, T(0), L, d1, (long)d1 * d2, (int)naux,
-          "xform_factor");
+  // The smaller of d1, d2 is contracted first: 2 n^2 min(d1,d2) naux + 2 n d1 d2 naux flops.
+  if (d1 <= d2) {
+    T* Y = static_cast<T*>(scratch(ctx, 4, sizeof(T) * (size_t)(n * d1 * naux)));
+    // Y[nu, p, P] = B_P^T C1 ; L[p, q, P] = Y_P^T C2
+    gemm<T>(ctx, true, false, n, d1, n, T(1), B, n, n * n, C1, n, 0, T(0), Y, n, n * d1, (int)naux,
+            "xform_factor");
+    gemm<T>(ctx, true, false, d1, d2, n, T(1), Y, n, n * d1, C2, n, 0, T(0), L, d1, (long)d1 * d2, (int)naux,
+            "xform_factor");
+  } else {
+    T* Y = static_cast<T*>(scratch(ctx, 4, sizeof(T) * (size_t)(n * d2 * naux)));
+    // Y[mu, q, P] = B_P C2 ; L[p, q, P] = C1^T Y_P
+    gemm<T>(ctx, false, false, n, d2, n, T(1), B, n, n * n, C2, n, 0, T(0), Y, n, n * d2, (int)naux,
+            "xform_factor");
+    gemm<T>(ctx, true, false, d1, d2, n, T(1), C1, n, 0, Y, n, n * d2, T(0), L, d1, (long)d1 * d2, (int)naux,
+            "xform_factor");
+  }
 }
 
 template <typename T>
@@ -95,9 +106,12 @@ void tei_transform_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, 
     const long naux = ctx->naux;
     JUES_REQUIRE(ctx->Bfac.p, "no AO integrals set");
     T* L = static_cast<T*>(scratch(ctx, 5, sizeof(T) * (size_t)((long)d1 * d2 * naux)));
-    T* R = static_cast<T*>(scratch(ctx, 6, sizeof(T) * (size_t)((long)d3 * d4 * naux)));
     mo_factors_dev<T>(ctx, C1, d1, C2, d2, L);
-    mo_factors_dev<T>(ctx, C3, d3, C4, d4, R);
+    T* R = L;  // (pq|pq)-shaped blocks such as (ia|jb): both sides are the same factors
+    if (C1 != C3 || C2 != C4 || d1 != d3 || d2 != d4) {
+      R = static_cast<T*>(scratch(ctx, 6, sizeof(T) * (size_t)((long)d3 * d4 * naux)));
+      mo_factors_dev<T>(ctx, C3, d3, C4, d4, R);
+    }
     const long m = (long)d1 * d2, nn = (long)d3 * d4;
     gemm<T>(ctx, false, true, m, nn, naux, T(1), L, m, 0, R, nn, 0, T(0), out, m, 0, 1, "xform_factor_gemm");
   }

@@ -1,6 +1,6 @@
-"""Closed-shell coupled cluster on the GPU: the ``RCCD``, ``mRCCD``, ``RCCSD`` and
-``mRCCSD`` modules of CoupledCluster/ (RCCD.jl:33, mRCCD.jl:38, RCCSD.jl:33,
-mRCCSD.jl:35).
+"""Closed-shell coupled cluster on the GPU: the ``RCCD``, ``mRCCD``, ``RCCSD``,
+``mRCCSD`` and ``DFRCCD`` modules of CoupledCluster/ (RCCD.jl:33, mRCCD.jl:38,
+RCCSD.jl:33, mRCCSD.jl:35, DF-RCCD.jl:11).
 
 Each ``do_rccd`` / ``do_rccsd`` takes a ``Wfn`` and returns the correlation
 energy (and the amplitudes when ``return_T2`` / ``return_T`` is set), like the
@@ -18,7 +18,7 @@ import types
 import numpy as np
 
 from . import _lib
-from .Wavefunction import Wfn, ensure_ao, prec_code
+from .Wavefunction import Wfn, ensure_ao, is_factored, prec_code
 
 defaults = dict(cc_max_iter=100, cc_max_rms=1e-9, cc_e_conv=1e-10, diis=True, fcn=0)
 
@@ -112,8 +112,45 @@ def _mrccsd(refWfn: Wfn, **kwargs):
     return _run(1, refWfn, kwargs, "mp2")
 
 
+def _df_rccd(refWfn: Wfn, maxit: int = 40, doprint: bool = False, return_T2: bool = False,
+              dfbname: str = "default", *, prec: str = "fp64", device: int = 0, callback=None, **kwargs):
+    """DF-RCCD.jl:11-53: CCD from the 3-index factors b (``DFIntegrals`` or ``FactoredERI``).
+
+    As the reference: the MP2 guess t = (ia|jb)/D (``T2_init!``, :113-135),
+    exactly ``maxit`` plain Jacobi sweeps with no DIIS and no convergence test
+    (:27-46), and the energy of the last amplitudes (:47-52).  The pp-ladder is
+    recomputed from bvv.bvv every sweep (:207); ``ladder="stored"`` builds V+-
+    once instead (same result).  The residual is RCCD.jl's (:242-256): the last
+    exchange term of WmBeJ is -1/2 sum (ne|mf) t[n,j,f,b] (RCCD.jl:400-401),
+    where DF-RCCD.jl:250 reads (me|nf).  ``prec`` other than "fp64" runs the
+    same ``maxit`` sweeps with the contractions in that form (amplitudes,
+    energies and denominators stay FP64; there is no refinement stage).
+    ``dfbname`` names the auxiliary basis in the reference's psi4 front end;
+    here the basis is fixed by the integrals passed in, so only "default" is
+    accepted.
+    """
+    if dfbname not in ("default", None):
+        raise ValueError("dfbname: the auxiliary basis is the one the DFIntegrals were built in")
+    if not is_factored(refWfn.ao_eri):
+        raise TypeError("do_df_rccd needs DFIntegrals (pqP, Jpqh) or FactoredERI integrals")
+    kw = dict(kwargs)
+    kw.setdefault("ladder", "recompute")
+    kw.update(maxit=maxit, doprint=doprint, diis=False, guess="mp2", cc_e_conv=0.0, cc_max_rms=0.0)
+    solver = Solver(refWfn, method="rccd", prec=prec, device=device, **kw)
+    history = []
+    for _ in range(maxit):
+        info, _ = solver.iterate(1, prec=prec)
+        history.append(info)
+        if callback is not None:
+            callback(info)
+    _run.last_history = history
+    E, T2 = solver.result()
+    return (E, T2) if return_T2 else E
+
+
 RCCD = types.SimpleNamespace(do_rccd=_rccd)
 mRCCD = types.SimpleNamespace(do_rccd=_mrccd)
+DFRCCD = types.SimpleNamespace(do_df_rccd=_df_rccd)
 RCCSD = types.SimpleNamespace(do_rccsd=_rccsd)
 mRCCSD = types.SimpleNamespace(do_rccsd=_mrccsd)
 
@@ -160,4 +197,4 @@ class Solver:
         return (float(E[0]), T2) if self.method == 0 else (float(E[0]), T1, T2)
 
 
-__all__ = ["RCCD", "mRCCD", "RCCSD", "mRCCSD", "Solver", "defaults", "last_history"]
+__all__ = ["RCCD", "mRCCD", "RCCSD", "mRCCSD", "DFRCCD", "Solver", "defaults", "last_history"]

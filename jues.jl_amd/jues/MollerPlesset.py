@@ -1,10 +1,11 @@
-"""``do_rmp2`` (MollerPlesset/RMP2.jl:11-45) and its adjoint on the GPU."""
+"""``do_rmp2`` (MollerPlesset/RMP2.jl:11-45), ``do_df_rmp2``
+(MollerPlesset/DF-RMP2.jl:1-46) and the MP2 adjoint on the GPU."""
 from __future__ import annotations
 
 import numpy as np
 
 from . import _lib
-from .Wavefunction import Wfn, FactoredERI, ensure_ao, prec_code
+from .Wavefunction import Wfn, DFIntegrals, ensure_ao, is_factored, prec_code
 
 
 def do_rmp2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0, **kwargs) -> float:
@@ -19,6 +20,21 @@ def do_rmp2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0, **kwargs) -> fl
     return float(E[0])
 
 
+def do_df_rmp2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0) -> float:
+    """Density-fitted closed-shell MP2 (DF-RMP2.jl:1-46).
+
+    ``refWfn.ao_eri`` must be a ``DFIntegrals`` (pqP, Jpqh), the pair the
+    reference's ``setup_df`` returns (DF-RMP2.jl:7), or a ``FactoredERI``
+    holding b[mu,nu,Q] already contracted with Jpqh (DF-RMP2.jl:13-15).  The
+    energy sum over (i,j) pairs of (ia|jb)(2(ia|jb) - (ib|ja))/D with
+    (ia|jb) = sum_Q b_ia^Q b_jb^Q (DF-RMP2.jl:22-43) is the factor route of
+    ``do_rmp2``: b_ia^Q is formed on the device and never leaves it.
+    """
+    if not is_factored(refWfn.ao_eri):
+        raise TypeError("do_df_rmp2 needs DFIntegrals (pqP, Jpqh) or FactoredERI integrals")
+    return do_rmp2(refWfn, prec=prec, device=device)
+
+
 def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: int = 0):
     """(E, dE/dC (n x n), dE/dg (n^4 or None)) with every C and g entry independent.
 
@@ -31,11 +47,11 @@ def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: 
     E = np.zeros(1)
     dC = np.zeros((n, n), order="F")
     dg = None
-    if want_dg and not isinstance(refWfn.ao_eri, FactoredERI):
+    if want_dg and not is_factored(refWfn.ao_eri):
         dg = np.zeros((n, n, n, n), order="F")
     ctx.check(ctx.lib.jues_rmp2_grad(ctx.h, prec_code(prec), _lib.ptr(C), refWfn.nalpha, n, _lib.ptr(eps),
                                      _lib.ptr(E), _lib.ptr(dC), _lib.ptr(dg)))
     return float(E[0]), dC, dg
 
 
-__all__ = ["do_rmp2", "rmp2_grad"]
+__all__ = ["do_rmp2", "do_df_rmp2", "rmp2_grad"]

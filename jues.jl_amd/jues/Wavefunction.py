@@ -4,7 +4,10 @@ Field names follow the Julia struct.  ``ao_eri`` is either a dense n^4 array
 (chemist order) or a ``FactoredERI`` holding B[mu,nu,P] with
 (mu nu|lam sig) = sum_P B[mu,nu,P] B[lam,sig,P]; the second form plays the
 role the reference gives ``DiskFourTensor`` (Wavefunction.jl:87): an ERI carrier
-for systems whose dense tensor does not fit.
+for systems whose dense tensor does not fit.  A ``DFIntegrals`` holds the pair
+``setup_df`` returns (Backend/DF.jl:30-51), pqP[mu,nu,P] and Jpqh = (P|Q)^(-1/2);
+uploading it forms b = pqP.Jpqh on the device and from then on it is the
+factored form.
 """
 from __future__ import annotations
 
@@ -23,6 +26,16 @@ class FactoredERI:
     @property
     def n(self) -> int:
         return self.B.shape[0]
+
+
+@dataclasses.dataclass
+class DFIntegrals:
+    pqP: np.ndarray   # (n, n, naux) three-index integrals (mu nu|P)
+    Jpqh: np.ndarray  # (naux, naux) inverse square root of the metric (P|Q)
+
+    @property
+    def n(self) -> int:
+        return self.pqP.shape[0]
 
 
 @dataclasses.dataclass
@@ -46,7 +59,7 @@ class Wfn:
     hao: Optional[np.ndarray]
     epsa: np.ndarray
     epsb: np.ndarray
-    ao_eri: Union[np.ndarray, FactoredERI]
+    ao_eri: Union[np.ndarray, FactoredERI, DFIntegrals]
 
     @classmethod
     def from_orbitals(cls, C: np.ndarray, eps: np.ndarray, nocc: int, ao_eri, energy: float = 0.0,
@@ -70,6 +83,8 @@ class Wfn:
 def _ao_key(eri) -> tuple:
     if isinstance(eri, FactoredERI):
         return ("factors", id(eri.B), eri.B.shape)
+    if isinstance(eri, DFIntegrals):
+        return ("df", id(eri.pqP), id(eri.Jpqh), eri.pqP.shape)
     return ("dense", id(eri), eri.shape)
 
 
@@ -85,14 +100,24 @@ def ensure_ao(ctx: "_lib.Context", eri, force: bool = False) -> int:
     if force or ctx.ao_key != key or getattr(ctx, "ao_ref", None) is not _ao_obj(eri):
         if isinstance(eri, FactoredERI):
             ctx.set_ao_factors(eri.B, key)
+        elif isinstance(eri, DFIntegrals):
+            ctx.set_df_integrals(eri.pqP, eri.Jpqh, key)
         else:
             ctx.set_ao_eri(eri, key)
         ctx.ao_ref = _ao_obj(eri)
-    return eri.n if isinstance(eri, FactoredERI) else eri.shape[0]
+    return eri.n if isinstance(eri, (FactoredERI, DFIntegrals)) else eri.shape[0]
+
+
+def is_factored(eri) -> bool:
+    return isinstance(eri, (FactoredERI, DFIntegrals))
 
 
 def _ao_obj(eri):
-    return eri.B if isinstance(eri, FactoredERI) else eri
+    if isinstance(eri, FactoredERI):
+        return eri.B
+    if isinstance(eri, DFIntegrals):
+        return eri.pqP
+    return eri
 
 
 def invalidate(device: int = 0) -> None:

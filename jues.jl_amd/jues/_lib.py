@@ -25,7 +25,8 @@ CCSD_EQS = {"mRCCSD": 0, "RCCSD": 1}
 # names every build must export (checked by the CPU test-suite against include/jues.h)
 EXPORTS = (
     "jues_create", "jues_destroy", "jues_last_error", "jues_cc_default_opts", "jues_device_bytes",
-    "jues_release", "jues_set_ao_eri", "jues_set_ao_factors", "jues_tei_transform", "jues_get_eri",
+    "jues_release", "jues_set_ao_eri", "jues_set_ao_factors", "jues_set_df_integrals", "jues_get_ao_factors",
+    "jues_tei_transform", "jues_get_eri",
     "jues_rmp2", "jues_rmp2_grad", "jues_rccd", "jues_rccsd", "jues_cc_setup", "jues_cc_iterate",
     "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test",
 )
@@ -89,6 +90,8 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "jues_release": (i, [_ctxp]),
             "jues_set_ao_eri": (i, [_ctxp, _dp, i]),
             "jues_set_ao_factors": (i, [_ctxp, _dp, i, i]),
+            "jues_set_df_integrals": (i, [_ctxp, _dp, _dp, i, i]),
+            "jues_get_ao_factors": (i, [_ctxp, _dp, i, i]),
             "jues_tei_transform": (i, [_ctxp, i, _dp, i, _dp, i, _dp, i, _dp, i, i, _dp]),
             "jues_get_eri": (i, [_ctxp, i, C.c_char_p, C.c_char_p, _dp, i, _dp, i, i, i, _dp]),
             "jues_rmp2": (i, [_ctxp, i, _dp, i, _dp, i, i, _dp, _dp]),
@@ -180,6 +183,19 @@ class Context:
         assert n == n2
         self.check(self.lib.jues_set_ao_factors(self.h, ptr(b), n, naux))
         self.ao_key = key
+
+    def set_df_integrals(self, pqP: np.ndarray, Jpqh: np.ndarray, key=None):
+        t, J = f64(pqP), f64(Jpqh)
+        n, n2, naux = t.shape
+        if n != n2 or J.shape != (naux, naux):
+            raise ValueError(f"pqP {t.shape} and Jpqh {J.shape} do not match")
+        self.check(self.lib.jues_set_df_integrals(self.h, ptr(t), ptr(J), n, naux))
+        self.ao_key = key
+
+    def get_ao_factors(self, n: int, naux: int) -> np.ndarray:
+        B = np.zeros((n, n, naux), order="F")
+        self.check(self.lib.jues_get_ao_factors(self.h, ptr(B), n, naux))
+        return B
 
     # -- kernel timing ---------------------------------------------------------
     def stats_reset(self, enable: bool = True):

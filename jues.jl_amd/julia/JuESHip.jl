@@ -111,6 +111,37 @@ function ensure_ao!(ctx::Context, f::FactoredERI)
     size(f.B, 1)
 end
 
+"""The pair `setup_df` returns (Backend/DF.jl:30-51): pqP[mu,nu,P] and Jpqh = (P|Q)^(-1/2).
+Uploading it forms b = pqP*Jpqh on the device (make_bμν, DF.jl:52-58)."""
+struct DFIntegrals
+    pqP::Array{Float64,3}
+    Jpqh::Matrix{Float64}
+end
+
+function ensure_ao!(ctx::Context, d::DFIntegrals)
+    if ctx.ao_key !== d
+        n, _, naux = size(d.pqP)
+        size(d.Jpqh) == (naux, naux) || error("Jpqh must be naux x naux")
+        GC.@preserve d check(ctx, ccall((:jues_set_df_integrals, libjues), Cint,
+                                       (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Cint),
+                                       ctx.handle, d.pqP, d.Jpqh, n, naux))
+        ctx.ao_key = d
+    end
+    size(d.pqP, 1)
+end
+
+"make_bμν (DF.jl:52-58) on the device."
+function make_bμν(pqP::Array{Float64,3}, Jpqh::Matrix{Float64})
+    ctx = context()
+    d = DFIntegrals(pqP, Jpqh)
+    n = ensure_ao!(ctx, d)
+    naux = size(pqP, 3)
+    B = zeros(n, n, naux)
+    GC.@preserve B check(ctx, ccall((:jues_get_ao_factors, libjues), Cint,
+                                    (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint), ctx.handle, B, n, naux))
+    return B
+end
+
 # ---------------------------------------------------------------------------
 # Transformation.tei_transform   (Backend/Transformation.jl:15-93)
 # ---------------------------------------------------------------------------
@@ -160,9 +191,9 @@ end
 # MollerPlesset.do_rmp2   (MollerPlesset/RMP2.jl:11-45) and its adjoint
 # ---------------------------------------------------------------------------
 module MollerPlesset
-import ..JuESHip: context, ensure_ao!, check, libjues, precision_code
+import ..JuESHip: context, ensure_ao!, check, libjues, precision_code, FactoredERI, DFIntegrals
 using ChainRulesCore
-export do_rmp2, rmp2_energy
+export do_rmp2, do_df_rmp2, rmp2_energy
 
 function do_rmp2(refWfn; precision=:fp64, kwargs...)
     ctx = context()
@@ -173,6 +204,12 @@ function do_rmp2(refWfn; precision=:fp64, kwargs...)
         (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}),
         ctx.handle, precision_code(precision), Co, refWfn.nalpha, Cv, refWfn.nvira, n, eps, E))
     return E[]
+end
+
+"DF-RMP2.jl:1-46: the factor route of do_rmp2 on `ao_eri::Union{DFIntegrals,FactoredERI}`."
+function do_df_rmp2(refWfn; precision=:fp64)
+    refWfn.ao_eri isa Union{DFIntegrals,FactoredERI} || error("do_df_rmp2 needs DFIntegrals or FactoredERI")
+    return do_rmp2(refWfn; precision=precision)
 end
 
 """
@@ -218,7 +255,7 @@ end
 # CoupledCluster.{RCCD,mRCCD,RCCSD,mRCCSD}
 # ---------------------------------------------------------------------------
 module CoupledCluster
-import ..JuESHip: context, ensure_ao!, check, libjues, precision_code, CCOpts
+import ..JuESHip: context, ensure_ao!, check, libjues, precision_code, CCOpts, FactoredERI, DFIntegrals
 
 function _cc(method, refWfn; precision=:fp64, return_T=false, check_convergence=true,
              default_guess=:mp2, kwargs...)
@@ -276,6 +313,23 @@ module mRCCSD
 import ..CoupledCluster: _cc
 "mRCCSD.jl:35-101 (Fae T1 term of mRCCSD.jl:140)"
 do_rccsd(refWfn; return_T=false, kwargs...) = _cc(1, refWfn; return_T=return_T, ccsd_eqs=:mRCCSD, kwargs...)
+end
+
+module DFRCCD
+import ..CoupledCluster: _cc, FactoredERI, DFIntegrals
+"""DF-RCCD.jl:11-53: MP2 guess, exactly `maxit` Jacobi sweeps (no DIIS, no convergence
+test), pp-ladder recomputed from bvv*bvv; the residual is RCCD.jl's (its WmBeJ exchange
+term, RCCD.jl:400-401, where DF-RCCD.jl:250 reads (me|nf))."""
+function do_df_rccd(refWfn; maxit=40, doprint=false, return_T2=false, dfbname="default", precision=:fp64,
+                    kwargs...)
+    dfbname == "default" || error("dfbname: the auxiliary basis is the one the DFIntegrals were built in")
+    precision === :fp64 || error("do_df_rccd: fixed sweeps run in FP64 here (the Python Solver steps the fast forms)")
+    refWfn.ao_eri isa Union{DFIntegrals,FactoredERI} || error("do_df_rccd needs DFIntegrals or FactoredERI")
+    kw = Dict{Symbol,Any}(kwargs)
+    haskey(kw, :ladder) || (kw[:ladder] = :recompute)
+    _cc(0, refWfn; return_T=return_T2, default_guess=:mp2, maxit=maxit, doprint=doprint, diis=false,
+        cc_e_conv=0.0, cc_max_rms=0.0, check_convergence=false, kw...)
+end
 end
 end
 

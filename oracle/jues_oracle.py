@@ -24,6 +24,12 @@ How each part is pinned (the checks live in ``tests/test_oracle.py``):
 * ``rccsd_*``: mRCCSD.jl 6-block and RCCSD.jl 15-block restatements agree;
   with ooov = ovvv = 0 RCCSD reduces to RCCD exactly.
 * ``rmp2_grad``: central finite differences.
+* ``make_bmunu`` / ``df_rmp2`` / ``df_rccd_*`` (DF.jl, DF-RMP2.jl, DF-RCCD.jl):
+  b . J^(1/2) round trip and the fitted (mu nu|lam sig) = pqP J^-1 pqP^T closed
+  form; DF-MP2 equals ``rmp2`` on the fitted four-index integrals; one DF sweep
+  equals ``rccd_step`` (RCCD.jl) on the same integrals to 1e-12, while the
+  WmBeJ term of DF-RCCD.jl:250 as written does not; the MP2 guess's energy is
+  the DF-MP2 energy.
 * Reference known answers (water STO-3G) through ``oracle/sto3g.py``.
 """
 from __future__ import annotations
@@ -659,3 +665,94 @@ def solve_rccsd(I, eps, o, *, maxit=200, e_conv=1e-13, r_conv=1e-12, diis=True, 
         if abs(dE) < e_conv and rn < r_conv:
             break
     return ccsd_energy(oovv, t1, t2), t1, t2, it + 1
+
+
+# ---------------------------------------------------------------------------
+# Density fitting: Backend/DF.jl:52-58, MollerPlesset/DF-RMP2.jl:1-46,
+# CoupledCluster/DF-RCCD.jl:11-271
+# ---------------------------------------------------------------------------
+
+def metric_inv_sqrt(J):
+    """``Jpq^(-1/2)`` of DF.jl:50 for a symmetric positive definite metric (eigen route)."""
+    w, V = np.linalg.eigh(J)
+    return (V / np.sqrt(w)) @ V.T
+
+
+def make_bmunu(pqP, Jpqh):
+    """DF.jl:52-58: ``b[p,q,Q] = sum_P pqP[p,q,P] Jpqh[P,Q]``."""
+    return np.tensordot(pqP, Jpqh, axes=(2, 0))
+
+
+def df_rmp2(b, Co, Cv, eps):
+    """DF-RMP2.jl:17-43: half transforms to b_ia^Q, then the (i,j) pair loop with
+    bAB[a,b] = sum_Q b[i,a,Q] b[j,b,Q], ibja = bAB[b,a]."""
+    o = Co.shape[1]
+    biv = _es("mi,mnQ->inQ", Co, b)                                   # :17-19
+    bia = _es("na,inQ->iaQ", Cv, biv)                                 # :20-22
+    ev = eps[o:o + Cv.shape[1]]
+    E = 0.0
+    for i in range(o):
+        for j in range(o):
+            bAB = bia[i] @ bia[j].T                                   # :33-35
+            bBA = bAB.T                                               # :36
+            D = eps[i] + eps[j] - ev[:, None] - ev[None, :]
+            E += float(np.sum(bAB * (2.0 * bAB - bBA) / D))           # :37-42
+    return E
+
+
+def df_rccd_integrals(b, Cao, Cav):
+    """DF-RCCD.jl:78-89: bov[i,a,Q], bvo[a,i,Q], boo[i,j,Q], bvv[a,b,Q]."""
+    h = lambda C1, C2: _es("mp,mnQ,nq->pqQ", C1, b, C2)  # noqa: E731
+    return dict(bov=h(Cao, Cav), bvo=h(Cav, Cao), boo=h(Cao, Cao), bvv=h(Cav, Cav))
+
+
+def df_ccenergy(t, bov):
+    """DF-RCCD.jl:91-111: ``sum bAB[a,b](2t[i,j,a,b] - t[j,i,a,b])``."""
+    iajb = _es("iaQ,jbQ->ijab", bov, bov)
+    return float(np.sum(iajb * (2.0 * t - t.transpose(1, 0, 2, 3))))
+
+
+def df_rccd_sweep(t, b, D, literal=False):
+    """One ``cciter`` (DF-RCCD.jl:55-76): the intermediates of :137-271 and
+    ``form_T2`` (:172-219), every integral contracted from the factors.
+
+    ``literal=True`` keeps DF-RCCD.jl:250 as written: the last WmBeJ term is
+    -1/2 sum (me|nf) t[n,j,f,b].  The default uses (ne|mf) there, the
+    <nm|ef> of RCCD.jl:400-401 (the two differ unless (me|nf) = (ne|mf)).
+    """
+    bov, bvo, boo, bvv = b["bov"], b["bvo"], b["boo"], b["bvv"]
+    menf = _es("meQ,nfQ->menf", bov, bov)                                    # (me|nf)
+    tx = t.transpose(1, 0, 2, 3)
+    Fae = -_es("menf,mnaf->ae", menf, 2 * t - tx)                              # :148-150
+    Fmi = _es("menf,inef->mi", menf, 2 * t - t.transpose(0, 1, 3, 2))          # :166-168
+    Wmnij = _es("miQ,njQ->mnij", boo, boo) + _es("ijef,menf->mnij", t, menf) / 2   # :233-235
+    exch = menf if literal else menf.transpose(2, 1, 0, 3)                      # (me|nf) | (ne|mf)
+    WmBeJ = (_es("meQ,bjQ->mbej", bov, bvo) + _es("menf,njfb->mbej", menf, 2 * t - tx) / 2
+             - _es("menf,njfb->mbej", exch, t) / 2)                             # :248-252
+    WmBEj = (-_es("mjQ,beQ->mbej", boo, bvv)
+             + _es("jnfb,nemf->mbej", t, menf) / 2)                             # :268-271
+    R = (_es("iaQ,jbQ->ijab", bov, bov) + _es("ijae,be->ijab", t, Fae) + _es("jibe,ae->ijab", t, Fae)
+         - _es("imab,mj->ijab", t, Fmi) - _es("mjab,mi->ijab", t, Fmi)
+         + _es("mnab,mnij->ijab", t, Wmnij)
+         + _es("ijef,mnab,menf->ijab", t, t, menf) / 2
+         + _es("ijef,aeQ,bfQ->ijab", t, bvv, bvv)                                # :207 ladder
+         + 2 * _es("imae,mbej->ijab", t, WmBeJ) - _es("miae,mbej->ijab", t, WmBeJ)
+         + _es("imae,mbej->ijab", t, WmBEj) + _es("mibe,maej->ijab", t, WmBEj)
+         + _es("mjae,mbei->ijab", t, WmBEj)
+         + 2 * _es("jmbe,maei->ijab", t, WmBeJ) - _es("mjbe,maei->ijab", t, WmBeJ)
+         + _es("jmbe,maei->ijab", t, WmBEj))                                    # :198-216
+    return R / D                                                               # :217
+
+
+def df_rccd(bmunu, Cao, Cav, eps, maxit=40, literal=False):
+    """DF-RCCD.jl:11-53: T2_init! (:113-135), ``maxit`` sweeps with no DIIS and no
+    convergence test, energy of the last amplitudes.  Returns (E, T2, energies)."""
+    o, v = Cao.shape[1], Cav.shape[1]
+    b = df_rccd_integrals(bmunu, Cao, Cav)
+    D = form_Dijab(eps, o, v)
+    t = _es("iaQ,jbQ->ijab", b["bov"], b["bov"]) / D
+    hist = []
+    for _ in range(maxit):
+        t = df_rccd_sweep(t, b, D, literal)
+        hist.append(df_ccenergy(t, b["bov"]))
+    return df_ccenergy(t, b["bov"]), t, hist

@@ -62,6 +62,21 @@ def test_tei_transform_general_tensor(ctx):
     assert rel_err(out, ref) < 1e-12
 
 
+@pytest.mark.parametrize("dims", [(3, 4, 5, 2), (4, 3, 2, 5), (6, 6, 6, 6)])
+def test_tei_transform_nonsymmetric_factors(ctx, dims):
+    """Factor route with B[mu,nu,P] != B[nu,mu,P]: both contraction orders of the
+    half-transform (d1 <= d2 and d1 > d2) against the exact 3-index sum."""
+    from jues import Transformation
+    from jues.Wavefunction import FactoredERI
+    rng = np.random.default_rng(6)
+    n, naux = 9, 7
+    B = np.asfortranarray(rng.normal(size=(n, n, naux)))
+    Cs = [rng.normal(size=(n, d)) for d in dims]
+    ref = O.mo_from_factors(B, *Cs)
+    out = Transformation.tei_transform(FactoredERI(B), *Cs)
+    assert rel_err(out, ref) < 1e-12
+
+
 @pytest.mark.parametrize("dense", [True, False])
 def test_get_eri(cfg1, dense):
     from jues import IntegralTransformation as IT
