@@ -105,6 +105,10 @@ def main():
     ap.add_argument("--df", action="store_true",
                     help="instead of RCCD: density fitting at o=60 v=340 naux=1200 (make_bmunu, DF-RMP2, "
                          "DF-RCCD sweeps)")
+    ap.add_argument("--triples", action="store_true",
+                    help="instead of RCCD: (T) on the o=60 v=340 factors with seeded amplitudes")
+    ap.add_argument("--pt-o", type=int, default=60, help="--triples: occupied orbitals used (first pt-o)")
+    ap.add_argument("--pt-prec", default="fp64,fp32_mixed", help="--triples: forms to run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -115,6 +119,9 @@ def main():
         return
     if args.df:
         print(json.dumps(df_bench(args.coupling)), flush=True)
+        return
+    if args.triples:
+        print(json.dumps(triples_bench(args.pt_o, args.pt_prec.split(","))), flush=True)
         return
 
     from jues import synthetic, _lib
@@ -304,6 +311,51 @@ def df_bench(coupling: float) -> dict:
         out["stages"][f"df_rccd_sweep_{prec}"] = {"ms_per_sweep": [round(h["ms"], 1) for h in hist],
                                                   "wall_s_incl_setup": round(wall, 1)}
         print(f"[bench] df_rccd {prec}: {[round(h['ms']) for h in hist]} ms/sweep", file=sys.stderr, flush=True)
+    return out
+
+
+def triples_bench(o: int, forms) -> dict:
+    """(T) on the factors of the n=400, v=340 system with the first ``o`` occupied
+    orbitals and seeded pair-symmetric amplitudes (|t1| ~ |t2| ~ 1e-2); the integral
+    blocks are built on the device.  Device times from HIP events."""
+    import numpy as np
+    from jues import synthetic, _lib
+    from jues import CoupledCluster as CC
+    from jues.Wavefunction import Wfn, FactoredERI
+    s = synthetic.workload("cfg4_rccd_n400")
+    v = s.v
+    import dataclasses
+    w = Wfn.from_orbitals(s.C, s.eps, s.o, FactoredERI(s.B))
+    w = dataclasses.replace(w, nalpha=o, nbeta=o, Cao=np.asfortranarray(s.C[:, :o]), Cbo=np.asfortranarray(s.C[:, :o]),
+                            epsa=np.concatenate([s.eps[:o], s.eps[s.o:]]))
+    rng = np.random.default_rng(17)
+    T1 = rng.standard_normal((o, v)) * 1e-2
+    T2 = rng.standard_normal((o, o, v, v)) * 1e-2
+    T2 = (T2 + T2.transpose(1, 0, 3, 2)) / 2
+    ctx = _lib.context(0)
+    ntrip = o * (o + 1) * (o + 2) // 6
+    flops_w = ntrip * (12.0 * v**4 + 12.0 * v**3 * o)
+    out = {"workload": f"(T), o={o} v={v} (n=400 factors), {ntrip} occupied triples i>=j>=k",
+           "flops_W": flops_w, "forms": {}}
+    for prec in forms:
+        ctx.stats_reset(True)
+        t0 = time.perf_counter()
+        Et = CC.PerturbativeTriples.rccsd_pT(w, T1, T2, prec=prec, doprint=True)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.stats_reset(False)
+        gw = sum(x["ms"] for k, x in st.items() if k.startswith("pt_w"))
+        peak = 78.6 if prec == "fp64" else 157.3
+        out["forms"][prec] = {"Et": Et, "wall_s": round(wall, 2), "W_gemm_ms": round(gw, 1),
+                              "W_TFLOP/s": round(flops_w / gw / 1e9, 1),
+                              "W_frac_of_peak": round(flops_w / gw / 1e9 / peak, 3),
+                              "ms_per_triple": round(wall * 1e3 / ntrip, 3),
+                              "kernels_ms": {k: round(x["ms"], 1) for k, x in st.items()}}
+        print(f"[bench] (T) {prec}: Et={Et:.12e} wall {wall:.1f} s, W GEMMs {flops_w / gw / 1e9:.1f} TFLOP/s",
+              file=sys.stderr, flush=True)
+    if "fp64" in out["forms"] and "fp32_mixed" in out["forms"]:
+        e64, e32 = out["forms"]["fp64"]["Et"], out["forms"]["fp32_mixed"]["Et"]
+        out["fp32_rel_diff"] = abs(e32 - e64) / abs(e64)
     return out
 
 

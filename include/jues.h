@@ -189,6 +189,21 @@ jues_status jues_rccsd(jues_ctx* ctx, int prec, const jues_cc_opts* opts,
                        const double* eps, double* E, double* T1_inout, double* T2_inout,
                        jues_iter_cb cb, void* user);
 
+/* ---- PerturbativeTriples.compute_pT (CoupledCluster/PerturbativeTriples.jl:35-138) ----
+ * Et = (T) correction from the CCSD amplitudes T1 (o x v), T2 (o x o x v x v) and the
+ * integral arrays AutoRCCSD.jl:295-297 passes: Vvvvo (v x v x v x o), Vvooo (v x o x o x o),
+ * Vvovo (v x o x v x o), with fo (o) and fv (v) the diagonal Fock elements.  The loop of
+ * :79-133 (i >= j >= k, a >= b >= c, weights (2-d_ij-d_jk)/(1+d_ab+d_bc)) runs on the
+ * device; FP32_MIXED forms W in FP32 MFMA and sums the energy in FP64.
+ * jues_rccsd_pT builds the same three blocks on the device from the context's AO
+ * integrals (dense or factors) and canonical orbitals, fo/fv = eps (AutoRCCSD.jl:294-300).
+ * verbose != 0 prints one progress line per occupied orbital i to stderr, as :81 does. */
+jues_status jues_compute_pT(jues_ctx* ctx, int prec, const double* T1, const double* T2, const double* Vvvvo,
+                            const double* Vvooo, const double* Vvovo, const double* fo, const double* fv, int o,
+                            int v, int verbose, double* Et);
+jues_status jues_rccsd_pT(jues_ctx* ctx, int prec, const double* Cocc, int o, const double* Cvir, int v, int n,
+                          const double* eps, const double* T1, const double* T2, int verbose, double* Et);
+
 /* ---- stepwise solver (the same loop as jues_rccd / jues_rccsd, split so that a
  * caller can time iterations with every input already resident on the device) ----
  * method: 0 = RCCD, 1 = RCCSD.  setup builds the MO integral blocks on the device and

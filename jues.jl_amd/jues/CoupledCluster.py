@@ -1,6 +1,7 @@
 """Closed-shell coupled cluster on the GPU: the ``RCCD``, ``mRCCD``, ``RCCSD``,
-``mRCCSD`` and ``DFRCCD`` modules of CoupledCluster/ (RCCD.jl:33, mRCCD.jl:38,
-RCCSD.jl:33, mRCCSD.jl:35, DF-RCCD.jl:11).
+``mRCCSD``, ``DFRCCD`` and ``PerturbativeTriples`` modules of CoupledCluster/
+(RCCD.jl:33, mRCCD.jl:38, RCCSD.jl:33, mRCCSD.jl:35, DF-RCCD.jl:11,
+PerturbativeTriples.jl:35).
 
 Each ``do_rccd`` / ``do_rccsd`` takes a ``Wfn`` and returns the correlation
 energy (and the amplitudes when ``return_T2`` / ``return_T`` is set), like the
@@ -148,6 +149,50 @@ def _df_rccd(refWfn: Wfn, maxit: int = 40, doprint: bool = False, return_T2: boo
     return (E, T2) if return_T2 else E
 
 
+def _compute_pT(*, T1, T2, Vvvvo, Vvooo, Vvovo, fo, fv, prec: str = "fp64", device: int = 0,
+                doprint: bool = False) -> float:
+    """PerturbativeTriples.compute_pT (PerturbativeTriples.jl:35-138) on the GPU.
+
+    Same keyword arguments as the reference: the CCSD amplitudes and the integral
+    arrays AutoRCCSD.jl:295-297 passes (Vvvvo v x v x v x o, Vvooo v x o x o x o,
+    Vvovo v x o x v x o), fo/fv the diagonal Fock elements.  ``prec="fp32_mixed"``
+    forms W in FP32 MFMA; the energy sums stay FP64.
+    """
+    ctx = _lib.context(device)
+    T1, T2 = _lib.f64(T1), _lib.f64(T2)
+    o, v = T1.shape
+    arrs = [_lib.f64(x) for x in (Vvvvo, Vvooo, Vvovo, fo, fv)]
+    shapes = [(v, v, v, o), (v, o, o, o), (v, o, v, o), (o,), (v,)]
+    for a, sh, nm in zip(arrs, shapes, ("Vvvvo", "Vvooo", "Vvovo", "fo", "fv")):
+        if a.shape != sh:
+            raise ValueError(f"{nm} has shape {a.shape}, expected {sh}")
+    if T2.shape != (o, o, v, v):
+        raise ValueError(f"T2 has shape {T2.shape}, expected {(o, o, v, v)}")
+    Et = np.zeros(1)
+    ctx.check(ctx.lib.jues_compute_pT(ctx.h, prec_code(prec), _lib.ptr(T1), _lib.ptr(T2),
+                                      *[_lib.ptr(a) for a in arrs], o, v, int(doprint), _lib.ptr(Et)))
+    return float(Et[0])
+
+
+def _rccsd_pT(refWfn: Wfn, T1, T2, *, prec: str = "fp64", device: int = 0, doprint: bool = False) -> float:
+    """(T) after a closed-shell CCSD on canonical orbitals (AutoRCCSD.jl:294-300): the
+    Vvvvo/Vvooo/Vvovo blocks are built on the device from ``refWfn.ao_eri`` (dense or
+    factored) and fo/fv = epsa."""
+    ctx = _lib.context(device)
+    n = ensure_ao(ctx, refWfn.ao_eri)
+    T1, T2 = _lib.f64(T1), _lib.f64(T2)
+    o, v = refWfn.nalpha, refWfn.nvira
+    if T1.shape != (o, v) or T2.shape != (o, o, v, v):
+        raise ValueError("T1/T2 do not match the Wfn's occupied/virtual sizes (no frozen core here)")
+    Co, Cv, eps = _lib.f64(refWfn.Cao), _lib.f64(refWfn.Cav), _lib.f64(refWfn.epsa)
+    Et = np.zeros(1)
+    ctx.check(ctx.lib.jues_rccsd_pT(ctx.h, prec_code(prec), _lib.ptr(Co), o, _lib.ptr(Cv), v, n, _lib.ptr(eps),
+                                    _lib.ptr(T1), _lib.ptr(T2), int(doprint), _lib.ptr(Et)))
+    return float(Et[0])
+
+
+PerturbativeTriples = types.SimpleNamespace(compute_pT=_compute_pT, rccsd_pT=_rccsd_pT)
+
 RCCD = types.SimpleNamespace(do_rccd=_rccd)
 mRCCD = types.SimpleNamespace(do_rccd=_mrccd)
 DFRCCD = types.SimpleNamespace(do_df_rccd=_df_rccd)
@@ -197,4 +242,5 @@ class Solver:
         return (float(E[0]), T2) if self.method == 0 else (float(E[0]), T1, T2)
 
 
-__all__ = ["RCCD", "mRCCD", "RCCSD", "mRCCSD", "DFRCCD", "Solver", "defaults", "last_history"]
+__all__ = ["RCCD", "mRCCD", "RCCSD", "mRCCSD", "DFRCCD", "PerturbativeTriples", "Solver", "defaults",
+           "last_history"]

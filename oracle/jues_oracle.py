@@ -756,3 +756,110 @@ def df_rccd(bmunu, Cao, Cav, eps, maxit=40, literal=False):
         t = df_rccd_sweep(t, b, D, literal)
         hist.append(df_ccenergy(t, b["bov"]))
     return df_ccenergy(t, b["bov"]), t, hist
+
+
+# ---------------------------------------------------------------------------
+# CoupledCluster/PerturbativeTriples.jl:35-138 and AutoRCCSD.jl:294-300
+# ---------------------------------------------------------------------------
+
+def pt_integrals(tei, Cao, Cav):
+    """AutoRCCSD.jl:239-240,295-297: the phys blocks OOOV, OOVV, OVVV permuted into
+    Vvvvo[a,b,c,i] = <ib|ca> = (ic|ba), Vvooo[a,i,j,k] = <ji|ka> = (jk|ia),
+    Vvovo[a,i,b,j] = <ij|ab> = (ia|jb).  ``tei(C1,C2,C3,C4)`` returns chemist
+    (pq|rs) over the four orbital blocks."""
+    OOOV = np.transpose(tei(Cao, Cao, Cao, Cav), (0, 2, 1, 3))    # <ij|ka>
+    OOVV = np.transpose(tei(Cao, Cav, Cao, Cav), (0, 2, 1, 3))    # <ij|ab>
+    OVVV = np.transpose(tei(Cao, Cav, Cav, Cav), (0, 2, 1, 3))    # <ia|bc>
+    Vvvvo = _pd(OVVV, (4, 2, 3, 1))
+    Vvooo = _pd(OOOV, (4, 2, 1, 3))
+    Vvovo = _pd(OOVV, (3, 1, 4, 2))
+    return Vvvvo, Vvooo, Vvovo
+
+
+def _pt_W(T2, Vvvvo, Vvooo, i, j, k):
+    """PerturbativeTriples.jl:105-111: W[a,b,c] for one occupied triple."""
+    T = lambda p, q: T2[p, q]                                  # noqa: E731  [c,d] blocks
+    Ti = lambda p: T2[p]                                       # noqa: E731  [l,a,b]
+    Vo = lambda p, q: Vvooo[:, p, q, :]                        # noqa: E731  [c,l]
+    Vv = lambda p: Vvvvo[:, :, :, p]                           # noqa: E731  [b,d,a]
+    return (_es("bda,cd->abc", Vv(i), T(k, j)) - _es("cl,lab->abc", Vo(k, j), Ti(i))
+            + _es("cda,bd->abc", Vv(i), T(j, k)) - _es("bl,lac->abc", Vo(j, k), Ti(i))
+            + _es("adc,bd->abc", Vv(k), T(j, i)) - _es("bl,lca->abc", Vo(j, i), Ti(k))
+            + _es("bdc,ad->abc", Vv(k), T(i, j)) - _es("al,lcb->abc", Vo(i, j), Ti(k))
+            + _es("cdb,ad->abc", Vv(j), T(i, k)) - _es("al,lbc->abc", Vo(i, k), Ti(j))
+            + _es("adb,cd->abc", Vv(j), T(k, i)) - _es("cl,lba->abc", Vo(k, i), Ti(j)))
+
+
+def _pt_summand(W, V):
+    """PerturbativeTriples.jl:118-125 for every (a,b,c): the bracket E before the
+    division by D; symmetric under any permutation of (a,b,c)."""
+    P = lambda A, s: np.einsum(s + "->abc", A)                 # noqa: E731
+    Wc = W + P(W, "bca") + P(W, "cab")                         # W[abc]+W[bca]+W[cab]
+    Wa = P(W, "acb") + P(W, "bac") + P(W, "cba")
+    Y = V + P(V, "bca") + P(V, "cab")
+    Z = P(V, "acb") + P(V, "bac") + P(V, "cba")
+    X = sum(P(W, s) * P(V, s) for s in ("abc", "acb", "bac", "bca", "cab", "cba"))
+    return (Y - 2 * Z) * Wc + (Z - 2 * Y) * Wa + 3 * X
+
+
+def compute_pT(T1, T2, Vvvvo, Vvooo, Vvovo, fo, fv):
+    """PerturbativeTriples.jl:35-138: i>=j>=k, W and V per triple (:105-113), the
+    a>=b>=c sum with weights (2-d_ij-d_jk)/(D (1+d_ab+d_bc)) (:116-130)."""
+    o, v = T1.shape
+    a, b, c = np.meshgrid(np.arange(v), np.arange(v), np.arange(v), indexing="ij")
+    mask = (a >= b) & (b >= c)
+    wabc = np.where(mask, 1.0 / (1.0 + (a == b) + (b == c)), 0.0)
+    Et = 0.0
+    for i in range(o):
+        for j in range(i + 1):
+            for k in range(j + 1):
+                W = _pt_W(T2, Vvvvo, Vvooo, i, j, k)
+                V = (W + _es("bc,a->abc", Vvovo[:, j, :, k], T1[i]) + _es("ac,b->abc", Vvovo[:, i, :, k], T1[j])
+                     + _es("ab,c->abc", Vvovo[:, i, :, j], T1[k]))                       # :112
+                D = fo[i] + fo[j] + fo[k] - fv[:, None, None] - fv[None, :, None] - fv[None, None, :]
+                E = _pt_summand(W, V)
+                Et += (2 - (i == j) - (j == k)) * float(np.sum(E * wabc / D))
+    return Et
+
+
+def pT_spin_orbital(T1, T2, g_mo, eps, o):
+    """Independent check of ``compute_pT``: the spin-orbital (T) of Raghavachari et al.
+    (Chem. Phys. Lett. 157, 479 (1989)) in the form
+    D t_c = P(i/jk)P(a/bc)[sum_e t_jk^ae <ei||bc> - sum_m t_im^bc <ma||jk>],
+    D t_d = P(i/jk)P(a/bc) t_i^a <jk||bc>,  E = (1/36) sum t_c D (t_c + t_d),
+    with P(i/jk) f = f(ijk) - f(jik) - f(kji).  Spin orbitals are built from the
+    closed-shell T1, T2 (pair-symmetric), chemist MO integrals g_mo over all
+    orbitals and canonical orbital energies eps."""
+    n = g_mo.shape[0]
+    ns = 2 * n
+    spin = np.arange(ns) % 2
+    sp = np.arange(ns) // 2
+    # <PQ|RS> = (pr|qs) delta(sP,sR) delta(sQ,sS), antisymmetrised
+    phys = np.transpose(g_mo, (0, 2, 1, 3))[np.ix_(sp, sp, sp, sp)]
+    phys = phys * (spin[:, None, None, None] == spin[None, None, :, None]) \
+        * (spin[None, :, None, None] == spin[None, None, None, :])
+    A = phys - phys.transpose(0, 1, 3, 2)
+    occ = np.sort(np.where(sp < o)[0])
+    vir = np.sort(np.where(sp >= o)[0])
+    so, sv = sp[occ], sp[vir] - o
+    ao, av = spin[occ], spin[vir]
+    t1 = T1[np.ix_(so, sv)] * (ao[:, None] == av[None, :])
+    t2 = (T2[np.ix_(so, so, sv, sv)] * (ao[:, None, None, None] == av[None, None, :, None])
+          * (ao[None, :, None, None] == av[None, None, None, :])
+          - T2[np.ix_(so, so, sv, sv)].transpose(0, 1, 3, 2)
+          * (ao[:, None, None, None] == av[None, None, None, :]) * (ao[None, :, None, None] == av[None, None, :, None]))
+    e_o, e_v = eps[so], eps[sv + o]
+    D = (e_o[:, None, None, None, None, None] + e_o[None, :, None, None, None, None]
+         + e_o[None, None, :, None, None, None] - e_v[None, None, None, :, None, None]
+         - e_v[None, None, None, None, :, None] - e_v[None, None, None, None, None, :])
+    vovv = A[np.ix_(vir, occ, vir, vir)]      # <ei||bc>
+    ovoo = A[np.ix_(occ, vir, occ, occ)]      # <ma||jk>
+    oovv = A[np.ix_(occ, occ, vir, vir)]      # <jk||bc>
+
+    def P(x):
+        y = x - x.transpose(1, 0, 2, 3, 4, 5) - x.transpose(2, 1, 0, 3, 4, 5)
+        return y - y.transpose(0, 1, 2, 4, 3, 5) - y.transpose(0, 1, 2, 5, 4, 3)
+
+    tc = P(_es("jkae,eibc->ijkabc", t2, vovv) - _es("imbc,majk->ijkabc", t2, ovoo)) / D
+    td = P(_es("ia,jkbc->ijkabc", t1, oovv)) / D
+    return float(np.sum(tc * D * (tc + td)) / 36.0)
