@@ -315,6 +315,37 @@ import ..CoupledCluster: _cc
 do_rccsd(refWfn; return_T=false, kwargs...) = _cc(1, refWfn; return_T=return_T, ccsd_eqs=:mRCCSD, kwargs...)
 end
 
+module PerturbativeTriples
+import ...JuESHip: context, ensure_ao!, check, libjues, precision_code
+export compute_pT, rccsd_pT
+"""PerturbativeTriples.jl:35-138 on the device: same keyword arguments (T1, T2, Vvvvo, Vvooo,
+Vvovo, fo, fv); `precision=:fp32_mixed` forms W in FP32 MFMA."""
+function compute_pT(; T1::Array{Float64,2}, T2::Array{Float64,4}, Vvvvo::Array{Float64,4},
+                    Vvooo::Array{Float64,4}, Vvovo::Array{Float64,4}, fo::Array{Float64,1},
+                    fv::Array{Float64,1}, precision=:fp64, doprint=false)
+    ctx = context()
+    o, v = size(T1)
+    Et = Ref{Float64}(0.0)
+    GC.@preserve T1 T2 Vvvvo Vvooo Vvovo fo fv check(ctx, ccall((:jues_compute_pT, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Cint, Cint, Cint, Ref{Float64}),
+        ctx.handle, precision_code(precision), T1, T2, Vvvvo, Vvooo, Vvovo, fo, fv, o, v, doprint, Et))
+    return Et[]
+end
+"(T) after CCSD on canonical orbitals (AutoRCCSD.jl:294-300), integral blocks built on the device."
+function rccsd_pT(refWfn, T1::Array{Float64,2}, T2::Array{Float64,4}; precision=:fp64, doprint=false)
+    ctx = context()
+    n = ensure_ao!(ctx, refWfn.ao_eri)
+    Co, Cv, eps = refWfn.Cao, refWfn.Cav, refWfn.epsa
+    Et = Ref{Float64}(0.0)
+    GC.@preserve Co Cv eps T1 T2 check(ctx, ccall((:jues_rccsd_pT, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Cint, Ref{Float64}),
+        ctx.handle, precision_code(precision), Co, refWfn.nalpha, Cv, refWfn.nvira, n, eps, T1, T2, doprint, Et))
+    return Et[]
+end
+end
+
 module DFRCCD
 import ..CoupledCluster: _cc, FactoredERI, DFIntegrals
 """DF-RCCD.jl:11-53: MP2 guess, exactly `maxit` Jacobi sweeps (no DIIS, no convergence
