@@ -8,7 +8,7 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 def header_functions():
     txt = (ROOT / "include" / "jues.h").read_text()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"\b(jues_[a-z0-9_]+)\s*\(", txt)
+    names = re.findall(r"\b(jues_[A-Za-z0-9_]+)\s*\(", txt)
     return sorted(set(n for n in names if n not in ("jues_iter_cb",)))
 
 
