@@ -189,6 +189,23 @@ jues_status jues_rccsd(jues_ctx* ctx, int prec, const jues_cc_opts* opts,
                        const double* eps, double* E, double* T1_inout, double* T2_inout,
                        jues_iter_cb cb, void* user);
 
+/* ---- MollerPlesset.do_ump2 (MollerPlesset/UMP2.jl:12-79) ------------------------
+ * Unrestricted MP2 on a UHF reference: alpha-alpha and beta-beta pairs i > j, a > b
+ * of <ij||ab>^2 / D, and alpha-beta pairs of (ia|JB)^2 / D.  Cao (n x na), Cav
+ * (n x va), Cbo (n x nb), Cbv (n x vb); epsa (na + va) and epsb (nb + vb) hold the
+ * occupied energies first.  nb may be 0 (then Cbo may be NULL). */
+jues_status jues_ump2(jues_ctx* ctx, int prec, const double* Cao, int na, const double* Cav, int va,
+                      const double* Cbo, int nb, const double* Cbv, int vb, int n, const double* epsa,
+                      const double* epsb, double* E);
+
+/* ---- IntegralTransformation.get_fock (Backend/IntegralTransformation.jl:119-141) --
+ * f[p,q] = sum C[mu,p] C[nu,q] (h[mu,nu] + 2 (mu nu|lam sig) D[lam,sig] - (mu lam|nu sig) D[lam,sig]),
+ * D = Co Co^T, for the m columns of C (n x m) and the o columns of Co (n x o), from
+ * the context's AO integrals (dense or factors) and the core Hamiltonian hao (n x n);
+ * f is m x m.  The reference's spin keyword selects (C, Co) = (Ca, Cao) or (Cb, Cbo). */
+jues_status jues_get_fock(jues_ctx* ctx, const double* hao, const double* C, int m, const double* Co, int o, int n,
+                          double* f);
+
 /* ---- PerturbativeTriples.compute_pT (CoupledCluster/PerturbativeTriples.jl:35-138) ----
  * Et = (T) correction from the CCSD amplitudes T1 (o x v), T2 (o x o x v x v) and the
  * integral arrays AutoRCCSD.jl:295-297 passes: Vvvvo (v x v x v x o), Vvooo (v x o x o x o),

@@ -193,6 +193,11 @@ void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* 
 // RMP2 energy from K[i,a,j,b] = (ia|jb) (device, o x v x o x v) and eps (device FP64, n)
 template <typename T>
 double mp2_energy_dev(jues_ctx* ctx, const T* K, const double* eps, int o, int v);
+// One UMP2 spin block from K[i,a,J,B] = (ia|JB) (oA vA x oB vB) and the two spins' orbital
+// energies (occupied first); same = alpha-alpha or beta-beta (K square, eA == eB).
+template <typename T>
+double ump2_block_dev(jues_ctx* ctx, const T* K, const double* eA, const double* eB, int oA, int vA, int oB, int vB,
+                      bool same);
 
 // ---- host <-> device helpers ---------------------------------------------------
 void h2d(jues_ctx* ctx, void* dst, const void* src, size_t bytes);

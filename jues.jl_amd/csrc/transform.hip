@@ -156,7 +156,55 @@ __global__ __launch_bounds__(256) void sum_fixed_kernel(const double* __restrict
   if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// UMP2 blocks (UMP2.jl:30-77).  One block per (J,B) column of K[i,a,J,B] = (ia|JB).
+// same != 0 (alpha-alpha or beta-beta, K square): accumulates (ia|jb)((ia|jb) - (ib|ja)) / D,
+// half of which is the sum over i > j, a > b of <ij||ab>^2 / D; otherwise (ia|JB)^2 / D.
+template <typename T>
+__global__ __launch_bounds__(256) void ump2_energy_kernel(const T* __restrict__ K, const double* __restrict__ eA,
+                                                          const double* __restrict__ eB, int oA, int vA, int oB,
+                                                          int same, double* __restrict__ part) {
+  __shared__ double sh[4];
+  const int ovA = oA * vA;
+  const int jb = blockIdx.x;
+  const int j = jb % oB, b = jb / oB;
+  const double ejb = eB[j] - eB[oB + b];
+  const long col = (long)jb * ovA;
+  double s = 0.0;
+  for (int ia = threadIdx.x; ia < ovA; ia += 256) {
+    const int i = ia % oA, a = ia / oA;
+    const double k = (double)K[col + ia];
+    const double d = eA[i] - eA[oA + a] + ejb;
+    if (same) {
+      const double kx = (double)K[(long)(i + oA * b) + (long)ovA * (j + oA * a)];  // (ib|ja)
+      s += k * (k - kx) / d;
+    } else {
+      s += k * k / d;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[jb] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
 }  // namespace
+
+template <typename T>
+double ump2_block_dev(jues_ctx* ctx, const T* K, const double* eA, const double* eB, int oA, int vA, int oB, int vB,
+                      bool same) {
+  const int cols = oB * vB;
+  double* part = static_cast<double*>(scratch(ctx, 14, sizeof(double) * (size_t)(cols + 1)));
+  {
+    KTimer t(ctx, "ump2_energy", 4.0 * oA * vA * (double)cols, (double)oA * vA * cols * sizeof(T) * (same ? 2 : 1));
+    hipLaunchKernelGGL((ump2_energy_kernel<T>), dim3(cols), dim3(256), 0, ctx->stream, K, eA, eB, oA, vA, oB,
+                       same ? 1 : 0, part);
+    hipLaunchKernelGGL(sum_fixed_kernel, dim3(1), dim3(256), 0, ctx->stream, part, cols, part + cols);
+    JUES_HIP(hipGetLastError());
+  }
+  double E = 0.0;
+  d2h(ctx, &E, part + cols, sizeof(double));
+  return same ? 0.5 * E : E;
+}
 
 template <typename T>
 double mp2_energy_dev(jues_ctx* ctx, const T* K, const double* eps, int o, int v) {
@@ -181,5 +229,9 @@ template void tei_transform_dev<float>(jues_ctx*, const float*, int, const float
                                        const float*, int, float*);
 template double mp2_energy_dev<double>(jues_ctx*, const double*, const double*, int, int);
 template double mp2_energy_dev<float>(jues_ctx*, const float*, const double*, int, int);
+template double ump2_block_dev<double>(jues_ctx*, const double*, const double*, const double*, int, int, int, int,
+                                       bool);
+template double ump2_block_dev<float>(jues_ctx*, const float*, const double*, const double*, int, int, int, int,
+                                      bool);
 
 }  // namespace jues

@@ -146,9 +146,9 @@ void pt_pack_k(jues_ctx* ctx, PTOperands<T>& S, const KView& K, const double* t1
     for (int e = 0; e < 4; ++e)
       if (ax[e].role == want[d]) perm[d] = e;
   S.Kc.alloc(sizeof(double) * (size_t)o * o * v * v);
-  permute<double, double>(ctx, K.p, dims, perm, S.Kc.as<double>(), 1.0, 0.0, "pt_prep");
+  permute<double, double>(ctx, K.p, dims, perm, S.Kc.template as<double>(), 1.0, 0.0, "pt_prep");
   S.t1t.alloc(sizeof(double) * (size_t)o * v);
-  permute<double, double>(ctx, t1, {o, v}, {1, 0}, S.t1t.as<double>(), 1.0, 0.0, "pt_prep");
+  permute<double, double>(ctx, t1, {o, v}, {1, 0}, S.t1t.template as<double>(), 1.0, 0.0, "pt_prep");
 }
 
 // dst rows of `width` bytes every `dpitch` bytes <- src rows every `spitch` bytes

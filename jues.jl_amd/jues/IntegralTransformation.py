@@ -1,4 +1,4 @@
-"""``get_eri`` (Backend/IntegralTransformation.jl:38-101) on the GPU."""
+"""``get_eri`` (Backend/IntegralTransformation.jl:38-101) and ``get_fock`` (:119-141) on the GPU."""
 from __future__ import annotations
 
 import numpy as np
@@ -31,4 +31,25 @@ def get_eri(wfn: Wfn, eri_string: str, notation: str = "phys", fcn: int = 0, *, 
     return out
 
 
-__all__ = ["get_eri"]
+def get_fock(wfn: Wfn, spin: str = "alpha", *, device: int = 0) -> np.ndarray:
+    """MO Fock matrix f = C^T (h + 2J - K) C of the spin's occupied density
+    (IntegralTransformation.jl:119-141); needs ``wfn.hao``."""
+    sp = spin.lower()
+    if sp in ("alpha", "up", "a"):
+        C, Co = wfn.Ca, wfn.Cao
+    elif sp in ("beta", "down", "b"):
+        C, Co = wfn.Cb, wfn.Cbo
+    else:
+        raise ValueError(f"Invalid Spin option given to JuES.IntegralTransformation.get_fock: {spin}")
+    if wfn.hao is None:
+        raise ValueError("get_fock needs the core Hamiltonian wfn.hao")
+    ctx = _lib.context(device)
+    n = ensure_ao(ctx, wfn.ao_eri)
+    C, Co, h = _lib.f64(C), _lib.f64(Co), _lib.f64(wfn.hao)
+    m = C.shape[1]
+    f = np.zeros((m, m), order="F")
+    ctx.check(ctx.lib.jues_get_fock(ctx.h, _lib.ptr(h), _lib.ptr(C), m, _lib.ptr(Co), Co.shape[1], n, _lib.ptr(f)))
+    return f
+
+
+__all__ = ["get_eri", "get_fock"]

@@ -1,5 +1,6 @@
 """``do_rmp2`` (MollerPlesset/RMP2.jl:11-45), ``do_df_rmp2``
-(MollerPlesset/DF-RMP2.jl:1-46) and the MP2 adjoint on the GPU."""
+(MollerPlesset/DF-RMP2.jl:1-46), ``do_ump2`` (MollerPlesset/UMP2.jl:12-79) and the
+MP2 adjoint on the GPU."""
 from __future__ import annotations
 
 import numpy as np
@@ -35,6 +36,21 @@ def do_df_rmp2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0) -> float:
     return do_rmp2(refWfn, prec=prec, device=device)
 
 
+def do_ump2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0) -> float:
+    """Unrestricted MP2 (UMP2.jl:12-79): alpha-alpha and beta-beta <ij||ab>^2/D over
+    i > j, a > b, plus alpha-beta (ia|JB)^2/D.  On a restricted Wfn it equals do_rmp2."""
+    ctx = _lib.context(device)
+    n = ensure_ao(ctx, refWfn.ao_eri)
+    Cao, Cav = _lib.f64(refWfn.Cao), _lib.f64(refWfn.Cav)
+    Cbo, Cbv = _lib.f64(refWfn.Cbo), _lib.f64(refWfn.Cbv)
+    ea, eb = _lib.f64(refWfn.epsa), _lib.f64(refWfn.epsb)
+    E = np.zeros(1)
+    ctx.check(ctx.lib.jues_ump2(ctx.h, prec_code(prec), _lib.ptr(Cao), Cao.shape[1], _lib.ptr(Cav), Cav.shape[1],
+                                _lib.ptr(Cbo), Cbo.shape[1], _lib.ptr(Cbv), Cbv.shape[1], n, _lib.ptr(ea),
+                                _lib.ptr(eb), _lib.ptr(E)))
+    return float(E[0])
+
+
 def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: int = 0):
     """(E, dE/dC (n x n), dE/dg (n^4 or None)) with every C and g entry independent.
 
@@ -54,4 +70,4 @@ def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: 
     return float(E[0]), dC, dg
 
 
-__all__ = ["do_rmp2", "do_df_rmp2", "rmp2_grad"]
+__all__ = ["do_rmp2", "do_df_rmp2", "do_ump2", "rmp2_grad"]

@@ -75,6 +75,18 @@ class Wfn:
                    eps, eps, ao_eri)
 
     @classmethod
+    def from_uhf(cls, Ca: np.ndarray, Cb: np.ndarray, epsa: np.ndarray, epsb: np.ndarray, nalpha: int, nbeta: int,
+                 ao_eri, energy: float = 0.0, vnuc: float = 0.0, hao=None) -> "Wfn":
+        """Unrestricted Wfn (Wavefunction.jl:94-141 with unrestricted = true)."""
+        Ca = np.asfortranarray(Ca, dtype=np.float64)
+        Cb = np.asfortranarray(Cb, dtype=np.float64)
+        n = Ca.shape[0]
+        return cls(energy, vnuc, nalpha, nbeta, n - nalpha, n - nbeta, n, True, None, None, Ca, Cb,
+                   np.asfortranarray(Ca[:, :nalpha]), np.asfortranarray(Ca[:, nalpha:]),
+                   np.asfortranarray(Cb[:, :nbeta]), np.asfortranarray(Cb[:, nbeta:]), hao,
+                   np.asarray(epsa, dtype=np.float64), np.asarray(epsb, dtype=np.float64), ao_eri)
+
+    @classmethod
     def from_synthetic(cls, system, dense: bool = True) -> "Wfn":
         eri = system.ao_eri() if dense else FactoredERI(system.B)
         return cls.from_orbitals(system.C, system.eps, system.o, eri)

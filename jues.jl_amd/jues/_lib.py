@@ -29,7 +29,7 @@ EXPORTS = (
     "jues_tei_transform", "jues_get_eri",
     "jues_rmp2", "jues_rmp2_grad", "jues_rccd", "jues_rccsd", "jues_cc_setup", "jues_cc_iterate",
     "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test",
-    "jues_compute_pT", "jues_rccsd_pT",
+    "jues_compute_pT", "jues_rccsd_pT", "jues_ump2", "jues_get_fock",
 )
 
 
@@ -95,6 +95,8 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "jues_get_ao_factors": (i, [_ctxp, _dp, i, i]),
             "jues_compute_pT": (i, [_ctxp, i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, i, i, i, _dp]),
             "jues_rccsd_pT": (i, [_ctxp, i, _dp, i, _dp, i, i, _dp, _dp, _dp, i, _dp]),
+            "jues_ump2": (i, [_ctxp, i, _dp, i, _dp, i, _dp, i, _dp, i, i, _dp, _dp, _dp]),
+            "jues_get_fock": (i, [_ctxp, _dp, _dp, i, _dp, i, i, _dp]),
             "jues_tei_transform": (i, [_ctxp, i, _dp, i, _dp, i, _dp, i, _dp, i, i, _dp]),
             "jues_get_eri": (i, [_ctxp, i, C.c_char_p, C.c_char_p, _dp, i, _dp, i, i, i, _dp]),
             "jues_rmp2": (i, [_ctxp, i, _dp, i, _dp, i, i, _dp, _dp]),

@@ -863,3 +863,57 @@ def pT_spin_orbital(T1, T2, g_mo, eps, o):
     tc = P(_es("jkae,eibc->ijkabc", t2, vovv) - _es("imbc,majk->ijkabc", t2, ovoo)) / D
     td = P(_es("ia,jkbc->ijkabc", t1, oovv)) / D
     return float(np.sum(tc * D * (tc + td)) / 36.0)
+
+
+# ---------------------------------------------------------------------------
+# MollerPlesset/UMP2.jl:12-79 and IntegralTransformation.get_fock (:119-141)
+# ---------------------------------------------------------------------------
+
+def ump2(gao, Cao, Cav, Cbo, Cbv, epsa, epsb):
+    """UMP2.jl:12-79: alpha-alpha over i > j, a > b of <ij||ab>^2 / D (:30-44), alpha-beta over
+    all (i,J,a,B) of (ia|JB)^2 / D (:46-59), beta-beta as alpha-alpha (:61-77)."""
+    na, nb = Cao.shape[1], Cbo.shape[1]
+    E = 0.0
+    for Co, Cv, eps, no in ((Cao, Cav, epsa, na), (Cbo, Cbv, epsb, nb)):
+        K = tei_transform(gao, Co, Cv, Co, Cv)                      # (ia|jb)
+        A = K - K.transpose(0, 3, 2, 1)                              # <ij||ab> at [i,a,j,b]
+        eo, ev = eps[:no], eps[no:]
+        D = eo[:, None, None, None] + eo[None, None, :, None] - ev[None, :, None, None] - ev[None, None, None, :]
+        i, a, j, b = np.meshgrid(*(np.arange(s) for s in K.shape), indexing="ij")
+        E += float(np.sum(np.where((i > j) & (a > b), A * A / D, 0.0)))
+    K = tei_transform(gao, Cao, Cav, Cbo, Cbv)                       # (ia|JB)
+    eoa, eva, eob, evb = epsa[:na], epsa[na:], epsb[:nb], epsb[nb:]
+    D = eoa[:, None, None, None] + eob[None, None, :, None] - eva[None, :, None, None] - evb[None, None, None, :]
+    return E + float(np.sum(K * K / D))
+
+
+def ump2_spin_orbital(gao, Ca, Cb, epsa, epsb, na, nb):
+    """Independent check: 1/4 sum_{IJAB} |<IJ||AB>|^2 / D over spin orbitals built from
+    the UHF alpha and beta orbitals."""
+    n = Ca.shape[0]
+    C = np.zeros((2 * n, 2 * n))
+    C[:n, :n], C[n:, n:] = Ca, Cb                                     # AO (alpha, beta) x MO (alpha, beta)
+    spin_ao = np.r_[np.zeros(n), np.ones(n)]
+    g2 = np.zeros((2 * n,) * 4)
+    for s1 in (0, 1):
+        for s2 in (0, 1):
+            g2[s1 * n:(s1 + 1) * n, s1 * n:(s1 + 1) * n, s2 * n:(s2 + 1) * n, s2 * n:(s2 + 1) * n] = gao
+    gm = tei_transform(g2, C, C, C, C)                                # (pq|rs) over spin orbitals
+    phys = gm.transpose(0, 2, 1, 3)
+    A = phys - phys.transpose(0, 1, 3, 2)
+    eps = np.r_[epsa, epsb]
+    occ = np.r_[np.arange(na), n + np.arange(nb)]
+    vir = np.r_[np.arange(na, n), n + np.arange(nb, n)]
+    Ao = A[np.ix_(occ, occ, vir, vir)]
+    eo, ev = eps[occ], eps[vir]
+    D = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
+    del spin_ao
+    return float(np.sum(Ao * Ao / D) / 4.0)
+
+
+def get_fock(gao, hao, C, Co):
+    """IntegralTransformation.jl:119-141: f = C^T h C + 2 C^T J[Co Co^T] C - C^T K[Co Co^T] C."""
+    f = C.T @ hao @ C
+    f += 2 * _es("mp,nq,lk,sk,mnls->pq", C, C, Co, Co, gao)
+    f -= _es("mp,nq,lk,sk,mlns->pq", C, C, Co, Co, gao)
+    return f

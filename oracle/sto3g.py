@@ -330,3 +330,60 @@ def rhf_jues(S, H, g, nocc, maxit=50, Etol=1e-7, Dtol=1e-7):
         if dE < Etol and Drms < Dtol:
             break
     return E, C, e
+
+
+def uhf(S, H, g, na, nb, conv=1e-13, maxit=300):
+    """Pople-Nesbet UHF with DIIS on the stacked alpha/beta commutators, core guess;
+    returns (E_elec, Ca, Cb, epsa, epsb)."""
+    X = np.linalg.inv(np.linalg.cholesky(S)).T
+    Fa, Fb = H.copy(), H.copy()
+    Da_old = np.zeros_like(S)
+    hist = []
+    E_old = 0.0
+
+    def diag(F):
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        return e, X @ Cp
+
+    for it in range(maxit):
+        _, Ca = diag(Fa)
+        _, Cb = diag(Fb)
+        Da = Ca[:, :na] @ Ca[:, :na].T
+        Db = Cb[:, :nb] @ Cb[:, :nb].T
+        J = np.einsum("mnls,ls->mn", g, Da + Db)
+        Fa = H + J - np.einsum("mlns,ls->mn", g, Da)
+        Fb = H + J - np.einsum("mlns,ls->mn", g, Db)
+        E = 0.5 * np.sum((Da + Db) * H + Da * Fa + Db * Fb)
+        err = np.concatenate([(Fa @ Da @ S - S @ Da @ Fa).ravel(), (Fb @ Db @ S - S @ Db @ Fb).ravel()])
+        hist.append((Fa, Fb, err))
+        if len(hist) > 10:
+            hist.pop(0)
+        m = len(hist)
+        B = -np.ones((m + 1, m + 1))
+        B[m, m] = 0
+        for a in range(m):
+            for b in range(m):
+                B[a, b] = hist[a][2] @ hist[b][2]
+        rhs = np.zeros(m + 1)
+        rhs[m] = -1
+        c = np.linalg.solve(B, rhs)
+        Fa = sum(c[k] * hist[k][0] for k in range(m))
+        Fb = sum(c[k] * hist[k][1] for k in range(m))
+        dD = np.max(np.abs(Da - Da_old))
+        Da_old = Da
+        if it > 2 and dD < conv and abs(E - E_old) < 1e-14:
+            break
+        E_old = E
+    J = np.einsum("mnls,ls->mn", g, Da + Db)
+    ea, Ca = diag(H + J - np.einsum("mlns,ls->mn", g, Da))
+    eb, Cb = diag(H + J - np.einsum("mlns,ls->mn", g, Db))
+    return E, Ca, Cb, ea, eb
+
+
+def system_uhf(atoms, na, nb):
+    """AO integrals and UHF orbitals (psi4 "reference uhf", PK integrals)."""
+    bf = basis(atoms)
+    S, T, V = one_electron(bf, atoms)
+    g = eri(bf)
+    E, Ca, Cb, ea, eb = uhf(S, T + V, g, na, nb)
+    return dict(S=S, H=T + V, g=g, Ca=Ca, Cb=Cb, epsa=ea, epsb=eb, na=na, nb=nb, E_elec=E)
