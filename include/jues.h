@@ -189,6 +189,18 @@ jues_status jues_rccsd(jues_ctx* ctx, int prec, const jues_cc_opts* opts,
                        const double* eps, double* E, double* T1_inout, double* T2_inout,
                        jues_iter_cb cb, void* user);
 
+/* ---- AutoRCCSD.do_rccsd (CoupledCluster/AutoRCCSD.jl:193-300) -----------------
+ * Closed-shell CCSD driven by the full MO Fock matrix `fock` ((o + v) x (o + v) over
+ * [Cocc | Cvir], e.g. from jues_get_fock): the resolvents take its diagonal and the
+ * off-diagonal blocks f_oo, f_vv, f_ov enter the residuals (AutoRCCSD.jl:76-132) and
+ * the energy 2 f_ia t_ia (:39-51), so non-canonical and non-Hartree-Fock references
+ * are handled.  The equations are those of the RCCSD.jl set (opts->ccsd_eqs), the
+ * MP2 guess adds T1 = f_ia / d_ia (:249), and options, precision and outputs are as
+ * for jues_rccsd (coupling must be 1, continuation 1). */
+jues_status jues_rccsd_fock(jues_ctx* ctx, int prec, const jues_cc_opts* opts, const double* Cocc, int o,
+                            const double* Cvir, int v, int n, const double* fock, double* E, double* T1_inout,
+                            double* T2_inout, jues_iter_cb cb, void* user);
+
 /* ---- MollerPlesset.do_ump2 (MollerPlesset/UMP2.jl:12-79) ------------------------
  * Unrestricted MP2 on a UHF reference: alpha-alpha and beta-beta pairs i > j, a > b
  * of <ij||ab>^2 / D, and alpha-beta pairs of (ia|JB)^2 / D.  Cao (n x na), Cav

@@ -21,6 +21,9 @@ struct CCInts {
   DevBuf ovvv_c;  // (ia|bc) at [i,a,b,c]      (RCCSD)
   DevBuf Lvv;     // MO factors L[ae,P] (v^2 x naux) when the ladder is recomputed
   DevBuf Lvvt;    // the same with a and e exchanged: L[ea,P]
+  // off-diagonal Fock blocks of a non-canonical reference (AutoRCCSD.jl:215-232):
+  // f_mi at [m,i], f_ae at [a,e] (diagonals zero), f_ia at [i,a]
+  DevBuf fOO, fVV, fOV;
 };
 
 template <typename T>
@@ -76,6 +79,8 @@ struct CCState {
   double lambda = 1.0;  // coupling of the equations being iterated
   double E_prev = 0;
   bool have_prev = false;
+  bool fock_on = false;  // Fock-driven CCSD: the fOO/fVV/fOV terms and 2 f_ia t_ia in the energy
+  DevBuf fov2;           // FP64 2 f_ia (the T1 energy weight)
   int iter = 0;
   ~CCState() {
     for (auto& e : ev)

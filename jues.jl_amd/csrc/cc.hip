@@ -200,6 +200,9 @@ void cc_make_ints32(jues_ctx* ctx, CCState& S) {
   convert_buf<float>(ctx, a.ovvv_c, b.ovvv_c);
   convert_buf<float>(ctx, a.Lvv, b.Lvv);
   convert_buf<float>(ctx, a.Lvvt, b.Lvvt);
+  convert_buf<float>(ctx, a.fOO, b.fOO);
+  convert_buf<float>(ctx, a.fVV, b.fVV);
+  convert_buf<float>(ctx, a.fOV, b.fOV);
   sync(ctx);
 }
 
@@ -550,7 +553,7 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
     B.v = v;
     B.eps = S.eps.as<double>();
     B.R1 = S.R1.as<double>();
-    B.Lener1 = nullptr;
+    B.Lener1 = S.fock_on ? S.fov2.as<double>() : nullptr;
     B.lambda = S.lambda;
     B.t_old = t + N2;
     B.t_new = tn + N2;
@@ -599,8 +602,21 @@ template void cc_iteration<float>(jues_ctx*, CCState&, jues_iter_info*);
 // ---------------------------------------------------------------------------
 // setup / drivers
 // ---------------------------------------------------------------------------
+namespace {
+// t1[i,a] = f_ia / (eps_i - eps_a): the T1 guess of a Fock-driven CCSD (AutoRCCSD.jl:249)
+__global__ __launch_bounds__(256) void t1_fock_guess_kernel(const double* __restrict__ fov,
+                                                            const double* __restrict__ eps, int o, int v,
+                                                            double* __restrict__ t1) {
+  const long x = blockIdx.x * 256L + threadIdx.x;
+  if (x >= (long)o * v) return;
+  const int i = (int)(x % o), a = (int)(x / o);
+  t1[x] = fov[x] / (eps[i] - eps[o + a]);
+}
+}  // namespace
+
 void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, const double* Cocc, int o,
-              const double* Cvir, int v, int n, const double* eps, const double* T1_init, const double* T2_init) {
+              const double* Cvir, int v, int n, const double* eps, const double* T1_init, const double* T2_init,
+              const double* fock) {
   JUES_REQUIRE(method == 0 || method == 1, "cc_setup: method must be 0 (RCCD) or 1 (RCCSD)");
   JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED || prec == JUES_FP16X3_MIXED, "cc_setup: bad precision");
   JUES_REQUIRE(Cocc && Cvir && eps && o > 0 && v > 0, "cc_setup: bad arguments");
@@ -613,6 +629,8 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   JUES_REQUIRE(opts.guess >= 0 && opts.guess <= 3, "cc_setup: bad guess");
   JUES_REQUIRE(opts.guess != JUES_GUESS_USER || T2_init, "cc_setup: guess=USER needs T2");
   JUES_REQUIRE(opts.continuation >= 1 && std::isfinite(opts.coupling), "cc_setup: bad continuation/coupling");
+  JUES_REQUIRE(!fock || (method == 1 && opts.coupling == 1.0 && opts.continuation == 1),
+               "cc_setup: a Fock matrix needs CCSD at coupling 1 without continuation");
   sync(ctx);
   ctx->cc.reset();
   ctx->cc.reset(new CCState());
@@ -639,6 +657,29 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   S.i64.reset(new CCInts<double>());
   cc_build_ints(ctx, S, co.p, cv.p);
   if (method == 1) cc_build_ccsd_ints(ctx, S);
+  if (fock) {
+    // fock is the (o + v) x (o + v) MO Fock matrix over [Cocc | Cvir]; frozen core dropped
+    const int m = o + v;
+    std::vector<double> foo((size_t)oo_ * oo_), fvv((size_t)v * v), fov((size_t)oo_ * v), fov2((size_t)oo_ * v);
+    for (int i = 0; i < oo_; ++i)
+      for (int k = 0; k < oo_; ++k) foo[k + (size_t)oo_ * i] = k == i ? 0.0 : fock[(fc + k) + (size_t)m * (fc + i)];
+    for (int a = 0; a < v; ++a)
+      for (int e2 = 0; e2 < v; ++e2) fvv[a + (size_t)v * e2] = a == e2 ? 0.0 : fock[(o + a) + (size_t)m * (o + e2)];
+    for (int a = 0; a < v; ++a)
+      for (int i = 0; i < oo_; ++i) {
+        fov[i + (size_t)oo_ * a] = fock[(fc + i) + (size_t)m * (o + a)];
+        fov2[i + (size_t)oo_ * a] = 2.0 * fov[i + (size_t)oo_ * a];
+      }
+    S.i64->fOO.alloc(foo.size() * sizeof(double));
+    S.i64->fVV.alloc(fvv.size() * sizeof(double));
+    S.i64->fOV.alloc(fov.size() * sizeof(double));
+    S.fov2.alloc(fov2.size() * sizeof(double));
+    h2d(ctx, S.i64->fOO.p, foo.data(), foo.size() * sizeof(double));
+    h2d(ctx, S.i64->fVV.p, fvv.data(), fvv.size() * sizeof(double));
+    h2d(ctx, S.i64->fOV.p, fov.data(), fov.size() * sizeof(double));
+    h2d(ctx, S.fov2.p, fov2.data(), fov2.size() * sizeof(double));
+    S.fock_on = true;
+  }
   // amplitudes
   S.t.alloc(S.Nd * sizeof(double));
   double* t = S.t.as<double>();
@@ -649,6 +690,11 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
     case JUES_GUESS_MP2:
       JUES_HIP(hipMemcpyAsync(t, S.i64->Kpp.p, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
       divide_d2(ctx, t, S.eps.as<double>(), S.o, S.v, S.lambda);
+      if (S.fock_on) {
+        hipLaunchKernelGGL(t1_fock_guess_kernel, dim3((unsigned)((N1 + 255) / 256)), dim3(256), 0, ctx->stream,
+                           S.i64->fOV.as<double>(), S.eps.as<double>(), S.o, S.v, t + N2);
+        JUES_HIP(hipGetLastError());
+      }
       break;
     case JUES_GUESS_OVOV:
       JUES_HIP(hipMemcpyAsync(t, S.guess_buf.p, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -676,6 +722,7 @@ double cc_energy(jues_ctx* ctx, CCState& S) {
   const long N2 = (long)S.o * S.o * S.v * S.v;
   double E = dot<double, double>(ctx, S.Lener.as<double>(), S.t.as<double>(), N2);
   if (S.method == 1) E += cc_energy_t1(ctx, S);
+  if (S.fock_on) E += dot<double, double>(ctx, S.fov2.as<double>(), S.t.as<double>() + N2, (long)S.o * S.v);
   return S.lambda * E;
 }
 
@@ -736,10 +783,10 @@ namespace {
 
 jues_status cc_solve(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts, const double* Cocc, int o,
                      const double* Cvir, int v, int n, const double* eps, double* E, double* T1_inout,
-                     double* T2_inout, jues_iter_cb cb, void* user) {
+                     double* T2_inout, jues_iter_cb cb, void* user, const double* fock = nullptr) {
   JUES_API_BEGIN(ctx)
   JUES_REQUIRE(E, "null energy pointer");
-  cc_setup(ctx, method, prec, opts, Cocc, o, Cvir, v, n, eps, T1_inout, T2_inout);
+  cc_setup(ctx, method, prec, opts, Cocc, o, Cvir, v, n, eps, T1_inout, T2_inout, fock);
   CCState& S = *ctx->cc;
   jues_iter_info last{};
   bool conv = false;
@@ -793,11 +840,26 @@ jues_status jues_rccsd(jues_ctx* ctx, int prec, const jues_cc_opts* opts, const 
   return cc_solve(ctx, 1, prec, opts, Cocc, o, Cvir, v, n, eps, E, T1_inout, T2_inout, cb, user);
 }
 
+jues_status jues_rccsd_fock(jues_ctx* ctx, int prec, const jues_cc_opts* opts, const double* Cocc, int o,
+                            const double* Cvir, int v, int n, const double* fock, double* E, double* T1_inout,
+                            double* T2_inout, jues_iter_cb cb, void* user) {
+  if (!ctx) return JUES_BAD_ARG;
+  if (!fock || o <= 0 || v <= 0) {
+    ctx->err = "jues_rccsd_fock: bad arguments";
+    return JUES_BAD_ARG;
+  }
+  // the resolvents take the Fock diagonal (AutoRCCSD.jl:220-222,243-246)
+  const int m = o + v;
+  std::vector<double> diag((size_t)m);
+  for (int p = 0; p < m; ++p) diag[p] = fock[p + (size_t)m * p];
+  return cc_solve(ctx, 1, prec, opts, Cocc, o, Cvir, v, n, diag.data(), E, T1_inout, T2_inout, cb, user, fock);
+}
+
 jues_status jues_cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts, const double* Cocc, int o,
                           const double* Cvir, int v, int n, const double* eps, const double* T1_init,
                           const double* T2_init) {
   JUES_API_BEGIN(ctx)
-  cc_setup(ctx, method, prec, opts, Cocc, o, Cvir, v, n, eps, T1_init, T2_init);
+  cc_setup(ctx, method, prec, opts, Cocc, o, Cvir, v, n, eps, T1_init, T2_init, nullptr);
   JUES_API_END(ctx)
 }
 

@@ -136,6 +136,19 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   contract<T>(ctx, 2.0, TN(t1, "nf"), TN(Kov, "menf"), 0.0, TN(Fme, "me"), "cc_t1");
   contract<T>(ctx, -1.0, TN(t1, "nf"), TN(Kov, "nemf"), 1.0, TN(Fme, "me"), "cc_t1");
 
+  // ---- off-diagonal Fock terms of a non-canonical reference (AutoRCCSD.jl:76-81,127-132):
+  // Fae += f_ae - t1[ma] f_me / 2, Fmi += f_mi + t1[ie] f_me / 2, Fme += f_me
+  if (S.fock_on) {
+    const T* fOO = I.fOO.template as<T>();
+    const T* fVV = I.fVV.template as<T>();
+    const T* fOV = I.fOV.template as<T>();
+    axpby<T>(ctx, vv, 1.0, fVV, 1.0, Fae);
+    contract<T>(ctx, -0.5, TN(t1, "ma"), TN(fOV, "me"), 1.0, TN(Fae, "ae"), "cc_fock_t1");
+    axpby<T>(ctx, oo, 1.0, fOO, 1.0, Fmi);
+    contract<T>(ctx, 0.5, TN(t1, "ie"), TN(fOV, "me"), 1.0, TN(Fmi, "mi"), "cc_fock_t1");
+    axpby<T>(ctx, ov, 1.0, fOV, 1.0, Fme);
+  }
+
   // ---- T1 residual (mRCCSD.jl:202-212) ----------------------------------------------
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(Fae, "ae"), 0.0, TN(n1, "ia"), "cc_t1");
   contract<T>(ctx, -1.0, TN(t1, "ma"), TN(Fmi, "mi"), 1.0, TN(n1, "ia"), "cc_t1");
@@ -146,6 +159,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   contract<T>(ctx, 1.0, TN(Tov, "mane"), TN(ooov, "nime"), 1.0, TN(n1, "ia"), "cc_t1");
   contract<T>(ctx, 2.0, TN(Tov, "iemf"), TN(ovvv, "mfae"), 1.0, TN(n1, "ia"), "cc_t1");
   contract<T>(ctx, -1.0, TN(Tov, "iemf"), TN(ovvv, "meaf"), 1.0, TN(n1, "ia"), "cc_t1");
+  if (S.fock_on) axpby<T>(ctx, ov, 1.0, I.fOV.template as<T>(), 1.0, n1);  // f_ia
   convert<T, double>(ctx, n1, S.R1.as<double>(), ov);
 
   // ---- Fb = Fae - t1.Fme/2, Fj = Fmi + t1.Fme/2 ------------------------------------

@@ -45,7 +45,7 @@ def _opts(kw: dict, method_default_guess: str) -> _lib.CCOpts:
     return o
 
 
-def _run(method: int, refWfn: Wfn, kw: dict, default_guess: str):
+def _run(method: int, refWfn: Wfn, kw: dict, default_guess: str, fock=None):
     kw = dict(kw)
     prec = kw.pop("prec", "fp64")
     device = kw.pop("device", 0)
@@ -81,9 +81,13 @@ def _run(method: int, refWfn: Wfn, kw: dict, default_guess: str):
     if method == 0:
         st = ctx.lib.jues_rccd(ctx.h, prec_code(prec), C.byref(opts), _lib.ptr(Co), o, _lib.ptr(Cv), v, n,
                                _lib.ptr(eps), _lib.ptr(E), _lib.ptr(T2), cb, None)
-    else:
+    elif fock is None:
         st = ctx.lib.jues_rccsd(ctx.h, prec_code(prec), C.byref(opts), _lib.ptr(Co), o, _lib.ptr(Cv), v, n,
                                 _lib.ptr(eps), _lib.ptr(E), _lib.ptr(T1), _lib.ptr(T2), cb, None)
+    else:
+        f = _lib.f64(fock)
+        st = ctx.lib.jues_rccsd_fock(ctx.h, prec_code(prec), C.byref(opts), _lib.ptr(Co), o, _lib.ptr(Cv), v, n,
+                                     _lib.ptr(f), _lib.ptr(E), _lib.ptr(T1), _lib.ptr(T2), cb, None)
     ctx.check(st, allow)
     _run.last_history = history
     if return_T:
@@ -193,6 +197,38 @@ def _rccsd_pT(refWfn: Wfn, T1, T2, *, prec: str = "fp64", device: int = 0, dopri
 
 PerturbativeTriples = types.SimpleNamespace(compute_pT=_compute_pT, rccsd_pT=_rccsd_pT)
 
+
+def _auto_rccsd(wfn: Wfn, *, do_pT: bool = False, return_T: bool = False, diis: bool = False, **kwargs):
+    """AutoRCCSD.do_rccsd (AutoRCCSD.jl:193-300): CCSD driven by the MO Fock matrix of the
+    Wfn (``get_fock``, needs ``wfn.hao``), so non-canonical and non-Hartree-Fock orbitals
+    are handled: the resolvents take the Fock diagonal, the off-diagonal blocks enter the
+    equations and 2 f_ia t_ia the energy; MP2 guess with T1 = f_ia / d_ia.  Plain
+    updates by default, as the reference (``diis=True`` accelerates, same fixed point).
+    The stopping test is |dE| < cc_e_conv and ||dt||_2 < cc_max_rms.  Returns the
+    correlation energy, with the (T) correction as a second value when ``do_pT``
+    (fo, fv = the Fock diagonal, :294-300), and the amplitudes when ``return_T``."""
+    from .IntegralTransformation import get_fock
+    import dataclasses
+    device = kwargs.get("device", 0)
+    fock = get_fock(wfn, "alpha", device=device)
+    kw = dict(kwargs)
+    kw.setdefault("ccsd_eqs", "RCCSD")
+    kw.update(diis=diis, return_T=True)
+    E, T1, T2 = _run(1, wfn, kw, "mp2", fock=fock)
+    out = [E]
+    if do_pT:
+        fcn = int(kwargs.get("fcn", 0))
+        if fcn:
+            raise ValueError("do_pT with frozen core is not supported here")
+        w2 = dataclasses.replace(wfn, epsa=np.diag(fock).copy())
+        out.append(_rccsd_pT(w2, T1, T2, prec=kwargs.get("prec", "fp64"), device=device))
+    if return_T:
+        out += [T1, T2]
+    return out[0] if len(out) == 1 else tuple(out)
+
+
+AutoRCCSD = types.SimpleNamespace(do_rccsd=_auto_rccsd)
+
 RCCD = types.SimpleNamespace(do_rccd=_rccd)
 mRCCD = types.SimpleNamespace(do_rccd=_mrccd)
 DFRCCD = types.SimpleNamespace(do_df_rccd=_df_rccd)
@@ -242,5 +278,5 @@ class Solver:
         return (float(E[0]), T2) if self.method == 0 else (float(E[0]), T1, T2)
 
 
-__all__ = ["RCCD", "mRCCD", "RCCSD", "mRCCSD", "DFRCCD", "PerturbativeTriples", "Solver", "defaults",
-           "last_history"]
+__all__ = ["RCCD", "mRCCD", "RCCSD", "mRCCSD", "DFRCCD", "AutoRCCSD", "PerturbativeTriples", "Solver",
+           "defaults", "last_history"]

@@ -29,7 +29,7 @@ EXPORTS = (
     "jues_tei_transform", "jues_get_eri",
     "jues_rmp2", "jues_rmp2_grad", "jues_rccd", "jues_rccsd", "jues_cc_setup", "jues_cc_iterate",
     "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test",
-    "jues_compute_pT", "jues_rccsd_pT", "jues_ump2", "jues_get_fock",
+    "jues_compute_pT", "jues_rccsd_pT", "jues_ump2", "jues_get_fock", "jues_rccsd_fock",
 )
 
 
@@ -104,6 +104,8 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "jues_rccd": (i, [_ctxp, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp, ITER_CB, C.c_void_p]),
             "jues_rccsd": (i, [_ctxp, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp, _dp, ITER_CB,
                                C.c_void_p]),
+            "jues_rccsd_fock": (i, [_ctxp, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp, _dp, ITER_CB,
+                                    C.c_void_p]),
             "jues_cc_setup": (i, [_ctxp, i, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp]),
             "jues_cc_iterate": (i, [_ctxp, i, i, i, C.POINTER(IterInfo), C.POINTER(C.c_int)]),
             "jues_cc_reset_diis": (i, [_ctxp]),

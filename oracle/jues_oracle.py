@@ -917,3 +917,192 @@ def get_fock(gao, hao, C, Co):
     f += 2 * _es("mp,nq,lk,sk,mnls->pq", C, C, Co, Co, gao)
     f -= _es("mp,nq,lk,sk,mlns->pq", C, C, Co, Co, gao)
     return f
+
+
+# ---------------------------------------------------------------------------
+# CoupledCluster/AutoRCCSD.jl:39-300: CCSD with the full Fock matrix
+# ---------------------------------------------------------------------------
+
+def auto_rccsd_integrals(tei, Cao, Cav):
+    """AutoRCCSD.jl:239-240: phys OOOO, OOOV, OOVV, OVOV, OVVV, VVVV."""
+    ph = lambda a, b, c, d: np.transpose(tei(a, c, b, d), (0, 2, 1, 3))  # noqa: E731  <pq|rs>
+    o, v = Cao, Cav
+    return dict(oooo=ph(o, o, o, o), ooov=ph(o, o, o, v), oovv=ph(o, o, v, v), ovov=ph(o, v, o, v),
+                ovvv=ph(o, v, v, v), vvvv=ph(v, v, v, v))
+
+
+def auto_rccsd_energy(T1, T2, fov, oovv):
+    """AutoRCCSD.jl:39-51."""
+    B = -_es("lc,kd->lckd", T1, T1) - _es("lkcd->lckd", T2) + 2 * _es("klcd->lckd", T2)
+    return float(2 * np.sum(fov * T1) + _es("lckd,klcd->", B, oovv) + 2 * _es("lc,kd,lkcd->", T1, T1, oovv))
+
+
+# AutoRCCSD.jl:76-165, one row per term: (coefficient, einsum inputs, operand names);
+# F? are the off-diagonal Fock blocks, V.... the phys integral blocks.
+_AUTO_T1 = [
+    (1, "ia", "Fov"), (-1, "ik,ka", "Foo T1"), (1, "ca,ic", "Fvv T1"), (-1, "kc,ic,ka", "Fov T1 T1"),
+    (2, "kc,ikac", "Fov T2"), (-1, "kc,kiac", "Fov T2"), (-1, "kc,icka", "T1 ovov"), (2, "kc,kica", "T1 oovv"),
+    (-1, "kicd,kadc", "T2 ovvv"), (2, "ikcd,kadc", "T2 ovvv"), (-2, "klac,klic", "T2 ooov"),
+    (1, "lkac,klic", "T2 ooov"), (-2, "kc,la,lkic", "T1 T1 ooov"), (-1, "kc,id,kadc", "T1 T1 ovvv"),
+    (2, "kc,id,kacd", "T1 T1 ovvv"), (1, "kc,la,klic", "T1 T1 ooov"), (-2, "kc,ilad,lkcd", "T1 T2 oovv"),
+    (-2, "kc,liad,klcd", "T1 T2 oovv"), (1, "kc,liad,lkcd", "T1 T2 oovv"), (-2, "ic,lkad,lkcd", "T1 T2 oovv"),
+    (1, "ic,lkad,klcd", "T1 T2 oovv"), (-2, "la,ikdc,klcd", "T1 T2 oovv"), (1, "la,ikcd,klcd", "T1 T2 oovv"),
+    (1, "kc,id,la,lkcd", "T1 T1 T1 oovv"), (-2, "kc,id,la,klcd", "T1 T1 T1 oovv"),
+    (4, "kc,ilad,klcd", "T1 T2 oovv"),
+]
+_AUTO_T2 = [
+    (1, "ijab", "oovv"), (1, "ic,jd,cdab", "T1 T1 vvvv"), (1, "ijcd,cdab", "T2 vvvv"),
+    (1, "ka,lb,ijkl", "T1 T1 oooo"), (1, "klab,ijkl", "T2 oooo"), (-1, "ic,jd,ka,kbcd", "T1 T1 T1 ovvv"),
+    (-1, "ic,jd,kb,kadc", "T1 T1 T1 ovvv"), (1, "ic,ka,lb,lkjc", "T1 T1 T1 ooov"),
+    (1, "jc,ka,lb,klic", "T1 T1 T1 ooov"), (1, "klac,ijdb,klcd", "T2 T2 oovv"),
+    (-2, "ikac,ljbd,klcd", "T2 T2 oovv"), (-2, "lkac,ijdb,klcd", "T2 T2 oovv"),
+    (1, "kiac,ljdb,lkcd", "T2 T2 oovv"), (1, "ikac,ljbd,lkcd", "T2 T2 oovv"),
+    (-2, "ikac,jlbd,lkcd", "T2 T2 oovv"), (1, "kiac,ljbd,klcd", "T2 T2 oovv"),
+    (-2, "kiac,jlbd,klcd", "T2 T2 oovv"), (1, "ijac,lkbd,klcd", "T2 T2 oovv"),
+    (-2, "ijac,klbd,klcd", "T2 T2 oovv"), (1, "kjac,ildb,lkcd", "T2 T2 oovv"),
+    (4, "ikac,jlbd,klcd", "T2 T2 oovv"), (1, "ijdc,lkab,klcd", "T2 T2 oovv"),
+    (1, "ic,jd,ka,lb,klcd", "T1 T1 T1 T1 oovv"), (1, "ic,jd,lkab,lkcd", "T1 T1 T2 oovv"),
+    (1, "ka,lb,ijdc,lkcd", "T1 T1 T2 oovv"),
+]
+_AUTO_P = [
+    (-1, "ik,kjab", "Foo T2"), (1, "ca,ijcb", "Fvv T2"), (-1, "kb,jika", "T1 ooov"), (1, "jc,icab", "T1 ovvv"),
+    (-1, "kc,ic,kjab", "Fov T1 T2"), (-1, "kc,ka,ijcb", "Fov T1 T2"), (-1, "kiac,kjcb", "T2 oovv"),
+    (-1, "ic,ka,kjcb", "T1 T1 oovv"), (-1, "ic,kb,jcka", "T1 T1 ovov"), (2, "ikac,kjcb", "T2 oovv"),
+    (-1, "ikac,jckb", "T2 ovov"), (-1, "kjac,ickb", "T2 ovov"), (-2, "lb,ikac,lkjc", "T1 T2 ooov"),
+    (1, "lb,kiac,lkjc", "T1 T2 ooov"), (-1, "jc,ikdb,kacd", "T1 T2 ovvv"), (-1, "jc,kiad,kbdc", "T1 T2 ovvv"),
+    (-1, "jc,ikad,kbcd", "T1 T2 ovvv"), (1, "jc,lkab,lkic", "T1 T2 ooov"), (1, "lb,ikac,kljc", "T1 T2 ooov"),
+    (-1, "ka,ijdc,kbdc", "T1 T2 ovvv"), (1, "ka,ilcb,lkjc", "T1 T2 ooov"), (2, "jc,ikad,kbdc", "T1 T2 ovvv"),
+    (-1, "kc,ijad,kbdc", "T1 T2 ovvv"), (2, "kc,ijad,kbcd", "T1 T2 ovvv"), (1, "kc,ilab,kljc", "T1 T2 ooov"),
+    (-2, "kc,ilab,lkjc", "T1 T2 ooov"), (1, "jkcd,ilab,klcd", "T2 T2 oovv"),
+    (-2, "kc,jd,ilab,klcd", "T1 T1 T2 oovv"), (1, "kc,jd,ilab,lkcd", "T1 T1 T2 oovv"),
+    (-2, "kc,la,ijdb,klcd", "T1 T1 T2 oovv"), (1, "kc,la,ijdb,lkcd", "T1 T1 T2 oovv"),
+    (1, "ic,ka,ljbd,klcd", "T1 T1 T2 oovv"), (-2, "ic,ka,jlbd,klcd", "T1 T1 T2 oovv"),
+    (1, "ic,ka,ljdb,lkcd", "T1 T1 T2 oovv"), (1, "ic,lb,kjad,klcd", "T1 T1 T2 oovv"),
+    (-2, "ikdc,ljab,klcd", "T2 T2 oovv"),
+]
+
+
+def _auto_sum(terms, out, ops):
+    acc = 0.0
+    for coef, spec, names in terms:
+        acc = acc + coef * _es(spec + "->" + out, *[ops[nm] for nm in names.split()])
+    return acc
+
+
+def auto_rccsd_update(T1, T2, F, V, d, D):
+    """AutoRCCSD.jl:67-177: new T1, T2 and the residues r1, r2 (:170-174).
+    F = (fock_OO, fock_OV, fock_VV) with the diagonal removed; V = auto_rccsd_integrals."""
+    ops = dict(T1=T1, T2=T2, Foo=F[0], Fov=F[1], Fvv=F[2], **V)
+    n1 = _auto_sum(_AUTO_T1, "ia", ops)
+    P = _auto_sum(_AUTO_P, "ijab", ops)
+    n2 = _auto_sum(_AUTO_T2, "ijab", ops) + P + P.transpose(1, 0, 3, 2)
+    t1, t2 = n1 / d, n2 / D
+    r1 = np.sqrt(np.sum((t1 - T1) ** 2)) / T1.size
+    r2 = np.sqrt(np.sum((t2 - T2) ** 2)) / T2.size
+    return t1, t2, r1, r2
+
+
+def solve_auto_rccsd(fock, V, o, *, e_conv=1e-10, max_rms=1e-10, maxit=200):
+    """AutoRCCSD.jl:193-292 (fcn = 0): Fock diagonal in the resolvents, MP2-like guess
+    T1 = f_ov/d, T2 = <ij|ab>/D, plain updates until |dE| <= e_conv and max(r1,r2) <= max_rms."""
+    fd = np.diag(fock)
+    f = fock - np.diag(fd)
+    F = (f[:o, :o], f[:o, o:], f[o:, o:])
+    eo, ev = fd[:o], fd[o:]
+    d = eo[:, None] - ev[None, :]
+    D = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
+    T1, T2 = F[1] / d, V["oovv"] / D
+    E = auto_rccsd_energy(T1, T2, F[1], V["oovv"])
+    dE, rms, it = 1.0, 1.0, 0
+    while abs(dE) > e_conv or rms > max_rms:
+        if it >= maxit:
+            break
+        T1, T2, r1, r2 = auto_rccsd_update(T1, T2, F, V, d, D)
+        rms = max(r1, r2)
+        E_old, E = E, auto_rccsd_energy(T1, T2, F[1], V["oovv"])
+        dE = E - E_old
+        it += 1
+    return E, T1, T2, it
+
+
+def ccsd_spin_orbital(fock, g_mo, o, *, e_conv=1e-13, r_conv=1e-12, maxit=400):
+    """Independent check of the Fock-driven CCSD: the spin-orbital CCSD equations of
+    Stanton, Gauss, Watts and Bartlett (J. Chem. Phys. 94, 4334 (1991)) with the full
+    Fock matrix (any reference), from a closed-shell MO Fock matrix and chemist MO
+    integrals; Jacobi iterations with DIIS.  Returns the correlation energy
+    sum f_ia t_ia + 1/4 <ij||ab> t_ijab + 1/2 <ij||ab> t_ia t_jb."""
+    n = fock.shape[0]
+    sp = np.repeat(np.arange(n), 2)
+    spin = np.tile([0, 1], n)
+    same = spin[:, None] == spin[None, :]
+    f = fock[np.ix_(sp, sp)] * same
+    phys = np.transpose(g_mo, (0, 2, 1, 3))[np.ix_(sp, sp, sp, sp)]
+    phys = phys * same[:, None, :, None] * same[None, :, None, :]
+    A = phys - phys.transpose(0, 1, 3, 2)
+    O_, V_ = slice(0, 2 * o), slice(2 * o, 2 * n)
+    fd = np.diag(f)
+    Dia = fd[O_, None] - fd[None, V_]
+    Dijab = fd[O_, None, None, None] + fd[None, O_, None, None] - fd[None, None, V_, None] - fd[None, None, None, V_]
+    g = lambda a, b, c, d: A[a, b, c, d]  # noqa: E731
+    oo, ov, vv = f[O_, O_], f[O_, V_], f[V_, V_]
+    t1 = ov / Dia
+    t2 = g(O_, O_, V_, V_) / Dijab
+    no, nv = 2 * o, 2 * (n - o)
+    Io, Iv = np.eye(no), np.eye(nv)
+
+    def energy(t1, t2):
+        return float(np.sum(ov * t1) + 0.25 * np.sum(g(O_, O_, V_, V_) * t2)
+                     + 0.5 * _es("ijab,ia,jb->", g(O_, O_, V_, V_), t1, t1))
+
+    hist = []
+    E = energy(t1, t2)
+    for _ in range(maxit):
+        taut = t2 + 0.5 * (_es("ia,jb->ijab", t1, t1) - _es("ib,ja->ijab", t1, t1))
+        tau = t2 + _es("ia,jb->ijab", t1, t1) - _es("ib,ja->ijab", t1, t1)
+        Fae = (1 - Iv) * vv - 0.5 * _es("me,ma->ae", ov, t1) + _es("mf,mafe->ae", t1, g(O_, V_, V_, V_)) \
+            - 0.5 * _es("mnaf,mnef->ae", taut, g(O_, O_, V_, V_))
+        Fmi = (1 - Io) * oo + 0.5 * _es("ie,me->mi", t1, ov) + _es("ne,mnie->mi", t1, g(O_, O_, O_, V_)) \
+            + 0.5 * _es("inef,mnef->mi", taut, g(O_, O_, V_, V_))
+        Fme = ov + _es("nf,mnef->me", t1, g(O_, O_, V_, V_))
+        Wmnij = g(O_, O_, O_, O_) + _es("je,mnie->mnij", t1, g(O_, O_, O_, V_)) \
+            - _es("ie,mnje->mnij", t1, g(O_, O_, O_, V_)) + 0.25 * _es("ijef,mnef->mnij", tau, g(O_, O_, V_, V_))
+        Wabef = g(V_, V_, V_, V_) - _es("mb,amef->abef", t1, g(V_, O_, V_, V_)) \
+            + _es("ma,bmef->abef", t1, g(V_, O_, V_, V_)) + 0.25 * _es("mnab,mnef->abef", tau, g(O_, O_, V_, V_))
+        Wmbej = g(O_, V_, V_, O_) + _es("jf,mbef->mbej", t1, g(O_, V_, V_, V_)) \
+            - _es("nb,mnej->mbej", t1, g(O_, O_, V_, O_)) \
+            - _es("jnfb,mnef->mbej", 0.5 * t2 + _es("jf,nb->jnfb", t1, t1), g(O_, O_, V_, V_))
+        r1 = ov + _es("ie,ae->ia", t1, Fae) - _es("ma,mi->ia", t1, Fmi) + _es("imae,me->ia", t2, Fme) \
+            - _es("nf,naif->ia", t1, g(O_, V_, O_, V_)) - 0.5 * _es("imef,maef->ia", t2, g(O_, V_, V_, V_)) \
+            - 0.5 * _es("mnae,nmei->ia", t2, g(O_, O_, V_, O_))
+        Fb = Fae - 0.5 * _es("mb,me->be", t1, Fme)
+        Fj = Fmi + 0.5 * _es("je,me->mj", t1, Fme)
+        x = _es("ijae,be->ijab", t2, Fb)
+        r2 = g(O_, O_, V_, V_) + x - x.transpose(0, 1, 3, 2)
+        x = _es("imab,mj->ijab", t2, Fj)
+        r2 -= x - x.transpose(1, 0, 2, 3)
+        r2 += 0.5 * _es("mnab,mnij->ijab", tau, Wmnij) + 0.5 * _es("ijef,abef->ijab", tau, Wabef)
+        x = _es("imae,mbej->ijab", t2, Wmbej) - _es("ie,ma,mbej->ijab", t1, t1, g(O_, V_, V_, O_))
+        r2 += x - x.transpose(1, 0, 2, 3) - x.transpose(0, 1, 3, 2) + x.transpose(1, 0, 3, 2)
+        x = _es("ie,abej->ijab", t1, g(V_, V_, V_, O_))
+        r2 += x - x.transpose(1, 0, 2, 3)
+        x = _es("ma,mbij->ijab", t1, g(O_, V_, O_, O_))
+        r2 -= x - x.transpose(0, 1, 3, 2)
+        n1, n2 = r1 / Dia, r2 / Dijab
+        e = np.concatenate([(n1 - t1).ravel(), (n2 - t2).ravel()])
+        hist.append((np.concatenate([n1.ravel(), n2.ravel()]), e))
+        hist = hist[-8:]
+        m = len(hist)
+        B = -np.ones((m + 1, m + 1))
+        B[m, m] = 0
+        for a_ in range(m):
+            for b_ in range(m):
+                B[a_, b_] = hist[a_][1] @ hist[b_][1]
+        rhs = np.zeros(m + 1)
+        rhs[m] = -1
+        c = np.linalg.solve(B, rhs)
+        x = sum(c[k] * hist[k][0] for k in range(m))
+        t1, t2 = x[:t1.size].reshape(t1.shape), x[t1.size:].reshape(t2.shape)
+        E_old, E = E, energy(t1, t2)
+        if abs(E - E_old) < e_conv and np.linalg.norm(e) < r_conv:
+            break
+    return E
