@@ -109,6 +109,8 @@ def main():
                     help="instead of RCCD: (T) on the o=60 v=340 factors with seeded amplitudes")
     ap.add_argument("--pt-o", type=int, default=60, help="--triples: occupied orbitals used (first pt-o)")
     ap.add_argument("--pt-prec", default="fp64,fp32_mixed", help="--triples: forms to run")
+    ap.add_argument("--n4", action="store_true",
+                    help="instead of RCCD: UMP2, get_fock and Fock-driven CCSD iterations at o=60 v=340")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
@@ -122,6 +124,9 @@ def main():
         return
     if args.triples:
         print(json.dumps(triples_bench(args.pt_o, args.pt_prec.split(","))), flush=True)
+        return
+    if args.n4:
+        print(json.dumps(n4_bench()), flush=True)
         return
 
     from jues import synthetic, _lib
@@ -356,6 +361,61 @@ def triples_bench(o: int, forms) -> dict:
     if "fp64" in out["forms"] and "fp32_mixed" in out["forms"]:
         e64, e32 = out["forms"]["fp64"]["Et"], out["forms"]["fp32_mixed"]["Et"]
         out["fp32_rel_diff"] = abs(e32 - e64) / abs(e64)
+    return out
+
+
+def n4_bench() -> dict:
+    """On the n=400 factors: UMP2 (alpha = the o=60 orbitals, beta = a rotated set with 59
+    occupied), get_fock, and CCSD iterations (RCCSD equations, o=60, v=340, integrals
+    scaled by 0.005) with a canonical and a non-canonical Fock matrix (diag(eps) plus a
+    seeded symmetric 1e-3 perturbation); device times from HIP events."""
+    import numpy as np
+    from scipy.linalg import expm
+    from jues import synthetic, MollerPlesset, _lib
+    from jues import CoupledCluster as CC
+    from jues.IntegralTransformation import get_fock
+    from jues.Wavefunction import Wfn, FactoredERI
+    s = synthetic.workload("cfg4_rccd_n400")
+    n, o = s.n, s.o
+    rng = np.random.default_rng(23)
+    K = rng.standard_normal((n, n)) * 0.02
+    Cb = s.C @ expm(K - K.T)
+    h = rng.standard_normal((n, n))
+    h = (h + h.T) / 2
+    eri = FactoredERI(s.B)
+    ctx = _lib.context(0)
+    out = {"workload": f"n={n} factors (naux={s.naux}), o={o} v={s.v}", "stages": {}}
+
+    def timed(label, fn, reps=2):
+        for _ in range(reps - 1):
+            fn()
+        ctx.stats_reset(True)
+        r = fn()
+        st = ctx.stats()
+        ctx.stats_reset(False)
+        dev = sum(x["ms"] for x in st.values())
+        out["stages"][label] = {"device_ms": round(dev, 2), "kernels_ms": {k: round(x["ms"], 2) for k, x in st.items()}}
+        print(f"[bench] {label}: device {dev:.1f} ms", file=sys.stderr, flush=True)
+        return r
+
+    wu = Wfn.from_uhf(s.C, Cb, s.eps, s.eps, o, o - 1, eri, hao=h)
+    out["E_ump2_fp64"] = timed("ump2_fp64", lambda: MollerPlesset.do_ump2(wu))
+    out["E_ump2_fp32_rel_diff"] = abs(timed("ump2_fp32_mixed", lambda: MollerPlesset.do_ump2(wu, prec="fp32_mixed"))
+                                      - out["E_ump2_fp64"]) / abs(out["E_ump2_fp64"])
+    timed("get_fock", lambda: get_fock(wu))
+    # integrals scaled by 0.005 (the factors by its square root), where the iteration converges
+    wr = Wfn.from_orbitals(s.C, s.eps, o, FactoredERI(np.asfortranarray(s.B * np.sqrt(0.005))))
+    P = rng.standard_normal((n, n)) * 1e-3
+    fock = np.diag(s.eps) + (P + P.T) / 2
+    for label, fk in (("rccsd_canonical", None), ("rccsd_fock_noncanonical", fock)):
+        for prec in ("fp64", "fp32_mixed"):
+            hist = []
+            kw = dict(prec=prec, maxit=3, refine_maxit=0, diis=False, check_convergence=False, ccsd_eqs="RCCSD",
+                      callback=hist.append)
+            CC._run(1, wr, kw, "mp2", fock=fk)
+            ms = [round(x["ms"], 1) for x in hist if x["stage"] == (0 if prec == "fp64" else 1)]
+            out["stages"][f"{label}_{prec}"] = {"ms_per_iteration": ms}
+            print(f"[bench] {label} {prec}: {ms} ms/iter", file=sys.stderr, flush=True)
     return out
 
 

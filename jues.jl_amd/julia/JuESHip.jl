@@ -170,7 +170,7 @@ end
 # ---------------------------------------------------------------------------
 module IntegralTransformation
 import ..JuESHip: context, ensure_ao!, check, libjues, precision_code
-export get_eri
+export get_eri, get_fock
 
 function get_eri(wfn, eri_string::String; notation::String="phys", fcn::Int=0, precision=:fp64)
     sizeof(eri_string) == 4 || error("Invalid string given to JuES.IntegralTransformation.get_eri: $eri_string")
@@ -185,6 +185,27 @@ function get_eri(wfn, eri_string::String; notation::String="phys", fcn::Int=0, p
         ctx.handle, precision_code(precision), eri_string, notation, Co, o, Cv, v, n, fcn, out))
     return out
 end
+
+"IntegralTransformation.jl:119-141: MO Fock matrix of the spin's occupied density (needs `wfn.hao`)."
+function get_fock(wfn; spin="alpha")
+    sp = lowercase(spin)
+    if sp in ("alpha", "up", "a")
+        C, Co = wfn.Ca, wfn.Cao
+    elseif sp in ("beta", "down", "b")
+        C, Co = wfn.Cb, wfn.Cbo
+    else
+        error("Invalid Spin option given to JuES.IntegralTransformation.get_fock: $spin")
+    end
+    ctx = context()
+    n = ensure_ao!(ctx, wfn.ao_eri)
+    h = wfn.hao
+    m = size(C, 2)
+    f = zeros(m, m)
+    GC.@preserve h C Co f check(ctx, ccall((:jues_get_fock, libjues), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}),
+        ctx.handle, h, C, m, Co, size(Co, 2), n, f))
+    return f
+end
 end
 
 # ---------------------------------------------------------------------------
@@ -193,7 +214,7 @@ end
 module MollerPlesset
 import ..JuESHip: context, ensure_ao!, check, libjues, precision_code, FactoredERI, DFIntegrals
 using ChainRulesCore
-export do_rmp2, do_df_rmp2, rmp2_energy
+export do_rmp2, do_df_rmp2, do_ump2, rmp2_energy
 
 function do_rmp2(refWfn; precision=:fp64, kwargs...)
     ctx = context()
@@ -203,6 +224,21 @@ function do_rmp2(refWfn; precision=:fp64, kwargs...)
     GC.@preserve Co Cv eps check(ctx, ccall((:jues_rmp2, libjues), Cint,
         (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}),
         ctx.handle, precision_code(precision), Co, refWfn.nalpha, Cv, refWfn.nvira, n, eps, E))
+    return E[]
+end
+
+"UMP2.jl:12-79 on the device (alpha-alpha, alpha-beta, beta-beta blocks)."
+function do_ump2(refWfn; precision=:fp64)
+    ctx = context()
+    n = ensure_ao!(ctx, refWfn.ao_eri)
+    Cao, Cav, Cbo, Cbv = refWfn.Cao, refWfn.Cav, refWfn.Cbo, refWfn.Cbv
+    ea, eb = refWfn.epsa, refWfn.epsb
+    E = Ref{Float64}(0.0)
+    GC.@preserve Cao Cav Cbo Cbv ea eb check(ctx, ccall((:jues_ump2, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint,
+         Ptr{Float64}, Ptr{Float64}, Ref{Float64}),
+        ctx.handle, precision_code(precision), Cao, size(Cao, 2), Cav, size(Cav, 2), Cbo, size(Cbo, 2), Cbv,
+        size(Cbv, 2), n, ea, eb, E))
     return E[]
 end
 
@@ -343,6 +379,34 @@ function rccsd_pT(refWfn, T1::Array{Float64,2}, T2::Array{Float64,4}; precision=
          Cint, Ref{Float64}),
         ctx.handle, precision_code(precision), Co, refWfn.nalpha, Cv, refWfn.nvira, n, eps, T1, T2, doprint, Et))
     return Et[]
+end
+end
+
+module AutoRCCSD
+import ...JuESHip: context, ensure_ao!, check, libjues, precision_code, CCOpts
+import ...JuESHip.IntegralTransformation: get_fock
+"""AutoRCCSD.jl:193-300: CCSD driven by the Wfn's MO Fock matrix (off-diagonal blocks in the
+equations, its diagonal in the resolvents); plain updates as the reference unless diis=true."""
+function do_rccsd(wfn; precision=:fp64, return_T=false, diis=false, kwargs...)
+    f = get_fock(wfn; spin="alpha")
+    kw = Dict{Symbol,Any}(kwargs)
+    haskey(kw, :ccsd_eqs) || (kw[:ccsd_eqs] = :RCCSD)
+    haskey(kw, :guess) || (kw[:guess] = :mp2)
+    opts = Ref(CCOpts(; diis=diis, kw...))
+    ctx = context()
+    n = ensure_ao!(ctx, wfn.ao_eri)
+    o, v = wfn.nalpha, wfn.nvira
+    oa = o - opts[].fcn
+    Co, Cv = wfn.Cao, wfn.Cav
+    T2 = zeros(oa, oa, v, v)
+    T1 = zeros(oa, v)
+    E = Ref{Float64}(0.0)
+    st = GC.@preserve Co Cv f T1 T2 ccall((:jues_rccsd_fock, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ref{CCOpts}, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64},
+         Ref{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid}, Ptr{Cvoid}),
+        ctx.handle, precision_code(precision), opts, Co, o, Cv, v, n, f, E, T1, T2, C_NULL, C_NULL)
+    check(ctx, st)
+    return return_T ? (E[], T1, T2) : E[]
 end
 end
 
