@@ -255,6 +255,134 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
   }
 }
 
+// FP32 C = alpha A B + beta C with A m-contiguous and B k-contiguous (the NN case of
+// every FP32 product on the coupled-cluster path), 128x128 tile, 4 wavefronts of 64x64.
+// The K slice is 16 deep, so a workgroup needs 32 KB of LDS and at most 128 VGPRs per
+// lane: four workgroups (16 wavefronts) share a CU and cover each other's load and
+// barrier latency (two workgroups of 32-deep slices reach 73% MFMA-busy, this form ~88%).
+// LDS holds each slice as 64-byte rows of 16 k-values per m (A) or n (B); the 16-byte
+// chunk q of row r sits at chunk q ^ ((r >> 2) & 3), so slices are written with 16-byte
+// stores and fragments read with 16-byte loads.  Lane l of a 16x16x4 MFMA supplies
+// k = 4 (l / 16) + s at step s = 0..3 (the same permutation for both operands): one
+// 16-byte read per 16-row block.  B's four fragment blocks are held for the slice and A's
+// are streamed one block ahead of the MFMAs that use them.  K must be a multiple of 16
+// (no tail slice); rows of the last tile row or column beyond M or N load a clamped row
+// and are not stored.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
+
+template <int KIND>
+__global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, float alpha,
+                                                           const float* __restrict__ A, long lda, long sA,
+                                                           const float* __restrict__ B, long ldb, long sB,
+                                                           float beta, float* __restrict__ C, long ldc, long sC,
+                                                           int tiles_m, int tiles_n) {
+  constexpr int BM = 128, BN = 128, BK = 16;
+  __shared__ __attribute__((aligned(16))) float As[2][BM * BK];
+  __shared__ __attribute__((aligned(16))) float Bs[2][BN * BK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  A += (long)blockIdx.z * sA;
+  B += (long)blockIdx.z * sB;
+  C += (long)blockIdx.z * sC;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  // A share: row am, 4-k chunks ac0 and ac0 + 2; B share: rows bn and bn + 64, chunk bc
+  const int am = tid & 127, ac0 = tid >> 7;
+  const int bn = tid >> 2, bc = tid & 3;
+  float ra[8];
+  f4v rb[2];
+  // rows past the edge of M or N load the last row instead (their results are not stored)
+  const int a_voff = 4 * (min(m0 + am, M - 1) - m0);
+  const int b_voff0 = 4 * (4 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
+  const int b_voff1 = 4 * (4 * bc + (min(n0 + bn + 64, N - 1) - n0) * (int)ldb);
+  auto load = [&](int k0) {
+    const auto rA = make_rsrc(A + m0 + (long)k0 * lda);
+    const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        ra[4 * j + u] = bload<float>(rA, a_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)lda);
+    rb[0] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
+    rb[1] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff1, 0, 0));
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = ac0 + 2 * j;
+      *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(c, am)]) =
+          f4v{ra[4 * j], ra[4 * j + 1], ra[4 * j + 2], ra[4 * j + 3]};
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = bn + 64 * j;
+      *reinterpret_cast<f4v*>(&Bs[buf][n * BK + 4 * g32_chunk(bc, n)]) = rb[j];
+    }
+  };
+
+  fvec4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
+
+  const int fr = lane & 15, fk = lane >> 4;
+  const int nk = K / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    f4v bf[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = wn * 64 + j * 16 + fr;
+      bf[j] = *reinterpret_cast<const f4v*>(&Bs[cur][r * BK + 4 * g32_chunk(fk, r)]);
+    }
+    f4v a0 = *reinterpret_cast<const f4v*>(&As[cur][(wm * 64 + fr) * BK + 4 * g32_chunk(fk, wm * 64 + fr)]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      f4v a1;
+      if (i + 1 < 4) {
+        const int r = wm * 64 + (i + 1) * 16 + fr;
+        a1 = *reinterpret_cast<const f4v*>(&As[cur][r * BK + 4 * g32_chunk(fk, r)]);
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j][s2], a0[s2], acc[i][j], 0, 0, 0);
+      if (i + 1 < 4) a0 = a1;
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + wm * 64 + i * 16 + (lane & 15);
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * 64 + j * 16 + 4 * (lane >> 4) + r;
+        if (n < N) {
+          const long off = (long)m + (long)n * ldc;
+          float v = alpha * acc[i][j][r];
+          if (beta != 0.f) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ void scale_kernel(T* C, long M, long N, long ldc, T beta) {
   const long total = M * N;
@@ -280,6 +408,8 @@ __global__ void splitk_reduce_kernel(const T* __restrict__ W, int S, long M, lon
 }
 
 }  // namespace
+
+static inline long tiles_m128(long x) { return (x + 127) / 128; }
 
 template <typename T>
 static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
@@ -330,6 +460,30 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
+  // FP32 NN with K a multiple of 16: gemm32_nn_kernel (four workgroups per CU)
+  if constexpr (sizeof(T) == 4) if (!ta && !tb && K % 16 == 0 && shape == 0) {
+    const long span32 = 4L * (128 + 64) * (std::max(lda, ldb) + 1);
+    if (span32 < (1L << 31)) {
+      KTimer timer(ctx, name, flops, bytes);
+      for (int b0 = 0; b0 < batch; b0 += 65535) {
+        const int nb = std::min(batch - b0, 65535);
+        dim3 grid((unsigned)(tiles_m128(M) * tiles_m128(N)), 1, nb);
+        const float* Af = reinterpret_cast<const float*>(A + (long)b0 * sA);
+        const float* Bf = reinterpret_cast<const float*>(B + (long)b0 * sB);
+        float* Cf = reinterpret_cast<float*>(C + (long)b0 * sC);
+        const float fa = (float)alpha, fb = (float)beta;
+#define JUES_GEMM32(K_)                                                                                          \
+  hipLaunchKernelGGL((gemm32_nn_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, fa, Af, lda, sA, \
+                     Bf, ldb, sB, fb, Cf, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
+        if (kind == 1) JUES_GEMM32(1);
+        else if (kind == 2) JUES_GEMM32(2);
+        else JUES_GEMM32(0);
+#undef JUES_GEMM32
+        JUES_HIP(hipGetLastError());
+      }
+      return;
+    }
+  }
   KTimer timer(ctx, name, flops, bytes);
   // the buffer-load path addresses a tile's k-slice with 32-bit byte offsets
   const long span = (long)sizeof(T) * (std::max<long>(BM, BN) + 64) * (std::max(lda, ldb) + 1);
