@@ -265,9 +265,9 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
 // stores and fragments read with 16-byte loads.  Lane l of a 16x16x4 MFMA supplies
 // k = 4 (l / 16) + s at step s = 0..3 (the same permutation for both operands): one
 // 16-byte read per 16-row block.  B's four fragment blocks are held for the slice and A's
-// are streamed one block ahead of the MFMAs that use them.  K must be a multiple of 16
-// (no tail slice); rows of the last tile row or column beyond M or N load a clamped row
-// and are not stored.
+// are streamed one block ahead of the MFMAs that use them.  Rows of the last tile row or
+// column beyond M or N load a clamped row and are not stored.  K must be a positive
+// multiple of 16: the host adds a K tail with a separate rank update.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
@@ -460,11 +460,15 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
-  // FP32 NN with K a multiple of 16: gemm32_nn_kernel (four workgroups per CU)
-  if constexpr (sizeof(T) == 4) if (!ta && !tb && K % 16 == 0 && shape == 0) {
+  // FP32 NN: gemm32_nn_kernel (four workgroups per CU) over the first 16*floor(K/16) columns
+  // of A; a K tail is added afterwards by the general kernel as a rank update with beta = 1
+  if constexpr (sizeof(T) == 4) if (!ta && !tb && shape == 0 && K >= 16) {
     const long span32 = 4L * (128 + 64) * (std::max(lda, ldb) + 1);
+    const long K16 = K / 16 * 16;
     if (span32 < (1L << 31)) {
-      KTimer timer(ctx, name, flops, bytes);
+      {
+      KTimer timer(ctx, name, 2.0 * M * N * K16 * batch,
+                   4.0 * ((double)M * K16 + (double)K16 * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch);
       for (int b0 = 0; b0 < batch; b0 += 65535) {
         const int nb = std::min(batch - b0, 65535);
         dim3 grid((unsigned)(tiles_m128(M) * tiles_m128(N)), 1, nb);
@@ -473,7 +477,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
         float* Cf = reinterpret_cast<float*>(C + (long)b0 * sC);
         const float fa = (float)alpha, fb = (float)beta;
 #define JUES_GEMM32(K_)                                                                                          \
-  hipLaunchKernelGGL((gemm32_nn_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, fa, Af, lda, sA, \
+  hipLaunchKernelGGL((gemm32_nn_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K16, fa, Af, lda, sA, \
                      Bf, ldb, sB, fb, Cf, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
         if (kind == 1) JUES_GEMM32(1);
         else if (kind == 2) JUES_GEMM32(2);
@@ -481,6 +485,10 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
 #undef JUES_GEMM32
         JUES_HIP(hipGetLastError());
       }
+      }
+      if (K > K16)
+        gemm_impl<T>(ctx, false, false, M, N, K - K16, alpha, A + K16 * lda, lda, sA, B + K16, ldb, sB, T(1), C, ldc,
+                     sC, batch, "gemm_ktail", false);
       return;
     }
   }
