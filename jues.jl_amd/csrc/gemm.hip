@@ -255,8 +255,8 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
   }
 }
 
-// FP32 C = alpha A B + beta C with A m-contiguous and B k-contiguous (the NN case of
-// every FP32 product on the coupled-cluster path), 128x128 tile, 4 wavefronts of 64x64.
+// FP32 C = alpha A op(B) + beta C with A m-contiguous and B k-contiguous (NN) or
+// n-contiguous (NT), 128x128 tile, 4 wavefronts of 64x64.
 // The K slice is 16 deep, so a workgroup needs 32 KB of LDS and at most 128 VGPRs per
 // lane: four workgroups (16 wavefronts) share a CU and cover each other's load and
 // barrier latency (two workgroups of 32-deep slices reach 73% MFMA-busy, this form ~88%).
@@ -272,7 +272,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
 
-template <int KIND>
+template <int KIND, bool TB>
 __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, float alpha,
                                                            const float* __restrict__ A, long lda, long sA,
                                                            const float* __restrict__ B, long ldb, long sB,
@@ -291,24 +291,37 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, 
   const int m0 = tm * BM, n0 = tn * BN;
 
   // A share: row am, 4-k chunks ac0 and ac0 + 2; B share: rows bn and bn + 64, chunk bc
-  const int am = tid & 127, ac0 = tid >> 7;
+  // (ac0 is uniform within a wavefront: in a scalar register the per-k offsets are scalar too)
+  const int am = tid & 127, ac0 = __builtin_amdgcn_readfirstlane(tid >> 7);
   const int bn = tid >> 2, bc = tid & 3;
   float ra[8];
   f4v rb[2];
-  // rows past the edge of M or N load the last row instead (their results are not stored)
+  // rows past the edge of M or N load the last row instead (their results are not stored).
+  // B k-contiguous (TB false): 16-byte loads of 4 k; B n-contiguous (TB true, op(B) = B^T):
+  // loaded like A, one dword per k with row n = am of the tile.
   const int a_voff = 4 * (min(m0 + am, M - 1) - m0);
+  const int bt_voff = 4 * (min(n0 + am, N - 1) - n0);
   const int b_voff0 = 4 * (4 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
   const int b_voff1 = 4 * (4 * bc + (min(n0 + bn + 64, N - 1) - n0) * (int)ldb);
   auto load = [&](int k0) {
     const auto rA = make_rsrc(A + m0 + (long)k0 * lda);
-    const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         ra[4 * j + u] = bload<float>(rA, a_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)lda);
-    rb[0] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
-    rb[1] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff1, 0, 0));
+    if constexpr (TB) {
+      const auto rB = make_rsrc(B + n0 + (long)k0 * ldb);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          rb[j][u] = bload<float>(rB, bt_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)ldb);
+    } else {
+      const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
+      rb[0] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
+      rb[1] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff1, 0, 0));
+    }
   };
   auto store = [&](int buf) {
 #pragma unroll
@@ -319,8 +332,13 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, 
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int n = bn + 64 * j;
-      *reinterpret_cast<f4v*>(&Bs[buf][n * BK + 4 * g32_chunk(bc, n)]) = rb[j];
+      if constexpr (TB) {
+        const int c = ac0 + 2 * j;
+        *reinterpret_cast<f4v*>(&Bs[buf][am * BK + 4 * g32_chunk(c, am)]) = rb[j];
+      } else {
+        const int n = bn + 64 * j;
+        *reinterpret_cast<f4v*>(&Bs[buf][n * BK + 4 * g32_chunk(bc, n)]) = rb[j];
+      }
     }
   };
 
@@ -460,9 +478,9 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
-  // FP32 NN: gemm32_nn_kernel (four workgroups per CU) over the first 16*floor(K/16) columns
+  // FP32 NN and NT: gemm32_nn_kernel (four workgroups per CU) over the first 16*floor(K/16) columns
   // of A; a K tail is added afterwards by the general kernel as a rank update with beta = 1
-  if constexpr (sizeof(T) == 4) if (!ta && !tb && shape == 0 && K >= 16) {
+  if constexpr (sizeof(T) == 4) if (!ta && shape == 0 && K >= 16) {
     const long span32 = 4L * (128 + 64) * (std::max(lda, ldb) + 1);
     const long K16 = K / 16 * 16;
     if (span32 < (1L << 31)) {
@@ -476,19 +494,20 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
         const float* Bf = reinterpret_cast<const float*>(B + (long)b0 * sB);
         float* Cf = reinterpret_cast<float*>(C + (long)b0 * sC);
         const float fa = (float)alpha, fb = (float)beta;
-#define JUES_GEMM32(K_)                                                                                          \
-  hipLaunchKernelGGL((gemm32_nn_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K16, fa, Af, lda, sA, \
-                     Bf, ldb, sB, fb, Cf, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
-        if (kind == 1) JUES_GEMM32(1);
-        else if (kind == 2) JUES_GEMM32(2);
-        else JUES_GEMM32(0);
+#define JUES_GEMM32(K_, TB_)                                                                                     \
+  hipLaunchKernelGGL((gemm32_nn_kernel<K_, TB_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K16, fa, Af, lda, \
+                     sA, Bf, ldb, sB, fb, Cf, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
+        if (tb) JUES_GEMM32(0, true);
+        else if (kind == 1) JUES_GEMM32(1, false);
+        else if (kind == 2) JUES_GEMM32(2, false);
+        else JUES_GEMM32(0, false);
 #undef JUES_GEMM32
         JUES_HIP(hipGetLastError());
       }
       }
       if (K > K16)
-        gemm_impl<T>(ctx, false, false, M, N, K - K16, alpha, A + K16 * lda, lda, sA, B + K16, ldb, sB, T(1), C, ldc,
-                     sC, batch, "gemm_ktail", false);
+        gemm_impl<T>(ctx, false, tb, M, N, K - K16, alpha, A + K16 * lda, lda, sA, tb ? B + K16 * ldb : B + K16, ldb,
+                     sB, T(1), C, ldc, sC, batch, "gemm_ktail", false);
       return;
     }
   }
