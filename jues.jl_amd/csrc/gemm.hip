@@ -401,6 +401,139 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, 
   }
 }
 
+// FP64 C = alpha A op(B) + beta C with A m-contiguous and B k-contiguous (NN) or
+// n-contiguous (NT): 128x128 tile over 8 wavefronts of 64x32, so the accumulators take 64
+// VGPRs and a lane stays within 128 -- two 512-thread workgroups (16 wavefronts) per CU.
+// K slices are 16 doubles deep, held in LDS as 128-byte rows per m (A) or n (B) with the
+// 16-byte chunk q of row r at chunk q ^ (r & 7); lane l supplies k = 4 (l / 16) + s at MFMA
+// step s = 0..3 for both operands (two 16-byte fragment reads per 16-row block).  B's two
+// blocks are held for the slice and A's four streamed one ahead.  Rows past M or N load a
+// clamped row and are not stored; K must be a positive multiple of 16 (the host adds a tail).
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+__device__ inline int g64_chunk(int q, int r) { return q ^ (r & 7); }
+
+template <int KIND, bool TB>
+__global__ __launch_bounds__(512, 4) void gemm64_kernel(int M, int N, int K, double alpha,
+                                                        const double* __restrict__ A, long lda, long sA,
+                                                        const double* __restrict__ B, long ldb, long sB,
+                                                        double beta, double* __restrict__ C, long ldc, long sC,
+                                                        int tiles_m, int tiles_n) {
+  constexpr int BM = 128, BN = 128, BK = 16;
+  __shared__ __attribute__((aligned(16))) double As[2][BM * BK];
+  __shared__ __attribute__((aligned(16))) double Bs[2][BN * BK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 wavefronts of 64 x 32
+  A += (long)blockIdx.z * sA;
+  B += (long)blockIdx.z * sB;
+  C += (long)blockIdx.z * sC;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  // A (and n-contiguous B) share: row am, k = 4 kq + u; k-contiguous B share: rows bn, bn + 64, chunk bc
+  const int am = tid & 127, kq = __builtin_amdgcn_readfirstlane(tid >> 7);
+  const int bn = tid >> 3, bc = tid & 7;
+  double ra[4];
+  d2v rb[2];
+  const int a_voff = 8 * (min(m0 + am, M - 1) - m0);
+  const int bt_voff = 8 * (min(n0 + am, N - 1) - n0);
+  const int b_voff0 = 8 * (2 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
+  const int b_voff1 = 8 * (2 * bc + (min(n0 + bn + 64, N - 1) - n0) * (int)ldb);
+  auto load = [&](int k0) {
+    const auto rA = make_rsrc(A + m0 + (long)k0 * lda);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (4 * kq + u) * (int)lda);
+    if constexpr (TB) {
+      const auto rB = make_rsrc(B + n0 + (long)k0 * ldb);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) rb[u >> 1][u & 1] = bload<double>(rB, bt_voff, 8 * (4 * kq + u) * (int)ldb);
+    } else {
+      const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
+      rb[0] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
+      rb[1] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff1, 0, 0));
+    }
+  };
+  auto store = [&](int buf) {
+    *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(2 * kq, am)]) = d2v{ra[0], ra[1]};
+    *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(2 * kq + 1, am)]) = d2v{ra[2], ra[3]};
+    if constexpr (TB) {
+      *reinterpret_cast<d2v*>(&Bs[buf][am * BK + 2 * g64_chunk(2 * kq, am)]) = rb[0];
+      *reinterpret_cast<d2v*>(&Bs[buf][am * BK + 2 * g64_chunk(2 * kq + 1, am)]) = rb[1];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = bn + 64 * j;
+        *reinterpret_cast<d2v*>(&Bs[buf][n * BK + 2 * g64_chunk(bc, n)]) = rb[j];
+      }
+    }
+  };
+
+  dvec4 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = dvec4{0, 0, 0, 0};
+  const int fr = lane & 15, fk = lane >> 4;
+  const int nk = K / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    d2v bf[2][2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int r = wn * 32 + j * 16 + fr;
+      bf[j][0] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+      bf[j][1] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+    }
+    d2v a0[2], a1[2];
+    {
+      const int r = wm * 64 + fr;
+      a0[0] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+      a0[1] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i + 1 < 4) {
+        const int r = wm * 64 + (i + 1) * 16 + fr;
+        a1[0] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+        a1[1] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j][s2 >> 1][s2 & 1], a0[s2 >> 1][s2 & 1], acc[i][j],
+                                                           0, 0, 0);
+      if (i + 1 < 4) { a0[0] = a1[0]; a0[1] = a1[1]; }
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+
+  // acc[i][j][r]: C(m, n) with m = lane % 16 of block i, n = lane / 16 + 4 r of block j
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + wm * 64 + i * 16 + (lane & 15);
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * 32 + j * 16 + (lane >> 4) + 4 * r;
+        if (n < N) {
+          const long off = (long)m + (long)n * ldc;
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ void scale_kernel(T* C, long M, long N, long ldc, T beta) {
   const long total = M * N;
@@ -504,6 +637,38 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
 #undef JUES_GEMM32
         JUES_HIP(hipGetLastError());
       }
+      }
+      if (K > K16)
+        gemm_impl<T>(ctx, false, tb, M, N, K - K16, alpha, A + K16 * lda, lda, sA, tb ? B + K16 * ldb : B + K16, ldb,
+                     sB, T(1), C, ldc, sC, batch, "gemm_ktail", false);
+      return;
+    }
+  }
+  // FP64 NN and NT: gemm64_kernel (two 512-thread workgroups per CU), same K split
+  if constexpr (sizeof(T) == 8) if (!ta && shape == 0 && K >= 16) {
+    const long span64 = 8L * (128 + 64) * (std::max(lda, ldb) + 1);
+    const long K16 = K / 16 * 16;
+    if (span64 < (1L << 31)) {
+      {
+        KTimer timer(ctx, name, 2.0 * M * N * K16 * batch,
+                     8.0 * ((double)M * K16 + (double)K16 * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch);
+        for (int b0 = 0; b0 < batch; b0 += 65535) {
+          const int nb = std::min(batch - b0, 65535);
+          dim3 grid((unsigned)(tiles_m128(M) * tiles_m128(N)), 1, nb);
+          const double* Ad = reinterpret_cast<const double*>(A + (long)b0 * sA);
+          const double* Bd = reinterpret_cast<const double*>(B + (long)b0 * sB);
+          double* Cd = reinterpret_cast<double*>(C + (long)b0 * sC);
+          const double da = (double)alpha, db = (double)beta;
+#define JUES_GEMM64(K_, TB_)                                                                                      \
+  hipLaunchKernelGGL((gemm64_kernel<K_, TB_>), grid, dim3(512), 0, ctx->stream, (int)M, (int)N, (int)K16, da, Ad, lda, \
+                     sA, Bd, ldb, sB, db, Cd, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
+          if (tb) JUES_GEMM64(0, true);
+          else if (kind == 1) JUES_GEMM64(1, false);
+          else if (kind == 2) JUES_GEMM64(2, false);
+          else JUES_GEMM64(0, false);
+#undef JUES_GEMM64
+          JUES_HIP(hipGetLastError());
+        }
       }
       if (K > K16)
         gemm_impl<T>(ctx, false, tb, M, N, K - K16, alpha, A + K16 * lda, lda, sA, tb ? B + K16 * ldb : B + K16, ldb,

@@ -112,17 +112,19 @@ def test_rmp2(cfg1, dense):
     assert abs(E32 - ref) < 1e-5 * abs(ref)
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-14), (1, 2e-6)])
 @pytest.mark.parametrize("tb", ["N", "T"])
 @pytest.mark.parametrize("M,N,K", [(128, 128, 16), (129, 131, 33), (300, 257, 1000), (1000, 1000, 32),
                                    (256, 384, 48), (5, 7, 3), (1, 1, 16), (2048, 640, 1040), (40, 9000, 256),
                                    (9000, 40, 256)])
-def test_gemm_fp32_nn_nt(ctx, M, N, K, tb):
-    """FP32 products with A m-contiguous and B k- or n-contiguous: the four-workgroup kernel on
-    16*floor(K/16) columns, edge tiles with clamped rows, the K-tail rank update, skinny shapes."""
+def test_gemm_nn_nt_occupancy_kernels(ctx, M, N, K, tb, prec, tol):
+    """Products with A m-contiguous and B k- or n-contiguous on the high-occupancy kernels
+    (FP32: four 256-thread workgroups per CU; FP64: two 512-thread ones): 16*floor(K/16)
+    columns, edge tiles with clamped rows, the K-tail rank update, skinny shapes."""
     rng = np.random.default_rng(M + N + K)
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((K, N)) if tb == "N" else rng.standard_normal((N, K))
     C0 = rng.standard_normal((M, N))
     ref = 0.7 * (A @ (B if tb == "N" else B.T)) - 0.3 * C0
-    out = ctx.gemm_test(1, "N", tb, 0.7, np.asfortranarray(A), np.asfortranarray(B), -0.3, np.asfortranarray(C0))
-    assert rel_err(out, ref) < 2e-6 * max(1.0, np.sqrt(K) / 8)
+    out = ctx.gemm_test(prec, "N", tb, 0.7, np.asfortranarray(A), np.asfortranarray(B), -0.3, np.asfortranarray(C0))
+    assert rel_err(out, ref) < tol * max(1.0, np.sqrt(K) / 8)
