@@ -266,8 +266,8 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
 // k = 4 (l / 16) + s at step s = 0..3 (the same permutation for both operands): one
 // 16-byte read per 16-row block.  B's four fragment blocks are held for the slice and A's
 // are streamed one block ahead of the MFMAs that use them.  Rows of the last tile row or
-// column beyond M or N load a clamped row and are not stored.  K must be a positive
-// multiple of 16: the host adds a K tail with a separate rank update.
+// column beyond M or N load a clamped row and are not stored.  A K tail (16 not dividing K)
+// is the first slice: zero-padded, loaded element by element once in the prologue.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
@@ -349,13 +349,43 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, 
     for (int j = 0; j < 4; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
 
   const int fr = lane & 15, fk = lane >> 4;
-  const int nk = K / BK;
-  load(0);
+  // slices: the K tail (columns 16 floor(K/16) .. K-1, zero-padded) first when there is one, then the
+  // full slices from k = 0; the tail is loaded once, in the prologue
+  const int K16 = K / BK * BK, tail = K > K16 ? 1 : 0;
+  const int nk = K16 / BK + tail;
+  if (tail) {
+    const float* Ar = A + m0 + (a_voff >> 2);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gk = K16 + 4 * (ac0 + 2 * j) + u;
+        const float val = Ar[(long)min(gk, K - 1) * lda];
+        ra[4 * j + u] = gk < K ? val : 0.f;
+      }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        float val;
+        if constexpr (TB) {
+          const int gk = K16 + 4 * (ac0 + 2 * j) + u;
+          val = gk < K ? B[n0 + (bt_voff >> 2) + (long)min(gk, K - 1) * ldb] : 0.f;
+        } else {
+          const int gk = K16 + 4 * bc + u;
+          const int n = min(n0 + bn + 64 * j, N - 1);
+          val = gk < K ? B[(long)min(gk, K - 1) + (long)n * ldb] : 0.f;
+        }
+        rb[j][u] = val;
+      }
+  } else {
+    load(0);
+  }
   store(0);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+    if (kt + 1 < nk) load((kt + 1 - tail) * BK);
     f4v bf[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -408,7 +438,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, 
 // 16-byte chunk q of row r at chunk q ^ (r & 7); lane l supplies k = 4 (l / 16) + s at MFMA
 // step s = 0..3 for both operands (two 16-byte fragment reads per 16-row block).  B's two
 // blocks are held for the slice and A's four streamed one ahead.  Rows past M or N load a
-// clamped row and are not stored; K must be a positive multiple of 16 (the host adds a tail).
+// clamped row and are not stored; a K tail is the first slice, zero-padded, loaded in the prologue.
 typedef double d2v __attribute__((ext_vector_type(2)));
 
 __device__ inline int g64_chunk(int q, int r) { return q ^ (r & 7); }
@@ -474,13 +504,43 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(int M, int N, int K, dou
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = dvec4{0, 0, 0, 0};
   const int fr = lane & 15, fk = lane >> 4;
-  const int nk = K / BK;
-  load(0);
+  // the K tail first (zero-padded, loaded once in the prologue), then the full slices from k = 0
+  const int K16 = K / BK * BK, tail = K > K16 ? 1 : 0;
+  const int nk = K16 / BK + tail;
+  if (tail) {
+    const double* Ar = A + m0 + (a_voff >> 3);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int gk = K16 + 4 * kq + u;
+      const double val = Ar[(long)min(gk, K - 1) * lda];
+      ra[u] = gk < K ? val : 0.0;
+    }
+    if constexpr (TB) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gk = K16 + 4 * kq + u;
+        const double val = B[n0 + (bt_voff >> 3) + (long)min(gk, K - 1) * ldb];
+        rb[u >> 1][u & 1] = gk < K ? val : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int gk = K16 + 2 * bc + u;
+          const int n = min(n0 + bn + 64 * j, N - 1);
+          const double val = B[(long)min(gk, K - 1) + (long)n * ldb];
+          rb[j][u] = gk < K ? val : 0.0;
+        }
+    }
+  } else {
+    load(0);
+  }
   store(0);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+    if (kt + 1 < nk) load((kt + 1 - tail) * BK);
     d2v bf[2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -611,15 +671,12 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
-  // FP32 NN and NT: gemm32_nn_kernel (four workgroups per CU) over the first 16*floor(K/16) columns
-  // of A; a K tail is added afterwards by the general kernel as a rank update with beta = 1
+  // FP32 NN and NT: gemm32_nn_kernel (four workgroups per CU)
   if constexpr (sizeof(T) == 4) if (!ta && shape == 0 && K >= 16) {
     const long span32 = 4L * (128 + 64) * (std::max(lda, ldb) + 1);
-    const long K16 = K / 16 * 16;
     if (span32 < (1L << 31)) {
       {
-      KTimer timer(ctx, name, 2.0 * M * N * K16 * batch,
-                   4.0 * ((double)M * K16 + (double)K16 * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch);
+      KTimer timer(ctx, name, flops, bytes);
       for (int b0 = 0; b0 < batch; b0 += 65535) {
         const int nb = std::min(batch - b0, 65535);
         dim3 grid((unsigned)(tiles_m128(M) * tiles_m128(N)), 1, nb);
@@ -628,7 +685,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
         float* Cf = reinterpret_cast<float*>(C + (long)b0 * sC);
         const float fa = (float)alpha, fb = (float)beta;
 #define JUES_GEMM32(K_, TB_)                                                                                     \
-  hipLaunchKernelGGL((gemm32_nn_kernel<K_, TB_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K16, fa, Af, lda, \
+  hipLaunchKernelGGL((gemm32_nn_kernel<K_, TB_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, fa, Af, lda, \
                      sA, Bf, ldb, sB, fb, Cf, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
         if (tb) JUES_GEMM32(0, true);
         else if (kind == 1) JUES_GEMM32(1, false);
@@ -638,20 +695,15 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
         JUES_HIP(hipGetLastError());
       }
       }
-      if (K > K16)
-        gemm_impl<T>(ctx, false, tb, M, N, K - K16, alpha, A + K16 * lda, lda, sA, tb ? B + K16 * ldb : B + K16, ldb,
-                     sB, T(1), C, ldc, sC, batch, "gemm_ktail", false);
       return;
     }
   }
-  // FP64 NN and NT: gemm64_kernel (two 512-thread workgroups per CU), same K split
+  // FP64 NN and NT: gemm64_kernel (two 512-thread workgroups per CU)
   if constexpr (sizeof(T) == 8) if (!ta && shape == 0 && K >= 16) {
     const long span64 = 8L * (128 + 64) * (std::max(lda, ldb) + 1);
-    const long K16 = K / 16 * 16;
     if (span64 < (1L << 31)) {
       {
-        KTimer timer(ctx, name, 2.0 * M * N * K16 * batch,
-                     8.0 * ((double)M * K16 + (double)K16 * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch);
+        KTimer timer(ctx, name, flops, bytes);
         for (int b0 = 0; b0 < batch; b0 += 65535) {
           const int nb = std::min(batch - b0, 65535);
           dim3 grid((unsigned)(tiles_m128(M) * tiles_m128(N)), 1, nb);
@@ -660,7 +712,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
           double* Cd = reinterpret_cast<double*>(C + (long)b0 * sC);
           const double da = (double)alpha, db = (double)beta;
 #define JUES_GEMM64(K_, TB_)                                                                                      \
-  hipLaunchKernelGGL((gemm64_kernel<K_, TB_>), grid, dim3(512), 0, ctx->stream, (int)M, (int)N, (int)K16, da, Ad, lda, \
+  hipLaunchKernelGGL((gemm64_kernel<K_, TB_>), grid, dim3(512), 0, ctx->stream, (int)M, (int)N, (int)K, da, Ad, lda, \
                      sA, Bd, ldb, sB, db, Cd, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
           if (tb) JUES_GEMM64(0, true);
           else if (kind == 1) JUES_GEMM64(1, false);
@@ -670,9 +722,6 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
           JUES_HIP(hipGetLastError());
         }
       }
-      if (K > K16)
-        gemm_impl<T>(ctx, false, tb, M, N, K - K16, alpha, A + K16 * lda, lda, sA, tb ? B + K16 * ldb : B + K16, ldb,
-                     sB, T(1), C, ldc, sC, batch, "gemm_ktail", false);
       return;
     }
   }

@@ -116,11 +116,11 @@ def test_rmp2(cfg1, dense):
 @pytest.mark.parametrize("tb", ["N", "T"])
 @pytest.mark.parametrize("M,N,K", [(128, 128, 16), (129, 131, 33), (300, 257, 1000), (1000, 1000, 32),
                                    (256, 384, 48), (5, 7, 3), (1, 1, 16), (2048, 640, 1040), (40, 9000, 256),
-                                   (9000, 40, 256)])
+                                   (9000, 40, 256), (257, 300, 17), (640, 520, 4111), (130, 129, 31)])
 def test_gemm_nn_nt_occupancy_kernels(ctx, M, N, K, tb, prec, tol):
     """Products with A m-contiguous and B k- or n-contiguous on the high-occupancy kernels
-    (FP32: four 256-thread workgroups per CU; FP64: two 512-thread ones): 16*floor(K/16)
-    columns, edge tiles with clamped rows, the K-tail rank update, skinny shapes."""
+    (FP32: four 256-thread workgroups per CU; FP64: two 512-thread ones): full 16-column
+    slices, a zero-padded K-tail slice (1..15 columns), edge tiles with clamped rows, skinny shapes."""
     rng = np.random.default_rng(M + N + K)
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((K, N)) if tb == "N" else rng.standard_normal((N, K))
