@@ -17,6 +17,7 @@ struct CCInts {
   DevBuf Ka;      // <mn|ef> at [e,m,n,f]
   DevBuf oooo;    // <mn|ij> at [m,n,i,j]
   DevBuf Vp, Vm;  // packed pp-ladder integrals
+  DevBuf KP, KM;  // packed hole-hole ladder integrals (pair sums / differences of <kl|cd>)
   DevBuf ooov_c;  // (ij|ka) at [i,j,k,a]      (RCCSD)
   DevBuf ovvv_c;  // (ia|bc) at [i,a,b,c]      (RCCSD)
   DevBuf Lvv;     // MO factors L[ae,P] (v^2 x naux) when the ladder is recomputed
@@ -28,7 +29,8 @@ struct CCInts {
 
 template <typename T>
 struct CCWork {
-  DevBuf Tpp, Upp, Tov, Ttov, Uov, Ua, Tp, Tm, WJ, WE, Y, H, Lhh, Wp, Fae, Fmi, Lp, Lm;
+  DevBuf Tpp, Upp, Tov, Ttov, Uov, Ua, Tp, Tm, WJ, WE, Y, H, Wp, Fae, Fmi, Lp, Lm;
+  DevBuf Wpt, Wmt, Tq;  // hole-hole ladder on packed pairs
   DevBuf Xs, Vps, Vms;  // recomputed ladder: (ae|bf) slab and its packed V+/V- rows
   // RCCSD
   DevBuf t1, Fme, X2, Z, Q, n1, U2, Tau;
@@ -38,8 +40,8 @@ struct CCWork {
 // per-iteration operands (re-split on their first use in an iteration); the
 // others hold integral matrices, split once.
 enum SplitSlot {
-  SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT, SP_TPPT, SP_TPP, SP_WP,
-  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPPT, SP_LVVT, SP_LVVTT, SPLIT_SLOTS
+  SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT, SP_TQ, SP_TM, SP_WPT, SP_WMT,
+  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPT, SP_KMT, SP_LVVT, SP_LVVTT, SPLIT_SLOTS
 };
 struct SplitState {
   struct Entry {
@@ -102,10 +104,11 @@ T* bufp_t(DevBuf& b, size_t n) {
 // operand is symmetric; slotA/slotB name the operands for the split-FP16 stage.
 template <typename T>
 void ring_mm(jues_ctx* ctx, CCState& S, T alpha, const T* A, int slotA, const T* B, int slotB, T beta, T* C);
-// Hole-hole ladder: Wp[ij,kl] += sum_ab Kpp[ij,ab] Tpp[kl,ab]; Lhh = Wp^T Tpp
-// (Kpp, Tpp: oo x vv; Wp: oo x oo, holding oooo plus any T1 terms on entry).
+// Hole-hole ladder sum_kl W'[kl,ij] t[kl,ab], W' = Wp + <kl|cd> t[ij,cd], on packed pairs,
+// added into the pp-ladder's Lp / Lm (run after ladder_mm, whose Tp / Tm it reads).
+// Wp: [m,n,i,j], oooo plus any T1 terms.
 template <typename T>
-void hh_mm(jues_ctx* ctx, CCState& S, const T* Kpp, const T* Tpp, T* Wp, T* Lhh);
+void hh_mm(jues_ctx* ctx, CCState& S, const T* Wp, const T* Tp, const T* Tm, T* Lp, T* Lm);
 // Packed pp-ladder Lp/Lm from FP64 amplitudes `amp` (ijab); Tp/Tm are workspaces.
 template <typename T>
 void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm);

@@ -161,6 +161,8 @@ void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv
   permute<double, double>(ctx, Kov, {o, v, o, v}, {0, 2, 1, 3}, bufp<double>(I.Kpp, N2), 1.0, 0.0);
   permute<double, double>(ctx, Kov, {o, v, o, v}, {0, 3, 2, 1}, bufp<double>(I.Ktov, N2), 1.0, 0.0);
   permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {2, 0, 1, 3}, bufp<double>(I.Ka, N2), 1.0, 0.0);
+  pack_khh(ctx, S.P, I.Kpp.as<double>(), o, v, bufp<double>(I.KP, S.P.npo * S.P.npv),
+           bufp<double>(I.KM, std::max(1L, S.P.nmo * S.P.nmv)));
   // energy weights 2<ij|ab> - <ji|ab>
   double* Le = bufp<double>(S.Lener, N2);
   permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {0, 1, 2, 3}, Le, 2.0, 0.0);
@@ -195,7 +197,7 @@ void cc_make_ints32(jues_ctx* ctx, CCState& S) {
   convert_buf<float>(ctx, a.Kpp, b.Kpp);
   convert_buf<float>(ctx, a.Ka, b.Ka);
   convert_buf<float>(ctx, a.oooo, b.oooo);
-  // Vp/Vm are converted by the first FP32 (non-split) ladder product
+  // Vp/Vm and KP/KM are converted by the first FP32 (non-split) ladder products
   convert_buf<float>(ctx, a.ooov_c, b.ooov_c);
   convert_buf<float>(ctx, a.ovvv_c, b.ovvv_c);
   convert_buf<float>(ctx, a.Lvv, b.Lvv);
@@ -379,35 +381,69 @@ void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp
             P.nmo, 0, 1, "cc_pp_ladder");
 }
 
+// With P+ = (pairs k<=l), P- = (pairs k<l) and the pair conventions of Tp / Tm:
+//   W+^T[(ij),(kl)] = W'[kl,ij] + W'[lk,ij] (k<l), W'[kk,ij] (k=l) = O+ + sum_(c<=d) Tp[(ij),(cd)] KP[(kl),(cd)]
+//   W-^T[(ij),(kl)] = W'[kl,ij] - W'[lk,ij]                        = O- + sum_(c<d)  Tm[(ij),(cd)] KM[(kl),(cd)]
+//   Lp[(ij),(ab)] += 1/2 sum_(kl in P+) W+^T[(ij),(kl)] Tq[(kl),(ab)]   (Tq: Tp with (a,a) columns doubled)
+//   Lm[(ij),(ab)] += 1/2 sum_(kl in P-) W-^T[(ij),(kl)] Tm[(kl),(ab)]
+// since W'[kl,ij] = W'[lk,ji] and t[kl,ab] = t[lk,ba]: a quarter of the unpacked products' flops.
 template <typename T>
-void hh_mm(jues_ctx* ctx, CCState& S, const T* Kpp, const T* Tpp, T* Wp, T* Lhh) {
-  const long oo = (long)S.o * S.o, vv = (long)S.v * S.v;
+void hh_mm(jues_ctx* ctx, CCState& S, const T* Wp, const T* Tp, const T* Tm, T* Lp, T* Lm) {
+  const PairTables& P = S.P;
+  const bool minus = P.nmo > 0 && P.nmv > 0;
+  CCWork<T>& W = work<T>(S);
+  T* Wpt = bufp<T>(W.Wpt, P.npo * P.npo);
+  T* Wmt = bufp<T>(W.Wmt, std::max(1L, P.nmo * P.nmo));
+  T* Tq = bufp<T>(W.Tq, P.npo * P.npv);
+  pack_whh<T>(ctx, P, Wp, S.o, Wpt, Wmt);
+  pair_diag2<T>(ctx, P, Tp, Tq);
   if constexpr (sizeof(T) == 4) {
     if (S.split_on) {
-      // Wp += Kpp Tpp^T: both operands enter as planes of their transposes (k = ab)
-      const Split16& kt = split_operand<float>(ctx, S, SP_KPPT, Kpp, vv, oo, true);
-      const Split16& tt = split_operand<float>(ctx, S, SP_TPPT, Tpp, vv, oo, true);
-      gemm16_tn(ctx, oo, oo, 1.0f, kt, tt, 1.0f, Wp, oo, "cc_hh");
-      // Lhh = Wp^T Tpp: k = kl is already contiguous in both
-      const Split16& w = split_operand<float>(ctx, S, SP_WP, Wp, oo, oo);
-      const Split16& t = split_operand<float>(ctx, S, SP_TPP, Tpp, oo, vv);
-      gemm16_tn(ctx, oo, vv, 1.0f, w, t, 0.0f, Lhh, oo, "cc_hh");
+      // the TN kernel's operands are planes of the k-contiguous transposes
+      const CCInts<double>& I64 = *S.i64;
+      const Split16& tp = split_operand<float>(ctx, S, SP_TPT, Tp, P.npv, P.npo, true);
+      const Split16& kp = split_operand<double>(ctx, S, SP_KPT, I64.KP.as<double>(), P.npv, P.npo, true);
+      gemm16_tn(ctx, P.npo, P.npo, 1.0f, tp, kp, 1.0f, Wpt, P.npo, "cc_hh");
+      const Split16& wp = split_operand<float>(ctx, S, SP_WPT, Wpt, P.npo, P.npo, true);
+      const Split16& tq = split_operand<float>(ctx, S, SP_TQ, Tq, P.npo, P.npv);
+      gemm16_tn(ctx, P.npo, P.npv, 0.5f, wp, tq, 1.0f, Lp, P.npo, "cc_hh");
+      if (minus) {
+        const Split16& tmt = split_operand<float>(ctx, S, SP_TMT, Tm, P.nmv, P.nmo, true);
+        const Split16& km = split_operand<double>(ctx, S, SP_KMT, I64.KM.as<double>(), P.nmv, P.nmo, true);
+        gemm16_tn(ctx, P.nmo, P.nmo, 1.0f, tmt, km, 1.0f, Wmt, P.nmo, "cc_hh");
+        const Split16& wm = split_operand<float>(ctx, S, SP_WMT, Wmt, P.nmo, P.nmo, true);
+        const Split16& tm = split_operand<float>(ctx, S, SP_TM, Tm, P.nmo, P.nmv);
+        gemm16_tn(ctx, P.nmo, P.nmv, 0.5f, wm, tm, 1.0f, Lm, P.nmo, "cc_hh");
+      }
       return;
     }
+    CCInts<float>& b = *S.i32;
+    if (!b.KP.p) {
+      convert_buf<float>(ctx, S.i64->KP, b.KP);
+      convert_buf<float>(ctx, S.i64->KM, b.KM);
+    }
   }
-  gemm<T>(ctx, false, true, oo, oo, vv, T(1), Kpp, oo, 0, Tpp, oo, 0, T(1), Wp, oo, 0, 1, "cc_hh");
-  gemm<T>(ctx, true, false, oo, vv, oo, T(1), Wp, oo, 0, Tpp, oo, 0, T(0), Lhh, oo, 0, 1, "cc_hh");
+  CCInts<T>& I = ints<T>(S);
+  gemm<T>(ctx, false, true, P.npo, P.npo, P.npv, T(1), Tp, P.npo, 0, I.KP.template as<T>(), P.npo, 0, T(1), Wpt,
+          P.npo, 0, 1, "cc_hh");
+  gemm<T>(ctx, false, false, P.npo, P.npv, P.npo, T(0.5), Wpt, P.npo, 0, Tq, P.npo, 0, T(1), Lp, P.npo, 0, 1, "cc_hh");
+  if (minus) {
+    gemm<T>(ctx, false, true, P.nmo, P.nmo, P.nmv, T(1), Tm, P.nmo, 0, I.KM.template as<T>(), P.nmo, 0, T(1), Wmt,
+            P.nmo, 0, 1, "cc_hh");
+    gemm<T>(ctx, false, false, P.nmo, P.nmv, P.nmo, T(0.5), Wmt, P.nmo, 0, Tm, P.nmo, 0, T(1), Lm, P.nmo, 0, 1,
+            "cc_hh");
+  }
 }
 
-template void hh_mm<double>(jues_ctx*, CCState&, const double*, const double*, double*, double*);
-template void hh_mm<float>(jues_ctx*, CCState&, const float*, const float*, float*, float*);
+template void hh_mm<double>(jues_ctx*, CCState&, const double*, const double*, const double*, double*, double*);
+template void hh_mm<float>(jues_ctx*, CCState&, const float*, const float*, const float*, float*, float*);
 
 template void ladder_mm<double>(jues_ctx*, CCState&, const double*, double*, double*, double*, double*);
 template void ladder_mm<float>(jues_ctx*, CCState&, const double*, float*, float*, float*, float*);
 
 // Doubles-residual contractions that depend only on the doubles amplitudes `tau2`
 // (t2 for RCCD; the CCSD driver passes its own amplitudes and adds T1 terms).
-// Produces W.S (ov, symmetrised), W.Lhh (ijab), W.Lp / W.Lm (packed).
+// Produces W.S (ov, symmetrised) and W.Lp / W.Lm (both ladders, packed).
 template <typename T>
 void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   const int o = S.o, v = S.v;
@@ -425,7 +461,6 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   T* WE = bufp<T>(W.WE, N2);
   T* Y = bufp<T>(W.Y, N2);
   T* H = bufp<T>(W.H, N2);
-  T* Lhh = bufp<T>(W.Lhh, N2);
   T* Wp = bufp<T>(W.Wp, oo * oo);
   T* Fae = bufp<T>(W.Fae, vv);
   T* Fmi = bufp<T>(W.Fmi, oo);
@@ -447,11 +482,11 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   // Fae[a,e] = -sum U[a,(mnf)] K[e,(mnf)];  Fmi[m,i] = sum K[m,(nef)] U[i,(nef)]
   gemm<T>(ctx, false, true, v, v, oo * v, T(-1), Ua, v, 0, I.Ka.template as<T>(), v, 0, zero, Fae, v, 0, 1, "cc_fock");
   gemm<T>(ctx, false, true, o, o, ov * v, one, I.Kpp.template as<T>(), o, 0, Upp, o, 0, zero, Fmi, o, 0, 1, "cc_fock");
-  // hole-hole ladder: W' = oooo + K t^T ; Lhh = W'^T t
-  JUES_HIP(hipMemcpyAsync(Wp, I.oooo.p, oo * oo * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  hh_mm<T>(ctx, S, I.Kpp.template as<T>(), Tpp, Wp, Lhh);
-  // particle-particle ladder on the packed (anti)symmetric parts
+  // particle-particle ladder on the packed (anti)symmetric parts, then the hole-hole
+  // ladder (W' = oooo + K t^T) added into the same packed sums
   ladder_mm<T>(ctx, S, t2, Tp, Tm, Lp, Lm);
+  JUES_HIP(hipMemcpyAsync(Wp, I.oooo.p, oo * oo * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  hh_mm<T>(ctx, S, Wp, Tp, Tm, Lp, Lm);
   // ring terms (every left operand is a symmetric (ov) x (ov) matrix)
   const T* K = I.Kov.template as<T>();
   const T* Kx = I.Ktov.template as<T>();
@@ -530,7 +565,6 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
   A.Kpp = S.i64->Kpp.as<double>();
   A.Lener = S.Lener.as<double>();
   A.S = W.WJ.template as<T>();
-  A.Lhh = W.Lhh.template as<T>();
   A.Lp = W.Lp.template as<T>();
   A.Lm = W.Lm.template as<T>();
   A.extra = S.method == 1 ? W.X2.template as<T>() : nullptr;

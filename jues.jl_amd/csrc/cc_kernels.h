@@ -26,9 +26,8 @@ struct UpdateArgs {
   const double* Kpp;    // <ij|ab> (FP64, ijab)
   const double* Lener;  // 2<ij|ab> - <ji|ab> (FP64, ijab)
   const TR* S;          // ring + Fock-like terms, symmetrised, ov layout [i,a,j,b]
-  const TR* Lhh;        // hole-hole ladder (ijab)
-  const TR* Lp;         // packed pp-ladder, symmetric part
-  const TR* Lm;         // packed pp-ladder, antisymmetric part
+  const TR* Lp;         // packed ladders (pp + hh), symmetric part
+  const TR* Lm;         // packed ladders (pp + hh), antisymmetric part
   const TR* extra;      // any further ijab-ordered residual terms (RCCSD), may be null
   const double* t1n;    // new T1 (o x v) for the CCSD energy term, null for RCCD
   double lambda;        // coupling: t_new = lambda R / D, E = lambda sum Lener tau
@@ -70,6 +69,14 @@ void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T
 template <typename Ts, typename Td>
 void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm,
             bool slab = false);
+// hole-hole ladder integrals on pairs: KP[(k<=l),(c<=d)], KM[(k<l),(c<d)] from K[k,l,c,d] = <kl|cd>
+void pack_khh(jues_ctx* ctx, const PairTables& P, const double* K, int o, int v, double* KP, double* KM);
+// W[m,n,i,j] -> WPT[(i<=j),(k<=l)], WMT[(i<j),(k<l)] (the kl-pair sum / difference, transposed)
+template <typename T>
+void pack_whh(jues_ctx* ctx, const PairTables& P, const T* W, int o, T* WPT, T* WMT);
+// Tq = Tp with its (a,a) columns doubled
+template <typename T>
+void pair_diag2(jues_ctx* ctx, const PairTables& P, const T* Tp, T* Tq);
 // slab-local V+/V- rows (pairs with f in [f0,f1)) from X[e,a,b,f-f0] = (ae|bf)
 template <typename T>
 void pack_v_ea(jues_ctx* ctx, const PairTables& P, const T* X, int v, int f0, int f1, T* Vp, T* Vm);

@@ -13,7 +13,10 @@ __device__ inline long tri_lt(int p, int q) { return (long)p + (long)q * (q - 1)
 
 // Tp[(i<=j),(e<=f)] = t[ijef] + t[ijfe] (e<f), t[ijee] (e=f)
 // Tm[(i<j),(e<f)]   = t[ijef] - t[ijfe]
-template <typename T>
+// With ROWDIAG the single term is taken on the row diagonal instead: for the hole-hole
+// ladder integrals K[kl,cd] = <kl|cd>, KP[(k<=l),(c<=d)] = K[klcd] + K[kldc] (k<l),
+// K[kkcd] (k=l) and KM[(k<l),(c<d)] = K[klcd] - K[kldc].
+template <typename T, bool ROWDIAG = false>
 __global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ t, int o, int v,
                                                      const int2* __restrict__ pairs_o_le, const int2* __restrict__ pairs_v_le,
                                                      long npo, long npv, T* __restrict__ Tp,
@@ -28,7 +31,7 @@ __global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ 
       const long base = ij.x + (long)o * ij.y;
       const double a = t[base + oo * (ef.x + (long)v * ef.y)];
       const double b = t[base + oo * (ef.y + (long)v * ef.x)];
-      Tp[x] = (T)(ef.x == ef.y ? a : a + b);
+      Tp[x] = (T)((ROWDIAG ? ij.x == ij.y : ef.x == ef.y) ? a : a + b);
     } else {
       const long y = x - tot_p;
       const long r = y % nmo, c = y / nmo;
@@ -77,6 +80,38 @@ __global__ __launch_bounds__(256) void pack_v_kernel(const Ts* __restrict__ X, i
   }
 }
 
+// W[m,n,i,j] (o^4, holding <mn|ij> plus any T1 terms) on packed pairs, transposed:
+// WPT[(i<=j),(k<=l)] = W[klij] + W[lkij] (k<l), W[kkij] (k=l);  WMT[(i<j),(k<l)] = W[klij] - W[lkij]
+template <typename T>
+__global__ __launch_bounds__(256) void pack_whh_kernel(const T* __restrict__ W, int o, const int2* __restrict__ o_le,
+                                                       long npo, T* __restrict__ WPT, const int2* __restrict__ o_lt,
+                                                       long nmo, T* __restrict__ WMT) {
+  const long oo = (long)o * o;
+  const long tot_p = npo * npo, tot_m = nmo * nmo;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < tot_p + tot_m; x += (long)gridDim.x * 256L) {
+    const bool plus = x < tot_p;
+    const long y = plus ? x : x - tot_p, n = plus ? npo : nmo;
+    const long r = y % n, c = y / n;
+    const int2 ij = plus ? o_le[r] : o_lt[r], kl = plus ? o_le[c] : o_lt[c];
+    const long col = oo * (ij.x + (long)o * ij.y);
+    const double a = (double)W[kl.x + (long)o * kl.y + col];
+    const double b = (double)W[kl.y + (long)o * kl.x + col];
+    if (plus) WPT[y] = (T)(kl.x == kl.y ? a : a + b);
+    else WMT[y] = (T)(a - b);
+  }
+}
+
+// Tq = Tp with its (a,a) columns doubled (exact)
+template <typename T>
+__global__ __launch_bounds__(256) void pair_diag2_kernel(const T* __restrict__ Tp, long npo, long npv,
+                                                         const int2* __restrict__ v_le, T* __restrict__ Tq) {
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < npo * npv; x += (long)gridDim.x * 256L) {
+    const int2 ab = v_le[x / npo];
+    const T t = Tp[x];
+    Tq[x] = ab.x == ab.y ? t + t : t;
+  }
+}
+
 __device__ inline double block_sum256(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   __syncthreads();
@@ -85,7 +120,7 @@ __device__ inline double block_sum256(double v, double* sh) {
   return (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-// One block per (a,b): assemble R[ijab] = K[ijab] + S[ia,jb] + Lhh[ijab] + L+ + L-,
+// One block per (a,b): assemble R[ijab] = K[ijab] + S[ia,jb] + L+ + L- (both ladders),
 // t_new = R/D, e = t_new - t_old; write t_new and e into the DIIS slot; block
 // partial sums of E = sum Lener*t_new, e.e and e.e_k for the stored errors.
 template <typename TR>
@@ -108,7 +143,6 @@ __global__ __launch_bounds__(256) void t2_update_kernel(UpdateArgs<TR> A) {
     const long x = (long)ij + oo * ab;
     double R = A.Kpp[x];
     if (A.S) R += (double)A.S[(long)i + (long)o * a + ov * ((long)j + (long)o * b)];
-    if (A.Lhh) R += (double)A.Lhh[x];
     if (A.Lp) {
       const long pp = i <= j ? (long)i + (long)j * (j + 1) / 2 : (long)j + (long)i * (i + 1) / 2;
       R += (double)A.Lp[pp + A.npo * qp];
@@ -325,6 +359,32 @@ void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T
   JUES_HIP(hipGetLastError());
 }
 
+void pack_khh(jues_ctx* ctx, const PairTables& P, const double* K, int o, int v, double* KP, double* KM) {
+  const long tot = P.npo * P.npv + P.nmo * P.nmv;
+  KTimer k(ctx, "cc_pack_k", 0.0, (double)tot * 24.0);
+  hipLaunchKernelGGL((pack_t_kernel<double, true>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, K, o, v, P.o_le,
+                     P.v_le, P.npo, P.npv, KP, P.o_lt, P.v_lt, P.nmo, P.nmv, KM);
+  JUES_HIP(hipGetLastError());
+}
+
+template <typename T>
+void pack_whh(jues_ctx* ctx, const PairTables& P, const T* W, int o, T* WPT, T* WMT) {
+  const long tot = P.npo * P.npo + P.nmo * P.nmo;
+  KTimer k(ctx, "cc_hh_pack", 0.0, (double)tot * 3.0 * sizeof(T));
+  hipLaunchKernelGGL((pack_whh_kernel<T>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, W, o, P.o_le, P.npo, WPT,
+                     P.o_lt, P.nmo, WMT);
+  JUES_HIP(hipGetLastError());
+}
+
+template <typename T>
+void pair_diag2(jues_ctx* ctx, const PairTables& P, const T* Tp, T* Tq) {
+  const long tot = P.npo * P.npv;
+  KTimer k(ctx, "cc_hh_pack", 0.0, (double)tot * 2.0 * sizeof(T));
+  hipLaunchKernelGGL((pair_diag2_kernel<T>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, Tp, P.npo, P.npv, P.v_le,
+                     Tq);
+  JUES_HIP(hipGetLastError());
+}
+
 template <typename Ts, typename Td>
 void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm, bool slab) {
   const long r0p = (long)f0 * (f0 + 1) / 2, r0m = (long)f0 * (f0 - 1) / 2;
@@ -375,6 +435,10 @@ void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v, double
   JUES_HIP(hipGetLastError());
 }
 
+template void pack_whh<double>(jues_ctx*, const PairTables&, const double*, int, double*, double*);
+template void pack_whh<float>(jues_ctx*, const PairTables&, const float*, int, float*, float*);
+template void pair_diag2<double>(jues_ctx*, const PairTables&, const double*, double*);
+template void pair_diag2<float>(jues_ctx*, const PairTables&, const float*, float*);
 template void pack_t<double>(jues_ctx*, const PairTables&, const double*, int, int, double*, double*);
 template void pack_t<float>(jues_ctx*, const PairTables&, const double*, int, int, float*, float*);
 template <typename T>

@@ -85,7 +85,6 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   T* WE = bufp_t<T>(W.WE, N2);
   T* Y = bufp_t<T>(W.Y, N2);
   T* H = bufp_t<T>(W.H, N2);
-  T* Lhh = bufp_t<T>(W.Lhh, N2);
   T* Wp = bufp_t<T>(W.Wp, oo * oo);
   T* Fae = bufp_t<T>(W.Fae, vv);
   T* Fmi = bufp_t<T>(W.Fmi, oo);
@@ -166,14 +165,13 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   contract<T>(ctx, -0.5, TN(t1, "mb"), TN(Fme, "me"), 1.0, TN(Fae, "be"), "cc_t1");
   contract<T>(ctx, 0.5, TN(t1, "je"), TN(Fme, "me"), 1.0, TN(Fmi, "mj"), "cc_t1");
 
-  // ---- hole-hole ladder: W' = oooo + T1 terms + <mn|ef> tau[ij,ef]; Lhh = W'^T tau ----
+  // ---- particle-particle ladder on tau ------------------------------------------------
+  ladder_mm<T>(ctx, S, tau, Tp, Tm, Lp, Lm);
+  // ---- hole-hole ladder: W' = oooo + T1 terms + <mn|ef> tau[ij,ef], added into Lp / Lm ----
   JUES_HIP(hipMemcpyAsync(Wp, I.oooo.p, oo * oo * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
   contract<T>(ctx, 1.0, TN(t1, "je"), TN(ooov, "mine"), 1.0, TN(Wp, "mnij"), "cc_t1");
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(ooov, "njme"), 1.0, TN(Wp, "mnij"), "cc_t1");
-  hh_mm<T>(ctx, S, I.Kpp.template as<T>(), Tpp, Wp, Lhh);
-
-  // ---- particle-particle ladder on tau ------------------------------------------------
-  ladder_mm<T>(ctx, S, tau, Tp, Tm, Lp, Lm);
+  hh_mm<T>(ctx, S, Wp, Tp, Tm, Lp, Lm);
   // Wabef's T1 part: X2 = Lz + Lz^P,  Lz[ijab] = -sum_m t1[mb] Z[ij,ma],  Z = tau . (mf|ae)
   contract<T>(ctx, 1.0, TN(Tpp, "ijef"), TN(ovvv, "mfae"), 0.0, TN(Z, "ijma"), "cc_ovvv");
   contract<T>(ctx, -1.0, TN(t1, "mb"), TN(Z, "ijma"), 0.0, TN(Q, "ijab"), "cc_t1");
