@@ -261,6 +261,15 @@ jues_status jues_gemm_test(jues_ctx* ctx, int prec, char transa, char transb, in
                            int K, double alpha, const double* A, int lda, const double* B,
                            int ldb, double beta, double* C, int ldc);
 
+/* Two independent products C_i = alpha*A_i*op(B_i) + beta*C_i (A_i not transposed, op the
+ * same for both) in one launch of the MFMA GEMM kernel, as the coupled-cluster ladders use
+ * it; device copies of host arrays, FP64 or FP32 (prec).  Exposed for the GEMM unit tests. */
+jues_status jues_gemm_pair_test(jues_ctx* ctx, int prec, char transb, int M0, int N0, int K0,
+                                const double* A0, int lda0, const double* B0, int ldb0, double* C0,
+                                int ldc0, int M1, int N1, int K1, const double* A1, int lda1,
+                                const double* B1, int ldb1, double* C1, int ldc1, double alpha,
+                                double beta);
+
 #ifdef __cplusplus
 }
 #endif

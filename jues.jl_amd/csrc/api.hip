@@ -243,6 +243,36 @@ jues_status jues_gemm_test(jues_ctx* ctx, int prec, char transa, char transb, in
   JUES_API_END(ctx)
 }
 
+jues_status jues_gemm_pair_test(jues_ctx* ctx, int prec, char transb, int M0, int N0, int K0, const double* A0,
+                                int lda0, const double* B0, int ldb0, double* C0, int ldc0, int M1, int N1, int K1,
+                                const double* A1, int lda1, const double* B1, int ldb1, double* C1, int ldc1,
+                                double alpha, double beta) {
+  JUES_API_BEGIN(ctx)
+  const bool tb = transb == 'T' || transb == 't';
+  JUES_REQUIRE(M0 >= 0 && N0 >= 0 && K0 >= 0 && M1 >= 0 && N1 >= 0 && K1 >= 0 && A0 && B0 && C0 && A1 && B1 && C1,
+               "jues_gemm_pair_test: bad arguments");
+  JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED, "jues_gemm_pair_test: FP64 or FP32 only");
+  const size_t na0 = (size_t)lda0 * K0, nb0 = (size_t)ldb0 * (tb ? K0 : N0), nc0 = (size_t)ldc0 * N0;
+  const size_t na1 = (size_t)lda1 * K1, nb1 = (size_t)ldb1 * (tb ? K1 : N1), nc1 = (size_t)ldc1 * N1;
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    DeviceArray<T> a0(ctx, A0, na0), b0(ctx, B0, nb0), c0(ctx, C0, nc0);
+    DeviceArray<T> a1(ctx, A1, na1), b1(ctx, B1, nb1), c1(ctx, C1, nc1);
+    gemm_pair<T>(ctx, tb, M0, N0, K0, a0.p, lda0, b0.p, ldb0, c0.p, ldc0, M1, N1, K1, a1.p, lda1, b1.p, ldb1, c1.p,
+                 ldc1, (T)alpha, (T)beta);
+    for (int q = 0; q < 2; ++q) {
+      const T* c = q ? c1.p : c0.p;
+      const size_t n = q ? nc1 : nc0;
+      DevBuf c64(std::max<size_t>(1, n) * sizeof(double));
+      convert<T, double>(ctx, c, c64.as<double>(), (long)n);
+      d2h(ctx, q ? C1 : C0, c64.p, n * sizeof(double));
+    }
+  };
+  if (prec == JUES_FP64) run(0.0);
+  else run(0.0f);
+  JUES_API_END(ctx)
+}
+
 int jues_kernel_stats(jues_ctx* ctx, const char** names, double* ms, double* calls, double* flops, double* bytes,
                       int cap) {
   if (!ctx) return 0;

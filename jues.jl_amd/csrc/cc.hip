@@ -374,6 +374,7 @@ void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp
   }
   CCInts<T>& I = ints<T>(S);
   pack_t<T>(ctx, P, amp, S.o, S.v, Tp, Tm);
+  // (one launch per product: at over six rounds of workgroups each, a shared launch measured no faster)
   gemm<T>(ctx, false, false, P.npo, P.npv, P.npv, T(1), Tp, P.npo, 0, I.Vp.template as<T>(), P.npv, 0, T(0), Lp,
           P.npo, 0, 1, "cc_pp_ladder");
   if (minus)
@@ -424,13 +425,17 @@ void hh_mm(jues_ctx* ctx, CCState& S, const T* Wp, const T* Tp, const T* Tm, T* 
     }
   }
   CCInts<T>& I = ints<T>(S);
-  gemm<T>(ctx, false, true, P.npo, P.npo, P.npv, T(1), Tp, P.npo, 0, I.KP.template as<T>(), P.npo, 0, T(1), Wpt,
-          P.npo, 0, 1, "cc_hh");
-  gemm<T>(ctx, false, false, P.npo, P.npv, P.npo, T(0.5), Wpt, P.npo, 0, Tq, P.npo, 0, T(1), Lp, P.npo, 0, 1, "cc_hh");
+  // the plus and minus products share each launch: the W' builds have fewer tiles than one
+  // round of resident workgroups
   if (minus) {
-    gemm<T>(ctx, false, true, P.nmo, P.nmo, P.nmv, T(1), Tm, P.nmo, 0, I.KM.template as<T>(), P.nmo, 0, T(1), Wmt,
-            P.nmo, 0, 1, "cc_hh");
-    gemm<T>(ctx, false, false, P.nmo, P.nmv, P.nmo, T(0.5), Wmt, P.nmo, 0, Tm, P.nmo, 0, T(1), Lm, P.nmo, 0, 1,
+    gemm_pair<T>(ctx, true, P.npo, P.npo, P.npv, Tp, P.npo, I.KP.template as<T>(), P.npo, Wpt, P.npo, P.nmo, P.nmo,
+                 P.nmv, Tm, P.nmo, I.KM.template as<T>(), P.nmo, Wmt, P.nmo, T(1), T(1), "cc_hh");
+    gemm_pair<T>(ctx, false, P.npo, P.npv, P.npo, Wpt, P.npo, Tq, P.npo, Lp, P.npo, P.nmo, P.nmv, P.nmo, Wmt, P.nmo,
+                 Tm, P.nmo, Lm, P.nmo, T(0.5), T(1), "cc_hh");
+  } else {
+    gemm<T>(ctx, false, true, P.npo, P.npo, P.npv, T(1), Tp, P.npo, 0, I.KP.template as<T>(), P.npo, 0, T(1), Wpt,
+            P.npo, 0, 1, "cc_hh");
+    gemm<T>(ctx, false, false, P.npo, P.npv, P.npo, T(0.5), Wpt, P.npo, 0, Tq, P.npo, 0, T(1), Lp, P.npo, 0, 1,
             "cc_hh");
   }
 }

@@ -78,6 +78,37 @@ __device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, i
   tn = r / gsz;
 }
 
+// As tile_coords over nwg workgroup ids of which only the first tiles_m * tiles_n carry a
+// tile (a launch padded to a multiple of NXCD); false for the padding ids.
+__device__ inline bool tile_coords_bounded(int pid, int nwg, int tiles_m, int tiles_n, int& tm, int& tn) {
+  const int per = (nwg + NXCD - 1) / NXCD;
+  int tall = nwg % NXCD;
+  if (tall == 0) tall = NXCD;
+  const int xcd = pid % NXCD, loc = pid / NXCD;
+  const int p2 = xcd < tall ? xcd * per + loc : tall * per + (xcd - tall) * (per - 1) + loc;
+  if (p2 >= tiles_m * tiles_n) return false;
+  const int group_size = GROUP_M * tiles_n;
+  const int gid = p2 / group_size;
+  const int first_m = gid * GROUP_M;
+  const int gsz = min(tiles_m - first_m, GROUP_M);
+  const int r = p2 - gid * group_size;
+  tm = first_m + r % gsz;
+  tn = r / gsz;
+  return true;
+}
+
+// One product of a high-occupancy kernel launch: C = alpha A op(B) + beta C, batched over
+// blockIdx.z with strides sA, sB, sC.  A launch carries one or two of them (two
+// independent products sharing the grid, so that neither leaves a partly idle last round).
+template <typename T>
+struct GemmSide {
+  const T* A;
+  const T* B;
+  T* C;
+  long lda, ldb, ldc, sA, sB, sC;
+  int M, N, K, tiles_m, tiles_n;
+};
+
 // Shared-memory layout of one operand slice.  When the operand's contiguous
 // global dimension is the row index (m, or n for B) the slice is stored k-major,
 // [k][m], padded so that the four k rows an MFMA fragment read touches start 16
@@ -272,22 +303,27 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
 
+// Workgroups [0, nt0) run product p0 (nt0 padded up to a multiple of NXCD when a second
+// product follows), the rest p1.
 template <int KIND, bool TB>
-__global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(int M, int N, int K, float alpha,
-                                                           const float* __restrict__ A, long lda, long sA,
-                                                           const float* __restrict__ B, long ldb, long sB,
-                                                           float beta, float* __restrict__ C, long ldc, long sC,
-                                                           int tiles_m, int tiles_n) {
+__global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, GemmSide<float> p1, int nt0,
+                                                           float alpha, float beta) {
   constexpr int BM = 128, BN = 128, BK = 16;
   __shared__ __attribute__((aligned(16))) float As[2][BM * BK];
   __shared__ __attribute__((aligned(16))) float Bs[2][BN * BK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  A += (long)blockIdx.z * sA;
-  B += (long)blockIdx.z * sB;
-  C += (long)blockIdx.z * sC;
+  const bool second = (int)blockIdx.x >= nt0;
+  const GemmSide<float> g = second ? p1 : p0;
+  const int M = g.M, N = g.N, K = g.K;
+  const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+  const float* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
+  const float* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
+  float* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
   int tm, tn;
-  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  if (!tile_coords_bounded(second ? (int)blockIdx.x - nt0 : (int)blockIdx.x, second ? (int)gridDim.x - nt0 : nt0,
+                           g.tiles_m, g.tiles_n, tm, tn))
+    return;
   const int m0 = tm * BM, n0 = tn * BN;
 
   // A share: row am, 4-k chunks ac0 and ac0 + 2; B share: rows bn and bn + 64, chunk bc
@@ -444,21 +480,24 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 __device__ inline int g64_chunk(int q, int r) { return q ^ (r & 7); }
 
 template <int KIND, bool TB>
-__global__ __launch_bounds__(512, 4) void gemm64_kernel(int M, int N, int K, double alpha,
-                                                        const double* __restrict__ A, long lda, long sA,
-                                                        const double* __restrict__ B, long ldb, long sB,
-                                                        double beta, double* __restrict__ C, long ldc, long sC,
-                                                        int tiles_m, int tiles_n) {
+__global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, GemmSide<double> p1, int nt0,
+                                                        double alpha, double beta) {
   constexpr int BM = 128, BN = 128, BK = 16;
   __shared__ __attribute__((aligned(16))) double As[2][BM * BK];
   __shared__ __attribute__((aligned(16))) double Bs[2][BN * BK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 wavefronts of 64 x 32
-  A += (long)blockIdx.z * sA;
-  B += (long)blockIdx.z * sB;
-  C += (long)blockIdx.z * sC;
+  const bool second = (int)blockIdx.x >= nt0;
+  const GemmSide<double> g = second ? p1 : p0;
+  const int M = g.M, N = g.N, K = g.K;
+  const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+  const double* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
+  const double* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
+  double* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
   int tm, tn;
-  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  if (!tile_coords_bounded(second ? (int)blockIdx.x - nt0 : (int)blockIdx.x, second ? (int)gridDim.x - nt0 : nt0,
+                           g.tiles_m, g.tiles_n, tm, tn))
+    return;
   const int m0 = tm * BM, n0 = tn * BN;
   // A (and n-contiguous B) share: row am, k = 4 kq + u; k-contiguous B share: rows bn, bn + 64, chunk bc
   const int am = tid & 127, kq = __builtin_amdgcn_readfirstlane(tid >> 7);
@@ -622,6 +661,52 @@ __global__ void splitk_reduce_kernel(const T* __restrict__ W, int S, long M, lon
 
 static inline long tiles_m128(long x) { return (x + 127) / 128; }
 
+// Whether a product can run on the high-occupancy NN / NT kernels: A m-contiguous, 128x128
+// tiles, at least one full k-slice, and a tile's k-slice addressable with 32-bit offsets.
+template <typename T>
+static bool occ_kernel_ok(bool ta, long K, long lda, long ldb) {
+  return !ta && K >= 16 && (long)sizeof(T) * (128 + 64) * (std::max(lda, ldb) + 1) < (1L << 31);
+}
+
+template <typename T>
+static GemmSide<T> make_side(long M, long N, long K, const T* A, long lda, long sA, const T* B, long ldb, long sB,
+                             T* C, long ldc, long sC) {
+  return GemmSide<T>{A, B, C, lda, ldb, ldc, sA, sB, sC, (int)M, (int)N, (int)K, (int)tiles_m128(M),
+                     (int)tiles_m128(N)};
+}
+
+// One launch of the high-occupancy kernel over p0 (and p1 when `two`), nb batch entries.
+template <typename T>
+static void occ_launch(jues_ctx* ctx, bool tb, int kind, GemmSide<T> p0, GemmSide<T> p1, bool two, int nb, T alpha,
+                       T beta) {
+  const long t0 = (long)p0.tiles_m * p0.tiles_n;
+  const long nt0 = two ? (t0 + NXCD - 1) / NXCD * NXCD : t0;
+  const long total = two ? nt0 + (long)p1.tiles_m * p1.tiles_n : t0;
+  JUES_REQUIRE(total < (1L << 31), "gemm: too many tiles");
+  if (!two) p1 = p0;
+  const dim3 grid((unsigned)total, 1, nb);
+  if constexpr (sizeof(T) == 4) {
+#define JUES_GEMM32(K_, TB_)                                                                                 \
+  hipLaunchKernelGGL((gemm32_nn_kernel<K_, TB_>), grid, dim3(NT), 0, ctx->stream, p0, p1, (int)nt0, (float)alpha, \
+                     (float)beta)
+    if (tb) JUES_GEMM32(0, true);
+    else if (kind == 1) JUES_GEMM32(1, false);
+    else if (kind == 2) JUES_GEMM32(2, false);
+    else JUES_GEMM32(0, false);
+#undef JUES_GEMM32
+  } else {
+#define JUES_GEMM64(K_, TB_)                                                                               \
+  hipLaunchKernelGGL((gemm64_kernel<K_, TB_>), grid, dim3(512), 0, ctx->stream, p0, p1, (int)nt0, (double)alpha, \
+                     (double)beta)
+    if (tb) JUES_GEMM64(0, true);
+    else if (kind == 1) JUES_GEMM64(1, false);
+    else if (kind == 2) JUES_GEMM64(2, false);
+    else JUES_GEMM64(0, false);
+#undef JUES_GEMM64
+  }
+  JUES_HIP(hipGetLastError());
+}
+
 template <typename T>
 static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda,
                       long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch,
@@ -671,59 +756,17 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
-  // FP32 NN and NT: gemm32_nn_kernel (four workgroups per CU)
-  if constexpr (sizeof(T) == 4) if (!ta && shape == 0 && K >= 16) {
-    const long span32 = 4L * (128 + 64) * (std::max(lda, ldb) + 1);
-    if (span32 < (1L << 31)) {
-      {
-      KTimer timer(ctx, name, flops, bytes);
-      for (int b0 = 0; b0 < batch; b0 += 65535) {
-        const int nb = std::min(batch - b0, 65535);
-        dim3 grid((unsigned)(tiles_m128(M) * tiles_m128(N)), 1, nb);
-        const float* Af = reinterpret_cast<const float*>(A + (long)b0 * sA);
-        const float* Bf = reinterpret_cast<const float*>(B + (long)b0 * sB);
-        float* Cf = reinterpret_cast<float*>(C + (long)b0 * sC);
-        const float fa = (float)alpha, fb = (float)beta;
-#define JUES_GEMM32(K_, TB_)                                                                                     \
-  hipLaunchKernelGGL((gemm32_nn_kernel<K_, TB_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)K, fa, Af, lda, \
-                     sA, Bf, ldb, sB, fb, Cf, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
-        if (tb) JUES_GEMM32(0, true);
-        else if (kind == 1) JUES_GEMM32(1, false);
-        else if (kind == 2) JUES_GEMM32(2, false);
-        else JUES_GEMM32(0, false);
-#undef JUES_GEMM32
-        JUES_HIP(hipGetLastError());
-      }
-      }
-      return;
+  // FP32 NN and NT: gemm32_nn_kernel (four workgroups per CU); FP64: gemm64_kernel (two
+  // 512-thread workgroups per CU)
+  if (shape == 0 && occ_kernel_ok<T>(ta, K, lda, ldb)) {
+    KTimer timer(ctx, name, flops, bytes);
+    for (int b0 = 0; b0 < batch; b0 += 65535) {
+      const int nb = std::min(batch - b0, 65535);
+      const GemmSide<T> p = make_side<T>(M, N, K, A + (long)b0 * sA, lda, sA, B + (long)b0 * sB, ldb, sB,
+                                         C + (long)b0 * sC, ldc, sC);
+      occ_launch<T>(ctx, tb, kind, p, p, false, nb, alpha, beta);
     }
-  }
-  // FP64 NN and NT: gemm64_kernel (two 512-thread workgroups per CU)
-  if constexpr (sizeof(T) == 8) if (!ta && shape == 0 && K >= 16) {
-    const long span64 = 8L * (128 + 64) * (std::max(lda, ldb) + 1);
-    if (span64 < (1L << 31)) {
-      {
-        KTimer timer(ctx, name, flops, bytes);
-        for (int b0 = 0; b0 < batch; b0 += 65535) {
-          const int nb = std::min(batch - b0, 65535);
-          dim3 grid((unsigned)(tiles_m128(M) * tiles_m128(N)), 1, nb);
-          const double* Ad = reinterpret_cast<const double*>(A + (long)b0 * sA);
-          const double* Bd = reinterpret_cast<const double*>(B + (long)b0 * sB);
-          double* Cd = reinterpret_cast<double*>(C + (long)b0 * sC);
-          const double da = (double)alpha, db = (double)beta;
-#define JUES_GEMM64(K_, TB_)                                                                                      \
-  hipLaunchKernelGGL((gemm64_kernel<K_, TB_>), grid, dim3(512), 0, ctx->stream, (int)M, (int)N, (int)K, da, Ad, lda, \
-                     sA, Bd, ldb, sB, db, Cd, ldc, sC, (int)tiles_m128(M), (int)tiles_m128(N))
-          if (tb) JUES_GEMM64(0, true);
-          else if (kind == 1) JUES_GEMM64(1, false);
-          else if (kind == 2) JUES_GEMM64(2, false);
-          else JUES_GEMM64(0, false);
-#undef JUES_GEMM64
-          JUES_HIP(hipGetLastError());
-        }
-      }
-      return;
-    }
+    return;
   }
   KTimer timer(ctx, name, flops, bytes);
   // the buffer-load path addresses a tile's k-slice with 32-bit byte offsets
@@ -762,6 +805,43 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
           const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch, const char* tag) {
   gemm_impl<T>(ctx, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, tag, true);
 }
+
+template <typename T>
+void gemm_pair(jues_ctx* ctx, bool tb, long M0, long N0, long K0, const T* A0, long lda0, const T* B0, long ldb0,
+               T* C0, long ldc0, long M1, long N1, long K1, const T* A1, long lda1, const T* B1, long ldb1, T* C1,
+               long ldc1, T alpha, T beta, const char* tag) {
+  const bool skinny0 = (M0 <= 64 && N0 >= 8192) || (N0 <= 64 && M0 >= 8192);
+  const bool skinny1 = (M1 <= 64 && N1 >= 8192) || (N1 <= 64 && M1 >= 8192);
+  const bool empty0 = M0 <= 0 || N0 <= 0, empty1 = M1 <= 0 || N1 <= 0;
+  if (empty0 || empty1 || skinny0 || skinny1 || !occ_kernel_ok<T>(false, K0, lda0, ldb0) ||
+      !occ_kernel_ok<T>(false, K1, lda1, ldb1)) {
+    gemm<T>(ctx, false, tb, M0, N0, K0, alpha, A0, lda0, 0, B0, ldb0, 0, beta, C0, ldc0, 0, 1, tag);
+    gemm<T>(ctx, false, tb, M1, N1, K1, alpha, A1, lda1, 0, B1, ldb1, 0, beta, C1, ldc1, 0, 1, tag);
+    return;
+  }
+  JUES_REQUIRE(M0 < (1L << 31) && N0 < (1L << 31) && K0 < (1L << 31) && M1 < (1L << 31) && N1 < (1L << 31) &&
+                   K1 < (1L << 31),
+               "gemm: dimension too large");
+  JUES_REQUIRE(ldc0 >= M0 && ldc1 >= M1, "gemm: ldc < M");
+  JUES_REQUIRE(lda0 >= M0 && lda1 >= M1, "gemm: lda too small");
+  JUES_REQUIRE(ldb0 >= (tb ? N0 : K0) && ldb1 >= (tb ? N1 : K1), "gemm: ldb too small");
+  const char* name = tag ? tag : (sizeof(T) == 8 ? "gemm_f64" : "gemm_f32");
+  const int kind = !tag ? 0 : (std::strcmp(tag, "cc_ring") == 0 ? 1 : (std::strcmp(tag, "cc_pp_ladder") == 0 ? 2 : 0));
+  const double sz = sizeof(T), bf = beta != T(0) ? 2.0 : 1.0;
+  const double flops = 2.0 * M0 * N0 * K0 + 2.0 * M1 * N1 * K1;
+  const double bytes = sz * ((double)M0 * K0 + (double)K0 * N0 + bf * M0 * N0 + (double)M1 * K1 + (double)K1 * N1 +
+                             bf * M1 * N1);
+  KTimer timer(ctx, name, flops, bytes);
+  occ_launch<T>(ctx, tb, kind, make_side<T>(M0, N0, K0, A0, lda0, 0, B0, ldb0, 0, C0, ldc0, 0),
+                make_side<T>(M1, N1, K1, A1, lda1, 0, B1, ldb1, 0, C1, ldc1, 0), true, 1, alpha, beta);
+}
+
+template void gemm_pair<double>(jues_ctx*, bool, long, long, long, const double*, long, const double*, long, double*,
+                                long, long, long, long, const double*, long, const double*, long, double*, long,
+                                double, double, const char*);
+template void gemm_pair<float>(jues_ctx*, bool, long, long, long, const float*, long, const float*, long, float*, long,
+                               long, long, long, const float*, long, const float*, long, float*, long, float, float,
+                               const char*);
 
 template void gemm<double>(jues_ctx*, bool, bool, long, long, long, double, const double*, long, long,
                            const double*, long, long, double, double*, long, long, int, const char*);

@@ -104,6 +104,14 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
           long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch = 1,
           const char* tag = nullptr);
 
+// Two independent products C_i = alpha A_i op(B_i) + beta C_i (A_i m-contiguous, op the same
+// for both) in one launch, so that the second fills the first one's last partial round of
+// workgroups; falls back to two gemm calls where the high-occupancy kernels do not apply.
+template <typename T>
+void gemm_pair(jues_ctx* ctx, bool tb, long M0, long N0, long K0, const T* A0, long lda0, const T* B0, long ldb0,
+               T* C0, long ldc0, long M1, long N1, long K1, const T* A1, long lda1, const T* B1, long ldb1, T* C1,
+               long ldc1, T alpha, T beta, const char* tag = nullptr);
+
 // ---- split-FP16 MFMA GEMM (gemm16.hip) ---------------------------------------
 // An FP32 matrix X (rows x cols) as two FP16 planes hi + lo = s X, column-major and
 // zero-padded to rows_p (multiple of 32) x cols_p (>= cols + 256), interleaved per

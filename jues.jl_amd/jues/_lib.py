@@ -28,7 +28,7 @@ EXPORTS = (
     "jues_release", "jues_set_ao_eri", "jues_set_ao_factors", "jues_set_df_integrals", "jues_get_ao_factors",
     "jues_tei_transform", "jues_get_eri",
     "jues_rmp2", "jues_rmp2_grad", "jues_rccd", "jues_rccsd", "jues_cc_setup", "jues_cc_iterate",
-    "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test",
+    "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test", "jues_gemm_pair_test",
     "jues_compute_pT", "jues_rccsd_pT", "jues_ump2", "jues_get_fock", "jues_rccsd_fock",
 )
 
@@ -113,6 +113,8 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "jues_kernel_stats": (i, [_ctxp, C.POINTER(C.c_char_p), _dp, _dp, _dp, _dp, i]),
             "jues_kernel_stats_reset": (None, [_ctxp, i]),
             "jues_gemm_test": (i, [_ctxp, i, C.c_char, C.c_char, i, i, i, d, _dp, i, _dp, i, d, _dp, i]),
+            "jues_gemm_pair_test": (i, [_ctxp, i, C.c_char, i, i, i, _dp, i, _dp, i, _dp, i, i, i, i, _dp, i, _dp, i,
+                                        _dp, i, d, d]),
         }
         for name, (res, args) in proto.items():
             f = getattr(lib, name)
@@ -222,6 +224,18 @@ class Context:
         self.check(self.lib.jues_gemm_test(self.h, prec, ta.encode(), tb.encode(), M, N, K, alpha, ptr(A),
                                            A.shape[0], ptr(B), B.shape[0], beta, ptr(Cm), Cm.shape[0]))
         return Cm
+
+
+    def gemm_pair_test(self, prec, tb, alpha, A0, B0, C0, A1, B1, C1, beta):
+        """Both products of one launch: returns (C0', C1')."""
+        A0, B0, A1, B1 = f64(A0), f64(B0), f64(A1), f64(B1)
+        C0, C1 = f64(C0).copy(order="F"), f64(C1).copy(order="F")
+        (M0, N0), (M1, N1) = C0.shape, C1.shape
+        self.check(self.lib.jues_gemm_pair_test(self.h, prec, tb.encode(), M0, N0, A0.shape[1], ptr(A0), A0.shape[0],
+                                                ptr(B0), B0.shape[0], ptr(C0), C0.shape[0], M1, N1, A1.shape[1],
+                                                ptr(A1), A1.shape[0], ptr(B1), B1.shape[0], ptr(C1), C1.shape[0],
+                                                alpha, beta))
+        return C0, C1
 
 
 _ctx: dict[int, Context] = {}

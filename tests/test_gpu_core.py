@@ -128,3 +128,25 @@ def test_gemm_nn_nt_occupancy_kernels(ctx, M, N, K, tb, prec, tol):
     ref = 0.7 * (A @ (B if tb == "N" else B.T)) - 0.3 * C0
     out = ctx.gemm_test(prec, "N", tb, 0.7, np.asfortranarray(A), np.asfortranarray(B), -0.3, np.asfortranarray(C0))
     assert rel_err(out, ref) < tol * max(1.0, np.sqrt(K) / 8)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-14), (1, 2e-6)])
+@pytest.mark.parametrize("tb", ["N", "T"])
+@pytest.mark.parametrize("s0,s1", [((1830, 2000, 1000), (1770, 1990, 990)), ((130, 129, 31), (1000, 700, 1040)),
+                                   ((1, 1, 16), (257, 300, 17)), ((300, 257, 100), (2, 3, 4)),
+                                   ((2048, 640, 1040), (64, 9000, 33))])
+def test_gemm_pair(ctx, s0, s1, tb, prec, tol):
+    """Two products in one launch (the packed ladders' plus and minus parts): each against
+    its own reference, with the first product's tile count padded to a multiple of 8 and
+    the fallback to two launches (K < 16, skinny tiles)."""
+    rng = np.random.default_rng(sum(s0) + sum(s1))
+    args, refs = [], []
+    for (M, N, K) in (s0, s1):
+        A = np.asfortranarray(rng.standard_normal((M, K)))
+        B = np.asfortranarray(rng.standard_normal((K, N)) if tb == "N" else rng.standard_normal((N, K)))
+        C0 = np.asfortranarray(rng.standard_normal((M, N)))
+        args += [A, B, C0]
+        refs.append(0.7 * (A @ (B if tb == "N" else B.T)) - 0.3 * C0)
+    out0, out1 = ctx.gemm_pair_test(prec, tb, 0.7, *args, -0.3)
+    for out, ref, (M, N, K) in zip((out0, out1), refs, (s0, s1)):
+        assert rel_err(out, ref) < tol * max(1.0, np.sqrt(K) / 8)
