@@ -33,7 +33,7 @@ struct CCWork {
   DevBuf Wpt, Wmt, Tq;  // hole-hole ladder on packed pairs
   DevBuf Xs, Vps, Vms;  // recomputed ladder: (ae|bf) slab and its packed V+/V- rows
   // RCCSD
-  DevBuf t1, Fme, X2, Z, Q, n1, U2, Tau;
+  DevBuf t1, Fme, X2, Z, Q, n1, U2, Tau, FmiT;
 };
 
 // Split-FP16 operand planes of the fast stage.  Slots below SPLIT_CONST hold

@@ -476,17 +476,14 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   const T one = T(1), zero = T(0), half = T(0.5);
 
   // amplitudes in the compute type and the layouts the GEMMs read
-  permute<double, T>(ctx, t2, d4, {0, 1, 2, 3}, Tpp, 1.0, 0.0, "cc_prep");
-  permute<T, T>(ctx, Tpp, d4, {0, 1, 2, 3}, Upp, 2.0, 0.0, "cc_prep");
-  permute<T, T>(ctx, Tpp, d4, {1, 0, 2, 3}, Upp, -1.0, 1.0, "cc_prep");
-  permute<T, T>(ctx, Tpp, d4, {0, 2, 1, 3}, Tov, 1.0, 0.0, "cc_prep");    // T[i,a,j,b]
-  permute<T, T>(ctx, Tpp, d4, {0, 3, 1, 2}, Ttov, 1.0, 0.0, "cc_prep");   // t[i,j,b,a] at [i,a,j,b]
-  permute<T, T>(ctx, Upp, d4, {0, 2, 1, 3}, Uov, 1.0, 0.0, "cc_prep");
+  // Tpp = t, Upp = 2t - t^(ij), Tov = T[i,a,j,b], Ttov = t[i,j,b,a] at [i,a,j,b], Uov = U[i,a,j,b]
+  rccd_layouts<T>(ctx, t2, o, v, Tpp, Upp, Tov, Ttov, Uov);
   permute<T, T>(ctx, Upp, d4, {2, 0, 1, 3}, Ua, 1.0, 0.0, "cc_prep");     // U[a,m,n,f]
 
-  // Fae[a,e] = -sum U[a,(mnf)] K[e,(mnf)];  Fmi[m,i] = sum K[m,(nef)] U[i,(nef)]
+  // Fae[a,e] = -sum U[a,(mnf)] K[e,(mnf)];  Fmi[m,i] = sum K[m,(nef)] U[i,(nef)], kept transposed
+  // (Fmi^T[i,m]) so that its product with t below is a plain NN GEMM
   gemm<T>(ctx, false, true, v, v, oo * v, T(-1), Ua, v, 0, I.Ka.template as<T>(), v, 0, zero, Fae, v, 0, 1, "cc_fock");
-  gemm<T>(ctx, false, true, o, o, ov * v, one, I.Kpp.template as<T>(), o, 0, Upp, o, 0, zero, Fmi, o, 0, 1, "cc_fock");
+  gemm<T>(ctx, false, true, o, o, ov * v, one, Upp, o, 0, I.Kpp.template as<T>(), o, 0, zero, Fmi, o, 0, 1, "cc_fock");
   // particle-particle ladder on the packed (anti)symmetric parts, then the hole-hole
   // ladder (W' = oooo + K t^T) added into the same packed sums
   ladder_mm<T>(ctx, S, t2, Tp, Tm, Lp, Lm);
@@ -507,7 +504,7 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
   // Fock-like terms: H += t[ia j, e] Fae[b,e] ; H -= Fmi[m,i] t[m, a jb]
   gemm<T>(ctx, false, true, oo * v, v, v, one, Tov, oo * v, 0, Fae, v, 0, one, H, oo * v, 0, 1, "cc_fock");
-  gemm<T>(ctx, true, false, o, ov * v, o, T(-1), Fmi, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
+  gemm<T>(ctx, false, false, o, ov * v, o, T(-1), Fmi, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
   // H += Y^x ; S = H + H^T (into WJ)
   permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
   JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));

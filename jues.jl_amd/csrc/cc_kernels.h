@@ -74,6 +74,10 @@ void pack_khh(jues_ctx* ctx, const PairTables& P, const double* K, int o, int v,
 // W[m,n,i,j] -> WPT[(i<=j),(k<=l)], WMT[(i<j),(k<l)] (the kl-pair sum / difference, transposed)
 template <typename T>
 void pack_whh(jues_ctx* ctx, const PairTables& P, const T* W, int o, T* WPT, T* WMT);
+// RCCD amplitude layouts from FP64 t2 [i,j,a,b]: Tpp = t, Upp = 2t - t^(ij), Tov[i,a,j,b] = t,
+// Ttov[i,a,j,b] = t[i,j,b,a], Uov[i,a,j,b] = U[i,j,a,b]
+template <typename T>
+void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, T* Tov, T* Ttov, T* Uov);
 // Tq = Tp with its (a,a) columns doubled
 template <typename T>
 void pair_diag2(jues_ctx* ctx, const PairTables& P, const T* Tp, T* Tq);

@@ -112,6 +112,31 @@ __global__ __launch_bounds__(256) void pair_diag2_kernel(const T* __restrict__ T
   }
 }
 
+// The RCCD residual's amplitude layouts from t2 (FP64, [i,j,a,b]) in one pass:
+// Tpp = t, Upp = 2t - t^(ij) ([i,j,a,b]); Tov[i,a,j,b] = t[ijab]; Uov[i,a,j,b] = U[ijab];
+// Ttov[i,b,j,a] = t[ijab] (that is t[i,j,b,a] at [i,a,j,b]).  Threads run i fastest, so
+// every write is a contiguous run of o; the exchanged read t[jiab] stays in one o x o slab.
+template <typename T>
+__global__ __launch_bounds__(256) void rccd_layouts_kernel(const double* __restrict__ t, int o, int v,
+                                                           T* __restrict__ Tpp, T* __restrict__ Upp,
+                                                           T* __restrict__ Tov, T* __restrict__ Ttov,
+                                                           T* __restrict__ Uov) {
+  const long oo = (long)o * o, ov = (long)o * v, n = oo * v * v;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < n; x += (long)gridDim.x * 256L) {
+    const long ij = x % oo, ab = x / oo;
+    const int i = (int)(ij % o), j = (int)(ij / o);
+    const int a = (int)(ab % v), b = (int)(ab / v);
+    const double tv = t[x];
+    const double u = 2.0 * tv - t[j + (long)o * i + oo * ab];
+    Tpp[x] = (T)tv;
+    Upp[x] = (T)u;
+    const long y = i + (long)o * a + ov * (j + (long)o * b);
+    Tov[y] = (T)tv;
+    Uov[y] = (T)u;
+    Ttov[i + (long)o * b + ov * (j + (long)o * a)] = (T)tv;
+  }
+}
+
 __device__ inline double block_sum256(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   __syncthreads();
@@ -385,6 +410,15 @@ void pair_diag2(jues_ctx* ctx, const PairTables& P, const T* Tp, T* Tq) {
   JUES_HIP(hipGetLastError());
 }
 
+template <typename T>
+void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, T* Tov, T* Ttov, T* Uov) {
+  const long n = (long)o * o * v * v;
+  KTimer k(ctx, "cc_prep", 0.0, (double)n * (16.0 + 5.0 * sizeof(T)));
+  hipLaunchKernelGGL((rccd_layouts_kernel<T>), dim3(grid256(n)), dim3(256), 0, ctx->stream, t, o, v, Tpp, Upp, Tov,
+                     Ttov, Uov);
+  JUES_HIP(hipGetLastError());
+}
+
 template <typename Ts, typename Td>
 void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm, bool slab) {
   const long r0p = (long)f0 * (f0 + 1) / 2, r0m = (long)f0 * (f0 - 1) / 2;
@@ -435,6 +469,8 @@ void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v, double
   JUES_HIP(hipGetLastError());
 }
 
+template void rccd_layouts<double>(jues_ctx*, const double*, int, int, double*, double*, double*, double*, double*);
+template void rccd_layouts<float>(jues_ctx*, const double*, int, int, float*, float*, float*, float*, float*);
 template void pack_whh<double>(jues_ctx*, const PairTables&, const double*, int, double*, double*);
 template void pack_whh<float>(jues_ctx*, const PairTables&, const float*, int, float*, float*);
 template void pair_diag2<double>(jues_ctx*, const PairTables&, const double*, double*);

@@ -201,7 +201,9 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
   // Fock-like terms with Fb / Fj
   gemm<T>(ctx, false, true, oo * v, v, v, one, Tov, oo * v, 0, Fae, v, 0, one, H, oo * v, 0, 1, "cc_fock");
-  gemm<T>(ctx, true, false, o, ov * v, o, T(-1), Fmi, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
+  T* FmiT = bufp_t<T>(W.FmiT, oo);
+  permute<T, T>(ctx, Fmi, {o, o}, {1, 0}, FmiT, 1.0, 0.0, "cc_fock");
+  gemm<T>(ctx, false, false, o, ov * v, o, T(-1), FmiT, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
   permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
   // T1^2 terms: -t1[ie] t1[ma] (me|jb) and -t1[ie] t1[mb] (mj|ae)
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(Kov, "mejb"), 0.0, TN(Q, "imjb"), "cc_t1");

@@ -107,6 +107,7 @@ struct GemmSide {
   T* C;
   long lda, ldb, ldc, sA, sB, sC;
   int M, N, K, tiles_m, tiles_n;
+  int Klast;  // K of the last batch entry (a split-K launch's shorter last chunk), else K
 };
 
 // Shared-memory layout of one operand slice.  When the operand's contiguous
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   const int wm = wave & 1, wn = wave >> 1;
   const bool second = (int)blockIdx.x >= nt0;
   const GemmSide<float> g = second ? p1 : p0;
-  const int M = g.M, N = g.N, K = g.K;
+  const int M = g.M, N = g.N, K = (int)blockIdx.z == (int)gridDim.z - 1 ? g.Klast : g.K;
   const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
   const float* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
   const float* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 wavefronts of 64 x 32
   const bool second = (int)blockIdx.x >= nt0;
   const GemmSide<double> g = second ? p1 : p0;
-  const int M = g.M, N = g.N, K = g.K;
+  const int M = g.M, N = g.N, K = (int)blockIdx.z == (int)gridDim.z - 1 ? g.Klast : g.K;
   const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
   const double* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
   const double* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
@@ -672,7 +673,7 @@ template <typename T>
 static GemmSide<T> make_side(long M, long N, long K, const T* A, long lda, long sA, const T* B, long ldb, long sB,
                              T* C, long ldc, long sC) {
   return GemmSide<T>{A, B, C, lda, ldb, ldc, sA, sB, sC, (int)M, (int)N, (int)K, (int)tiles_m128(M),
-                     (int)tiles_m128(N)};
+                     (int)tiles_m128(N), (int)K};
 }
 
 // One launch of the high-occupancy kernel over p0 (and p1 when `two`), nb batch entries.
@@ -728,7 +729,12 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   // tile shape: 0 = 128x128, 1 = 64x256 (skinny M), 2 = 256x64 (skinny N)
   // (the ring and ladder kernels, KIND 1 and 2, are instantiated in the 128x128 shape only)
-  const int shape = kind != 0 ? 0 : (M <= 64 && N >= 8192) ? 1 : (N <= 64 && M >= 8192) ? 2 : 0;
+  // (a skinny product with K <= 128 is bound by streaming op(B) and C: the 128x128 kernel, whose
+  // extra MFMA work on the clamped rows costs nothing there, serves it)
+  const int shape = kind != 0 ? 0
+                    : (M <= 64 && N >= 8192 && (ta || K > 128)) ? 1
+                    : (N <= 64 && M >= 8192 && (ta || K > 128)) ? 2
+                                                                 : 0;
   const long BM = shape == 1 ? 64 : shape == 2 ? 256 : 128, BN = shape == 1 ? 256 : shape == 2 ? 64 : 128;
   const long tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   JUES_REQUIRE(tiles_m * tiles_n < (1L << 31), "gemm: too many tiles");
@@ -742,11 +748,19 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
     const int S = (int)((K + kc - 1) / kc);
     T* Wk = static_cast<T*>(scratch(ctx, 9, sizeof(T) * (size_t)S * M * N));
     const long sa = ta ? kc : kc * lda, sb = tb ? kc * ldb : kc;
-    if (S > 1)
-      gemm_impl<T>(ctx, ta, tb, M, N, kc, alpha, A, lda, sa, B, ldb, sb, T(0), Wk, M, M * N, S - 1, tag, false);
     const long klast = K - (long)(S - 1) * kc;
-    gemm_impl<T>(ctx, ta, tb, M, N, klast, alpha, A + (long)(S - 1) * sa, lda, 0, B + (long)(S - 1) * sb, ldb, 0,
-                 T(0), Wk + (long)(S - 1) * M * N, M, 0, 1, tag, false);
+    if (shape == 0 && occ_kernel_ok<T>(ta, std::min(kc, klast), lda, ldb)) {
+      // all chunks in one launch, the last one shorter
+      KTimer t1(ctx, name, 2.0 * M * N * K, (double)sizeof(T) * ((double)M * K + (double)K * N + (double)S * M * N));
+      GemmSide<T> p = make_side<T>(M, N, kc, A, lda, sa, B, ldb, sb, Wk, M, M * N);
+      p.Klast = (int)klast;
+      occ_launch<T>(ctx, tb, kind, p, p, false, S, alpha, T(0));
+    } else {
+      if (S > 1)
+        gemm_impl<T>(ctx, ta, tb, M, N, kc, alpha, A, lda, sa, B, ldb, sb, T(0), Wk, M, M * N, S - 1, tag, false);
+      gemm_impl<T>(ctx, ta, tb, M, N, klast, alpha, A + (long)(S - 1) * sa, lda, 0, B + (long)(S - 1) * sb, ldb, 0,
+                   T(0), Wk + (long)(S - 1) * M * N, M, 0, 1, tag, false);
+    }
     KTimer t(ctx, name, 0.0, (double)sizeof(T) * S * M * N);
     const long tot = M * N;
     const int nb = (int)std::max(1L, std::min((tot + 255) / 256, 65536L));
@@ -829,11 +843,39 @@ void gemm_pair(jues_ctx* ctx, bool tb, long M0, long N0, long K0, const T* A0, l
   const int kind = !tag ? 0 : (std::strcmp(tag, "cc_ring") == 0 ? 1 : (std::strcmp(tag, "cc_pp_ladder") == 0 ? 2 : 0));
   const double sz = sizeof(T), bf = beta != T(0) ? 2.0 : 1.0;
   const double flops = 2.0 * M0 * N0 * K0 + 2.0 * M1 * N1 * K1;
+  GemmSide<T> p0 = make_side<T>(M0, N0, K0, A0, lda0, 0, B0, ldb0, 0, C0, ldc0, 0);
+  GemmSide<T> p1 = make_side<T>(M1, N1, K1, A1, lda1, 0, B1, ldb1, 0, C1, ldc1, 0);
+  // Fewer tiles than about two rounds of resident workgroups (1024 FP32, 512 FP64) and long
+  // K: split both K ranges into S chunks (one launch; the last chunk of each shorter), write
+  // the partial products to a workspace and add them to C0 / C1 in a fixed order.
+  const long tiles = (long)p0.tiles_m * p0.tiles_n + (long)p1.tiles_m * p1.tiles_n;
+  const long slots = sizeof(T) == 4 ? 1024 : 512;
+  const long S = std::min(std::min(K0, K1) / 1024, (2 * slots + tiles - 1) / tiles);
+  if (tiles < slots && S >= 2) {
+    const long kc0 = (K0 + S - 1) / S, kc1 = (K1 + S - 1) / S;
+    const long n0 = M0 * N0, n1 = M1 * N1;
+    T* Wk = static_cast<T*>(scratch(ctx, 16, sizeof(T) * (size_t)S * (n0 + n1)));
+    {
+      KTimer timer(ctx, name, flops, sz * ((double)M0 * K0 + (double)K0 * N0 + (double)M1 * K1 + (double)K1 * N1 +
+                                           (double)S * (n0 + n1)));
+      p0 = make_side<T>(M0, N0, kc0, A0, lda0, kc0 * lda0, B0, ldb0, tb ? kc0 * ldb0 : kc0, Wk, M0, n0);
+      p1 = make_side<T>(M1, N1, kc1, A1, lda1, kc1 * lda1, B1, ldb1, tb ? kc1 * ldb1 : kc1, Wk + S * n0, M1, n1);
+      p0.Klast = (int)(K0 - (S - 1) * kc0);
+      p1.Klast = (int)(K1 - (S - 1) * kc1);
+      occ_launch<T>(ctx, tb, kind, p0, p1, true, (int)S, alpha, T(0));
+    }
+    KTimer t(ctx, name, 0.0, sz * (S + bf) * (n0 + n1));
+    hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)std::min((n0 + 255) / 256, 65536L)), dim3(256), 0,
+                       ctx->stream, Wk, (int)S, M0, N0, C0, ldc0, beta);
+    hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)std::min((n1 + 255) / 256, 65536L)), dim3(256), 0,
+                       ctx->stream, Wk + S * n0, (int)S, M1, N1, C1, ldc1, beta);
+    JUES_HIP(hipGetLastError());
+    return;
+  }
   const double bytes = sz * ((double)M0 * K0 + (double)K0 * N0 + bf * M0 * N0 + (double)M1 * K1 + (double)K1 * N1 +
                              bf * M1 * N1);
   KTimer timer(ctx, name, flops, bytes);
-  occ_launch<T>(ctx, tb, kind, make_side<T>(M0, N0, K0, A0, lda0, 0, B0, ldb0, 0, C0, ldc0, 0),
-                make_side<T>(M1, N1, K1, A1, lda1, 0, B1, ldb1, 0, C1, ldc1, 0), true, 1, alpha, beta);
+  occ_launch<T>(ctx, tb, kind, p0, p1, true, 1, alpha, beta);
 }
 
 template void gemm_pair<double>(jues_ctx*, bool, long, long, long, const double*, long, const double*, long, double*,

@@ -71,13 +71,16 @@ __device__ inline void tile_coords(int pid, int nwg, int tiles_m, int tiles_n, i
 }
 
 // KIND separates kernel symbols for profiles (0 generic, 1 CC ring, 2 CC pp-ladder, 3 CC hole-hole).
+// blockIdx.z selects a chunk of kchunk rows of K (a multiple of BK; the last chunk may be
+// shorter) whose product goes to C + z sC: a split-K launch writes partial products.
 template <int KIND>
 __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, float alpha,
                                                        const _Float16* __restrict__ A, long lda2,
                                                        const int* __restrict__ ea,
                                                        const _Float16* __restrict__ B, long ldb2,
                                                        const int* __restrict__ eb, float beta,
-                                                       float* __restrict__ C, long ldc, int tiles_m, int tiles_n) {
+                                                       float* __restrict__ C, long ldc, int tiles_m, int tiles_n,
+                                                       int kchunk, long sC) {
   constexpr int PA = BM * 128, PB = BN * 128;         // bytes of one slice of each operand
   constexpr int CA = BM * 8 / NT, CB = BN * 8 / NT;   // 16-byte chunks per thread per slice
   constexpr int IM = BM / WM / 16, JN = BN / WN / 16;
@@ -104,15 +107,18 @@ __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, flo
     voffB[p] = (int)(2 * ((q & 7) * 8 + (long)(q >> 3) * ldb2));
     loffB[p] = PA + lds_off(q >> 3, q & 7);
   }
-  const _Float16* aT = A + (long)m0 * lda2;
-  const _Float16* bT = B + (long)n0 * ldb2;
+  // this chunk's k-range starts 2 kz0 halves into every column (k-blocks of 32 hi + 32 lo)
+  const int kz0 = (int)blockIdx.z * kchunk;
+  const _Float16* aT = A + (long)m0 * lda2 + 2L * kz0;
+  const _Float16* bT = B + (long)n0 * ldb2 + 2L * kz0;
+  C += (long)blockIdx.z * sC;
 
   // Two register slots hold the next two k-slices in flight; the loop is unrolled by
   // two so that slot indices are compile-time constants.  Loads past the last slice
   // re-read the last slice (in bounds, never used), which keeps the body free of
   // branches so the scheduler can interleave loads, LDS traffic and MFMAs.  (Macros,
   // not lambdas: lambdas over the register arrays cost spills.)
-  const int nk = Kp / BK;
+  const int nk = min(kchunk, Kp - kz0) / BK;
   u4 sa0[CA], sb0[CB], sa1[CA], sb1[CB];
 #define GLOAD(SA, SB, KT)                                                                                 \
   {                                                                                                       \
@@ -222,6 +228,18 @@ __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, flo
 #undef LSTORE
 #undef COMPUTE
 #undef SCHEDULE
+
+// C = beta C + sum_s W[s] (fixed order), W[s] is M x N with leading dimension M
+__global__ __launch_bounds__(256) void splitk16_reduce_kernel(const float* __restrict__ W, int S, long M, long N,
+                                                              float* __restrict__ C, long ldc, float beta) {
+  const long total = M * N;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256L) {
+    float acc = 0.f;
+    for (int q = 0; q < S; ++q) acc += W[(long)q * total + e];
+    const long off = e % M + (e / M) * ldc;
+    C[off] = beta != 0.f ? acc + beta * C[off] : acc;
+  }
+}
 
 constexpr int MAX_PART = 1024;
 
@@ -409,18 +427,46 @@ void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, con
                            : std::strcmp(tag, "cc_pp_ladder") == 0 ? 2
                            : std::strcmp(tag, "cc_hh") == 0        ? 3
                                                                    : 0);
-  KTimer timer(ctx, name, flops, bytes);
-  dim3 grid((unsigned)(tiles_m * tiles_n));
-#define JUES_G16(K_)                                                                                           \
-  hipLaunchKernelGGL((gemm16_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)A.rows_p, alpha, \
-                     A.p, 2 * A.rows_p, A.sexp, B.p, 2 * B.rows_p, B.sexp, beta, C, ldc, (int)tiles_m,           \
-                     (int)tiles_n)
-  if (kind == 1) JUES_G16(1);
-  else if (kind == 2) JUES_G16(2);
-  else if (kind == 3) JUES_G16(3);
-  else JUES_G16(0);
+  // Fewer tiles than two rounds of workgroups (one per CU) and a long K: split K into S
+  // chunks in one launch, partial products to a workspace, summed into C in a fixed order.
+  const long Kp = A.rows_p, tiles = tiles_m * tiles_n;
+  long S = std::min(Kp / 1024, (512 + tiles - 1) / tiles);
+  long kchunk = Kp;
+  if (tiles < 256 && S >= 2) {
+    kchunk = ((Kp + S - 1) / S + BK - 1) / BK * BK;
+    S = (Kp + kchunk - 1) / kchunk;
+  } else {
+    S = 1;
+  }
+  float* Cw = C;
+  long ldw = ldc, sW = 0;
+  float bw = beta;
+  if (S > 1) {
+    Cw = static_cast<float*>(scratch(ctx, 17, sizeof(float) * (size_t)S * M * N));
+    ldw = M;
+    sW = M * N;
+    bw = 0.0f;
+  }
+  {
+    KTimer timer(ctx, name, flops, bytes);
+    dim3 grid((unsigned)tiles, 1, (unsigned)S);
+#define JUES_G16(K_)                                                                                             \
+  hipLaunchKernelGGL((gemm16_kernel<K_>), grid, dim3(NT), 0, ctx->stream, (int)M, (int)N, (int)Kp, alpha, A.p,    \
+                     2 * A.rows_p, A.sexp, B.p, 2 * B.rows_p, B.sexp, bw, Cw, ldw, (int)tiles_m, (int)tiles_n,     \
+                     (int)kchunk, sW)
+    if (kind == 1) JUES_G16(1);
+    else if (kind == 2) JUES_G16(2);
+    else if (kind == 3) JUES_G16(3);
+    else JUES_G16(0);
 #undef JUES_G16
-  JUES_HIP(hipGetLastError());
+    JUES_HIP(hipGetLastError());
+  }
+  if (S > 1) {
+    KTimer t(ctx, name, 0.0, 4.0 * (S + (beta != 0.0f ? 2 : 1)) * M * N);
+    hipLaunchKernelGGL(splitk16_reduce_kernel, dim3((unsigned)std::min((M * N + 255) / 256, 65536L)), dim3(256), 0,
+                       ctx->stream, Cw, (int)S, M, N, C, ldc, beta);
+    JUES_HIP(hipGetLastError());
+  }
 }
 
 template void split16<float>(jues_ctx*, const float*, long, long, long, Split16Buf&, const char*);

@@ -34,7 +34,7 @@ void DevBuf::release() {
 
 namespace {
 struct ScratchPool {
-  DevBuf slot[16];
+  DevBuf slot[20];
 };
 std::map<const jues_ctx*, ScratchPool>& pools() {
   static std::map<const jues_ctx*, ScratchPool> m;

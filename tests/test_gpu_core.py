@@ -116,7 +116,8 @@ def test_rmp2(cfg1, dense):
 @pytest.mark.parametrize("tb", ["N", "T"])
 @pytest.mark.parametrize("M,N,K", [(128, 128, 16), (129, 131, 33), (300, 257, 1000), (1000, 1000, 32),
                                    (256, 384, 48), (5, 7, 3), (1, 1, 16), (2048, 640, 1040), (40, 9000, 256),
-                                   (9000, 40, 256), (257, 300, 17), (640, 520, 4111), (130, 129, 31)])
+                                   (9000, 40, 256), (257, 300, 17), (640, 520, 4111), (130, 129, 31),
+                                   (2048, 2048, 256), (2100, 2300, 1041), (4099, 2049, 513)])
 def test_gemm_nn_nt_occupancy_kernels(ctx, M, N, K, tb, prec, tol):
     """Products with A m-contiguous and B k- or n-contiguous on the high-occupancy kernels
     (FP32: four 256-thread workgroups per CU; FP64: two 512-thread ones): full 16-column
@@ -134,11 +135,13 @@ def test_gemm_nn_nt_occupancy_kernels(ctx, M, N, K, tb, prec, tol):
 @pytest.mark.parametrize("tb", ["N", "T"])
 @pytest.mark.parametrize("s0,s1", [((1830, 2000, 1000), (1770, 1990, 990)), ((130, 129, 31), (1000, 700, 1040)),
                                    ((1, 1, 16), (257, 300, 17)), ((300, 257, 100), (2, 3, 4)),
-                                   ((2048, 640, 1040), (64, 9000, 33))])
+                                   ((2048, 640, 1040), (64, 9000, 33)),
+                                   # few tiles, long K: both K ranges split into chunks (one launch) and reduced
+                                   ((300, 257, 9000), (200, 150, 8000)), ((60, 60, 30000), (59, 58, 29001))])
 def test_gemm_pair(ctx, s0, s1, tb, prec, tol):
     """Two products in one launch (the packed ladders' plus and minus parts): each against
-    its own reference, with the first product's tile count padded to a multiple of 8 and
-    the fallback to two launches (K < 16, skinny tiles)."""
+    its own reference, with the first product's tile count padded to a multiple of 8, the
+    split-K form for few tiles and long K, and the fallback to two launches (K < 16, skinny)."""
     rng = np.random.default_rng(sum(s0) + sum(s1))
     args, refs = [], []
     for (M, N, K) in (s0, s1):

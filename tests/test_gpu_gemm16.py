@@ -20,7 +20,9 @@ def _normwise(out, ref, A, B, alpha, beta, C0):
 
 
 @pytest.mark.parametrize("M,N,K", [(3, 5, 7), (37, 150, 45), (128, 128, 32), (129, 257, 33),
-                                   (256, 256, 64), (1000, 777, 1300)])
+                                   (256, 256, 64), (1000, 777, 1300),
+                                   # few tiles and a long K: split into K-chunks summed in a second pass
+                                   (300, 200, 9000), (60, 60, 50000), (1830, 260, 4100)])
 def test_gemm16_shapes(ctx, M, N, K):
     rng = np.random.default_rng(M + 10 * N + 100 * K)
     A, B, C0 = rng.normal(size=(K, M)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
@@ -77,6 +79,10 @@ def test_gemm16_zero_operand_and_beta_zero_nan(ctx):
     A = rng.normal(size=(40, 70))
     out = ctx.gemm_test(FP16X3, "T", "N", 1.0, A, B, 0.0, np.full((70, 90), np.nan))
     assert np.all(np.isfinite(out))
+    # the same through the split-K pass (long K, one tile)
+    A, B = rng.normal(size=(6000, 70)), rng.normal(size=(6000, 90))
+    out = ctx.gemm_test(FP16X3, "T", "N", 1.0, A, B, 0.0, np.full((70, 90), np.nan))
+    assert _normwise(out, A.T @ B, A, B, 1.0, 0.0, np.zeros((70, 90))) < 4e-6
 
 
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "T")])
