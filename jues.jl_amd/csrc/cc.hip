@@ -243,14 +243,19 @@ SplitState& split_state(CCState& S) {
 // planes of operand X (rows x cols, leading dimension rows; with `trans`, of X^T
 // for X stored cols x rows) in `slot`.  Per-iteration slots are split on their
 // first use in an iteration, constant slots once.
+// Ttov holds the same values as Tov (a permutation of t), so its planes take Tov's scale when
+// Tov was split earlier in the same iteration.
 template <typename Ti>
 const Split16& split_operand(jues_ctx* ctx, CCState& S, int slot, const Ti* X, long rows, long cols,
                              bool trans = false) {
-  SplitState::Entry& e = split_state(S).e[slot];
+  SplitState& Z = split_state(S);
+  SplitState::Entry& e = Z.e[slot];
   const long stamp = slot < SPLIT_CONST ? (long)S.iter : -1;
   if (e.src != static_cast<const void*>(X) || e.stamp != stamp) {
+    const SplitState::Entry& tov = Z.e[SP_TOV];
+    const int* same = slot == SP_TTOV && !trans && tov.stamp == stamp && tov.src ? tov.buf.s.sexp : nullptr;
     if (trans) split16t<Ti>(ctx, X, rows, cols, cols, e.buf, "cc_split");
-    else split16<Ti>(ctx, X, rows, cols, rows, e.buf, "cc_split");
+    else split16<Ti>(ctx, X, rows, cols, rows, e.buf, "cc_split", same);
     e.src = X;
     e.stamp = stamp;
   }
@@ -497,8 +502,7 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, one, WJ);
   permute<T, T>(ctx, I.Jov.template as<T>(), {ov, ov}, {0, 1}, WE, -1.0, 0.0, "cc_prep");
   ring_mm<T>(ctx, S, half, Kx, SP_KX, Ttov, SP_TTOV, one, WE);
-  ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, Y);
-  JUES_HIP(hipMemcpyAsync(H, Y, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, H);
   ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);

@@ -194,8 +194,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   ring_mm<T>(ctx, S, half, Kx, SP_KX, Q, SP_Q, one, WE);
   contract<T>(ctx, -1.0, TN(t1, "jf"), TN(ovvv, "mfbe"), 1.0, TN(WE, "mejb"), "cc_t1");
   contract<T>(ctx, 1.0, TN(t1, "nb"), TN(ooov, "mjne"), 1.0, TN(WE, "mejb"), "cc_t1");
-  ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, Y);
-  JUES_HIP(hipMemcpyAsync(H, Y, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+  ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, H);
   ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);

@@ -204,6 +204,28 @@ __global__ __launch_bounds__(NT, 1) void gemm16_kernel(int M, int N, int Kp, flo
 
   const int descale = -(ea[0] + eb[0]);
   // acc[i][j][r] is C(m, n) with m = lane%16 of block i, n = 4(lane/16) + r of block j
+  if (beta != 0.0f && m0 + BM <= M && n0 + BN <= N) {
+    // interior tile accumulating into C: all of the lane's C loads first, then the updates
+    // (one load -> store round trip per element would serialise the epilogue)
+    float cv[IM][JN][4];
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          cv[i][j][r] = C[(long)(m0 + wm * (BM / WM) + i * 16 + fr) +
+                          (long)(n0 + wn * (BN / WN) + j * 16 + 4 * fc + r) * ldc];
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          C[(long)(m0 + wm * (BM / WM) + i * 16 + fr) + (long)(n0 + wn * (BN / WN) + j * 16 + 4 * fc + r) * ldc] =
+              alpha * ldexpf(acc[i][j][r], descale) + beta * cv[i][j][r];
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < IM; ++i) {
     const int m = m0 + wm * (BM / WM) + i * 16 + fr;
@@ -374,16 +396,22 @@ void Split16Buf::ensure(long rows, long cols) {
 }
 
 template <typename Ti>
-void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out, const char* tag) {
+void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out, const char* tag,
+             const int* sexp_of) {
   JUES_REQUIRE(ldx >= rows, "split16: ldx < rows");
   out.ensure(rows, cols);
   const Split16& s = out.s;
-  double* part = static_cast<double*>(scratch(ctx, 3, MAX_PART * sizeof(double)));
   const long n = rows * cols;
-  KTimer t(ctx, tag ? tag : "split16", 0.0, 2.0 * sizeof(Ti) * n + 4.0 * s.rows_p * s.cols_p);
-  const int nb = grid_for(n, MAX_PART);
-  hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, rows, cols, ldx, part);
-  hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
+  KTimer t(ctx, tag ? tag : "split16", 0.0, (sexp_of ? 1.0 : 2.0) * sizeof(Ti) * n + 4.0 * s.rows_p * s.cols_p);
+  if (sexp_of) {
+    // the caller vouches that X holds the same values as the operand whose scale this is
+    JUES_HIP(hipMemcpyAsync(s.sexp, sexp_of, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    double* part = static_cast<double*>(scratch(ctx, 3, MAX_PART * sizeof(double)));
+    const int nb = grid_for(n, MAX_PART);
+    hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, rows, cols, ldx, part);
+    hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
+  }
   hipLaunchKernelGGL((split_kernel<Ti>), dim3(grid_for(s.rows_p / 8 * s.cols_p, 131072)), dim3(256), 0, ctx->stream, X,
                      rows, cols, ldx, s.rows_p, s.cols_p, s.sexp, s.p);
   JUES_HIP(hipGetLastError());
@@ -469,8 +497,8 @@ void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, con
   }
 }
 
-template void split16<float>(jues_ctx*, const float*, long, long, long, Split16Buf&, const char*);
-template void split16<double>(jues_ctx*, const double*, long, long, long, Split16Buf&, const char*);
+template void split16<float>(jues_ctx*, const float*, long, long, long, Split16Buf&, const char*, const int*);
+template void split16<double>(jues_ctx*, const double*, long, long, long, Split16Buf&, const char*, const int*);
 template void split16t<float>(jues_ctx*, const float*, long, long, long, Split16Buf&, const char*);
 template void split16t<double>(jues_ctx*, const double*, long, long, long, Split16Buf&, const char*);
 

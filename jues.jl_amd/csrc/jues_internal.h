@@ -135,10 +135,11 @@ inline Split16 split16_cols(const Split16& s, long c0, long nc) {
   r.cols_p = s.cols_p - c0;
   return r;
 }
-// planes of X; the scale is found on the device (no host synchronisation)
+// planes of X; the scale is found on the device (no host synchronisation), or taken from
+// sexp_of when X is known to hold the same values as the operand that scale belongs to
 template <typename Ti>
 void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out,
-             const char* tag = nullptr);
+             const char* tag = nullptr, const int* sexp_of = nullptr);
 // planes (rows x cols) of X^T, X stored cols x rows with leading dimension ldx
 template <typename Ti>
 void split16t(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16Buf& out,
