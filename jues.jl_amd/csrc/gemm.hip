@@ -448,6 +448,25 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
     __syncthreads();
   }
 
+  if (beta != 0.f && m0 + BM <= M && n0 + BN <= N) {
+    // interior tile accumulating into C: per 16-row block, all of the lane's C loads before
+    // its stores (an element-wise load -> store round trip would serialise the epilogue)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long m = m0 + wm * 64 + i * 16 + (lane & 15);
+      float cv[4][4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[j][r] = C[m + (long)(n0 + wn * 64 + j * 16 + 4 * (lane >> 4) + r) * ldc];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          C[m + (long)(n0 + wn * 64 + j * 16 + 4 * (lane >> 4) + r) * ldc] = alpha * acc[i][j][r] + beta * cv[j][r];
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int m = m0 + wm * 64 + i * 16 + (lane & 15);
@@ -614,6 +633,24 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
   }
 
   // acc[i][j][r]: C(m, n) with m = lane % 16 of block i, n = lane / 16 + 4 r of block j
+  if (beta != 0.0 && m0 + BM <= M && n0 + BN <= N) {
+    // interior tile accumulating into C: per 16-row block, the loads before the stores
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long m = m0 + wm * 64 + i * 16 + (lane & 15);
+      double cv[2][4];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[j][r] = C[m + (long)(n0 + wn * 32 + j * 16 + (lane >> 4) + 4 * r) * ldc];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          C[m + (long)(n0 + wn * 32 + j * 16 + (lane >> 4) + 4 * r) * ldc] = alpha * acc[i][j][r] + beta * cv[j][r];
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int m = m0 + wm * 64 + i * 16 + (lane & 15);
