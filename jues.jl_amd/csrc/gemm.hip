@@ -765,11 +765,11 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
     return;
   }
   // tile shape: 0 = 128x128, 1 = 64x256 (skinny M), 2 = 256x64 (skinny N)
-  // (the ring and ladder kernels, KIND 1 and 2, are instantiated in the 128x128 shape only)
+  // (the ring and ladder kernel symbols, KIND 1 and 2, exist in the 128x128 shape only: a skinny
+  // product under those tags runs on the KIND 0 kernel of its shape)
   // (a skinny product with K <= 128 is bound by streaming op(B) and C: the 128x128 kernel, whose
   // extra MFMA work on the clamped rows costs nothing there, serves it)
-  const int shape = kind != 0 ? 0
-                    : (M <= 64 && N >= 8192 && (ta || K > 128)) ? 1
+  const int shape = (M <= 64 && N >= 8192 && (ta || K > 128)) ? 1
                     : (N <= 64 && M >= 8192 && (ta || K > 128)) ? 2
                                                                  : 0;
   const long BM = shape == 1 ? 64 : shape == 2 ? 256 : 128, BN = shape == 1 ? 256 : shape == 2 ? 64 : 128;
@@ -839,8 +839,8 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
     else JUES_GEMM_LAUNCH_S(TA_, TB_, K_, 128, 128, 2);                         \
   } while (0)
     if (!ta && !tb) {
-      if (kind == 1) JUES_GEMM_LAUNCH_S(false, false, 1, 128, 128, 2);
-      else if (kind == 2) JUES_GEMM_LAUNCH_S(false, false, 2, 128, 128, 2);
+      if (kind == 1 && shape == 0) JUES_GEMM_LAUNCH_S(false, false, 1, 128, 128, 2);
+      else if (kind == 2 && shape == 0) JUES_GEMM_LAUNCH_S(false, false, 2, 128, 128, 2);
       else JUES_GEMM_LAUNCH(false, false, 0);
     } else if (!ta && tb) JUES_GEMM_LAUNCH(false, true, 0);
     else if (ta && !tb) JUES_GEMM_LAUNCH(true, false, 0);

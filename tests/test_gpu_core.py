@@ -117,7 +117,9 @@ def test_rmp2(cfg1, dense):
 @pytest.mark.parametrize("M,N,K", [(128, 128, 16), (129, 131, 33), (300, 257, 1000), (1000, 1000, 32),
                                    (256, 384, 48), (5, 7, 3), (1, 1, 16), (2048, 640, 1040), (40, 9000, 256),
                                    (9000, 40, 256), (257, 300, 17), (640, 520, 4111), (130, 129, 31),
-                                   (2048, 2048, 256), (2100, 2300, 1041), (4099, 2049, 513)])
+                                   (2048, 2048, 256), (2100, 2300, 1041), (4099, 2049, 513),
+                                   # the packed ladder's M = o(o+1)/2 at o = 60: a last tile row of 38 rows
+                                   (1830, 8200, 4100)])
 def test_gemm_nn_nt_occupancy_kernels(ctx, M, N, K, tb, prec, tol):
     """Products with A m-contiguous and B k- or n-contiguous on the high-occupancy kernels
     (FP32: four 256-thread workgroups per CU; FP64: two 512-thread ones): full 16-column
