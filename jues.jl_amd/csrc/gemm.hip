@@ -108,6 +108,11 @@ struct GemmSide {
   long lda, ldb, ldc, sA, sB, sC;
   int M, N, K, tiles_m, tiles_n;
   int Klast;  // K of the last batch entry (a split-K launch's shorter last chunk), else K
+  // optional second contraction segment: C += alpha A1 op(B1) over K1 more columns (same lda,
+  // ldb and batch strides); with K1 > 0, K and K1 are multiples of 16
+  const T* A1;
+  const T* B1;
+  int K1;
 };
 
 // Shared-memory layout of one operand slice.  When the operand's contiguous
@@ -321,6 +326,9 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   const float* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
   const float* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
   float* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
+  const int K1 = g.K1;
+  const float* A1 = K1 > 0 ? g.A1 + (long)blockIdx.z * g.sA : A;
+  const float* B1 = K1 > 0 ? g.B1 + (long)blockIdx.z * g.sB : B;
   int tm, tn;
   if (!tile_coords_bounded(second ? (int)blockIdx.x - nt0 : (int)blockIdx.x, second ? (int)gridDim.x - nt0 : nt0,
                            g.tiles_m, g.tiles_n, tm, tn))
@@ -340,25 +348,31 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   const int bt_voff = 4 * (min(n0 + am, N - 1) - n0);
   const int b_voff0 = 4 * (4 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
   const int b_voff1 = 4 * (4 * bc + (min(n0 + bn + 64, N - 1) - n0) * (int)ldb);
-  auto load = [&](int k0) {
-    const auto rA = make_rsrc(A + m0 + (long)k0 * lda);
+  auto load_seg = [&](const float* Ab, const float* Bb, int k0) {
+    const auto rA = make_rsrc(Ab + m0 + (long)k0 * lda);
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         ra[4 * j + u] = bload<float>(rA, a_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)lda);
     if constexpr (TB) {
-      const auto rB = make_rsrc(B + n0 + (long)k0 * ldb);
+      const auto rB = make_rsrc(Bb + n0 + (long)k0 * ldb);
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           rb[j][u] = bload<float>(rB, bt_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)ldb);
     } else {
-      const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
+      const auto rB = make_rsrc(Bb + k0 + (long)n0 * ldb);
       rb[0] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
       rb[1] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff1, 0, 0));
     }
+  };
+  // full slice s of the contraction: segment 0's K / BK slices, then segment 1's
+  const int nk0 = K / BK;
+  auto load = [&](int s) {
+    if (s < nk0) load_seg(A, B, s * BK);
+    else load_seg(A1, B1, (s - nk0) * BK);
   };
   auto store = [&](int buf) {
 #pragma unroll
@@ -389,7 +403,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   // slices: the K tail (columns 16 floor(K/16) .. K-1, zero-padded) first when there is one, then the
   // full slices from k = 0; the tail is loaded once, in the prologue
   const int K16 = K / BK * BK, tail = K > K16 ? 1 : 0;
-  const int nk = K16 / BK + tail;
+  const int nk = K16 / BK + tail + K1 / BK;
   if (tail) {
     const float* Ar = A + m0 + (a_voff >> 2);
 #pragma unroll
@@ -422,7 +436,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1 - tail) * BK);
+    if (kt + 1 < nk) load(kt + 1 - tail);
     f4v bf[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -514,6 +528,9 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
   const double* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
   const double* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
   double* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
+  const int K1 = g.K1;
+  const double* A1 = K1 > 0 ? g.A1 + (long)blockIdx.z * g.sA : A;
+  const double* B1 = K1 > 0 ? g.B1 + (long)blockIdx.z * g.sB : B;
   int tm, tn;
   if (!tile_coords_bounded(second ? (int)blockIdx.x - nt0 : (int)blockIdx.x, second ? (int)gridDim.x - nt0 : nt0,
                            g.tiles_m, g.tiles_n, tm, tn))
@@ -528,19 +545,25 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
   const int bt_voff = 8 * (min(n0 + am, N - 1) - n0);
   const int b_voff0 = 8 * (2 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
   const int b_voff1 = 8 * (2 * bc + (min(n0 + bn + 64, N - 1) - n0) * (int)ldb);
-  auto load = [&](int k0) {
-    const auto rA = make_rsrc(A + m0 + (long)k0 * lda);
+  auto load_seg = [&](const double* Ab, const double* Bb, int k0) {
+    const auto rA = make_rsrc(Ab + m0 + (long)k0 * lda);
 #pragma unroll
     for (int u = 0; u < 4; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (4 * kq + u) * (int)lda);
     if constexpr (TB) {
-      const auto rB = make_rsrc(B + n0 + (long)k0 * ldb);
+      const auto rB = make_rsrc(Bb + n0 + (long)k0 * ldb);
 #pragma unroll
       for (int u = 0; u < 4; ++u) rb[u >> 1][u & 1] = bload<double>(rB, bt_voff, 8 * (4 * kq + u) * (int)ldb);
     } else {
-      const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
+      const auto rB = make_rsrc(Bb + k0 + (long)n0 * ldb);
       rb[0] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
       rb[1] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff1, 0, 0));
     }
+  };
+  // full slice s of the contraction: segment 0's K / BK slices, then segment 1's
+  const int nk0 = K / BK;
+  auto load = [&](int s) {
+    if (s < nk0) load_seg(A, B, s * BK);
+    else load_seg(A1, B1, (s - nk0) * BK);
   };
   auto store = [&](int buf) {
     *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(2 * kq, am)]) = d2v{ra[0], ra[1]};
@@ -565,7 +588,7 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
   const int fr = lane & 15, fk = lane >> 4;
   // the K tail first (zero-padded, loaded once in the prologue), then the full slices from k = 0
   const int K16 = K / BK * BK, tail = K > K16 ? 1 : 0;
-  const int nk = K16 / BK + tail;
+  const int nk = K16 / BK + tail + K1 / BK;
   if (tail) {
     const double* Ar = A + m0 + (a_voff >> 3);
 #pragma unroll
@@ -599,7 +622,7 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1 - tail) * BK);
+    if (kt + 1 < nk) load(kt + 1 - tail);
     d2v bf[2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -710,7 +733,7 @@ template <typename T>
 static GemmSide<T> make_side(long M, long N, long K, const T* A, long lda, long sA, const T* B, long ldb, long sB,
                              T* C, long ldc, long sC) {
   return GemmSide<T>{A, B, C, lda, ldb, ldc, sA, sB, sC, (int)M, (int)N, (int)K, (int)tiles_m128(M),
-                     (int)tiles_m128(N), (int)K};
+                     (int)tiles_m128(N), (int)K, nullptr, nullptr, 0};
 }
 
 // One launch of the high-occupancy kernel over p0 (and p1 when `two`), nb batch entries.
@@ -914,6 +937,38 @@ void gemm_pair(jues_ctx* ctx, bool tb, long M0, long N0, long K0, const T* A0, l
   KTimer timer(ctx, name, flops, bytes);
   occ_launch<T>(ctx, tb, kind, p0, p1, true, 1, alpha, beta);
 }
+
+template <typename T>
+void gemm_ksum(jues_ctx* ctx, bool tb, long M, long N, long K0, const T* A0, const T* B0, long K1, const T* A1,
+               const T* B1, long lda, long ldb, long sA, long sB, T alpha, T beta, T* C, long ldc, long sC, int batch,
+               const char* tag) {
+  const bool skinny = (M <= 64 && N >= 8192) || (N <= 64 && M >= 8192);
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  if (skinny || K0 < 16 || K1 < 16 || K0 % 16 != 0 || K1 % 16 != 0 || batch > 65535 ||
+      !occ_kernel_ok<T>(false, std::max(K0, K1), lda, ldb)) {
+    gemm<T>(ctx, false, tb, M, N, K0, alpha, A0, lda, sA, B0, ldb, sB, beta, C, ldc, sC, batch, tag);
+    gemm<T>(ctx, false, tb, M, N, K1, alpha, A1, lda, sA, B1, ldb, sB, T(1), C, ldc, sC, batch, tag);
+    return;
+  }
+  JUES_REQUIRE(M < (1L << 31) && N < (1L << 31) && K0 + K1 < (1L << 31), "gemm: dimension too large");
+  JUES_REQUIRE(ldc >= M && lda >= M && ldb >= (tb ? N : std::max(K0, K1)), "gemm: leading dimension too small");
+  const char* name = tag ? tag : (sizeof(T) == 8 ? "gemm_f64" : "gemm_f32");
+  const double K = (double)(K0 + K1);
+  KTimer timer(ctx, name, 2.0 * M * N * K * batch,
+               (double)sizeof(T) * ((double)M * K + K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch);
+  GemmSide<T> p = make_side<T>(M, N, K0, A0, lda, sA, B0, ldb, sB, C, ldc, sC);
+  p.A1 = A1;
+  p.B1 = B1;
+  p.K1 = (int)K1;
+  occ_launch<T>(ctx, tb, 0, p, p, false, batch, alpha, beta);
+}
+
+template void gemm_ksum<double>(jues_ctx*, bool, long, long, long, const double*, const double*, long, const double*,
+                                const double*, long, long, long, long, double, double, double*, long, long, int,
+                                const char*);
+template void gemm_ksum<float>(jues_ctx*, bool, long, long, long, const float*, const float*, long, const float*,
+                               const float*, long, long, long, long, float, float, float*, long, long, int,
+                               const char*);
 
 template void gemm_pair<double>(jues_ctx*, bool, long, long, long, const double*, long, const double*, long, double*,
                                 long, long, long, long, const double*, long, const double*, long, double*, long,

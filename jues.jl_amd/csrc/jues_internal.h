@@ -112,6 +112,14 @@ void gemm_pair(jues_ctx* ctx, bool tb, long M0, long N0, long K0, const T* A0, l
                T* C0, long ldc0, long M1, long N1, long K1, const T* A1, long lda1, const T* B1, long ldb1, T* C1,
                long ldc1, T alpha, T beta, const char* tag = nullptr);
 
+// C = alpha (A0 op(B0) + A1 op(B1)) + beta C: one product over two contraction segments that
+// share the leading dimensions and batch strides, so C is read and written once.  One launch
+// when K0 and K1 are multiples of 16 and the high-occupancy kernels apply, else two GEMMs.
+template <typename T>
+void gemm_ksum(jues_ctx* ctx, bool tb, long M, long N, long K0, const T* A0, const T* B0, long K1, const T* A1,
+               const T* B1, long lda, long ldb, long sA, long sB, T alpha, T beta, T* C, long ldc, long sC,
+               int batch, const char* tag = nullptr);
+
 // ---- split-FP16 MFMA GEMM (gemm16.hip) ---------------------------------------
 // An FP32 matrix X (rows x cols) as two FP16 planes hi + lo = s X, column-major and
 // zero-padded to rows_p (multiple of 32) x cols_p (>= cols + 256), interleaved per

@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void pt_reduce_kernel(const double* __restrict
 //   Pe [a,b,d',i] = Vvvvo[b,d,a,i] (d' = d < v),   -T2[i,l,a,b] (d' = v + l)
 //   P2e[a,b,d',i] = Vvvvo[a,d,b,i],                 -T2[i,l,b,a]
 //   Te [c,d',k,j] = T2[k,j,c,d],                     Vvooo[c,k,j,l]
-// and each of the six GEMMs per triple contracts over v + o.
+// with d' padded by zero columns to a multiple of 16 (pt_ve), so that the two GEMMs with
+// one output layout run as one product over two contraction segments.
 template <typename T>
 struct PTOperands {
   DevBuf Pe, P2e, Te, W;
@@ -157,11 +158,15 @@ static void copy2d(jues_ctx* ctx, void* dst, size_t dpitch, const void* src, siz
   JUES_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToDevice, ctx->stream));
 }
 
-// W[a,b,c] (PerturbativeTriples.jl:105-111) for one triple (i, j, k): six GEMMs, each a
-// Vvvvo term plus the Vvooo term with the same output layout.
+// contraction length of the packed (T) operands: v + o rounded up to a multiple of 16
+static inline long pt_ve(int o, int v) { return ((long)v + o + 15) / 16 * 16; }
+
+// W[a,b,c] (PerturbativeTriples.jl:105-111) for one triple (i, j, k): six terms, each a
+// Vvvvo term plus the Vvooo term with the same output layout, as three products of two
+// contraction segments each (W written once and updated twice).
 template <typename T>
 void pt_build_W(jues_ctx* ctx, const PTOperands<T>& S, int o, int v, int i, int j, int k) {
-  const long vv = (long)v * v, ve = (long)v + o, blkP = vv * ve, blkT = (long)v * ve;
+  const long vv = (long)v * v, ve = pt_ve(o, v), blkP = vv * ve, blkT = (long)v * ve;
   const T* Pe = S.Pe.template as<T>();
   const T* P2e = S.P2e.template as<T>();
   const T* Te = S.Te.template as<T>();
@@ -170,24 +175,18 @@ void pt_build_W(jues_ctx* ctx, const PTOperands<T>& S, int o, int v, int i, int 
   auto P2 = [&](int x) { return P2e + x * blkP; };
   auto Tq = [&](int p, int q) { return Te + (long)(p + (long)o * q) * blkT; };
   const T one = T(1);
-  // ijk abc: W[(a,b),c] = Pe_i[(a,b),d'] Te_kj[c,d']
-  gemm<T>(ctx, false, true, vv, v, ve, one, P(i), vv, 0, Tq(k, j), v, 0, T(0), W, vv, 0, 1, "pt_w");
-  // ikj acb: W[a,b,c] += Pe_i[a,c,d'] Te_jk[b,d']   (batched over c)
-  gemm<T>(ctx, false, true, v, v, ve, one, P(i), vv, v, Tq(j, k), v, 0, one, W, v, vv, v, "pt_w");
-  // kij cab: W[a,b,c] += P2e_k[a,c,d'] Te_ji[b,d']  (batched over c)
-  gemm<T>(ctx, false, true, v, v, ve, one, P2(k), vv, v, Tq(j, i), v, 0, one, W, v, vv, v, "pt_w");
-  // kji cba: W[a,(b,c)] += Te_ij[a,d'] P2e_k[(b,c),d']
-  gemm<T>(ctx, false, true, v, vv, ve, one, Tq(i, j), v, 0, P2(k), vv, 0, one, W, v, 0, 1, "pt_w");
-  // jki bca: W[a,(b,c)] += Te_ik[a,d'] Pe_j[(b,c),d']
-  gemm<T>(ctx, false, true, v, vv, ve, one, Tq(i, k), v, 0, P(j), vv, 0, one, W, v, 0, 1, "pt_w");
-  // jik bac: W[(a,b),c] += P2e_j[(a,b),d'] Te_ki[c,d']
-  gemm<T>(ctx, false, true, vv, v, ve, one, P2(j), vv, 0, Tq(k, i), v, 0, one, W, vv, 0, 1, "pt_w");
+  // ijk abc + jik bac: W[(a,b),c] = Pe_i[(a,b),d'] Te_kj[c,d'] + P2e_j[(a,b),d'] Te_ki[c,d']
+  gemm_ksum<T>(ctx, true, vv, v, ve, P(i), Tq(k, j), ve, P2(j), Tq(k, i), vv, v, 0, 0, one, T(0), W, vv, 0, 1, "pt_w");
+  // ikj acb + kij cab: W[a,b,c] += Pe_i[a,c,d'] Te_jk[b,d'] + P2e_k[a,c,d'] Te_ji[b,d']  (batched over c)
+  gemm_ksum<T>(ctx, true, v, v, ve, P(i), Tq(j, k), ve, P2(k), Tq(j, i), vv, v, v, 0, one, one, W, v, vv, v, "pt_w");
+  // kji cba + jki bca: W[a,(b,c)] += Te_ij[a,d'] P2e_k[(b,c),d'] + Te_ik[a,d'] Pe_j[(b,c),d']
+  gemm_ksum<T>(ctx, true, v, vv, ve, Tq(i, j), P2(k), ve, Tq(i, k), P(j), v, vv, 0, 0, one, one, W, v, 0, 1, "pt_w");
 }
 
 // The amplitude columns of Pe/P2e and the amplitude half of Te, from T2 (device FP64).
 template <typename T>
 void pt_pack_amplitudes(jues_ctx* ctx, PTOperands<T>& S, const double* T2d, int o, int v) {
-  const long vv = (long)v * v, ve = (long)v + o;
+  const long vv = (long)v * v, ve = pt_ve(o, v);
   const size_t es = sizeof(T);
   DevBuf tmp(es * (size_t)o * o * vv);
   // -T2[i,l,a,b] as [a,b,l,i] -> Pe columns v..v+o
@@ -248,11 +247,16 @@ double pt_run(jues_ctx* ctx, PTOperands<T>& S, const double* T2d, int o, int v, 
 // v^3 o scratch) and the Vvooo half of Te (from Vq[c,l,k,j] = Vvooo[c,k,j,l]).
 template <typename T, typename MakeP>
 void pt_pack_integrals(jues_ctx* ctx, PTOperands<T>& S, int o, int v, MakeP make, const T* Vq) {
-  const long vv = (long)v * v, ve = (long)v + o;
+  const long vv = (long)v * v, ve = pt_ve(o, v);
   const size_t es = sizeof(T);
   S.Pe.alloc(es * (size_t)vv * ve * o);
   S.P2e.alloc(es * (size_t)vv * ve * o);
   S.Te.alloc(es * (size_t)v * ve * o * o);
+  if (ve > (long)v + o) {  // the padding columns d' >= v + o stay zero
+    fill<T>(ctx, S.Pe.template as<T>(), vv * ve * o, 0.0);
+    fill<T>(ctx, S.P2e.template as<T>(), vv * ve * o, 0.0);
+    fill<T>(ctx, S.Te.template as<T>(), (long)v * ve * o * o, 0.0);
+  }
   {
     DevBuf tmp(es * (size_t)vv * v * o);
     make(tmp.as<T>(), 0);
