@@ -114,8 +114,8 @@ def main():
                     help="instead of RCCD: UMP2, get_fock and Fock-driven CCSD iterations at o=60 v=340")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one MI355X; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one MI355X only; the iteration is not sharded across GPUs",
+              file=sys.stderr)
         sys.exit(2)
     if args.mp2:
         print(json.dumps(mp2_bench()), flush=True)

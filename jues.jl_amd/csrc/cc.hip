@@ -452,7 +452,7 @@ template void ladder_mm<double>(jues_ctx*, CCState&, const double*, double*, dou
 template void ladder_mm<float>(jues_ctx*, CCState&, const double*, float*, float*, float*, float*);
 
 // Doubles-residual contractions that depend only on the doubles amplitudes `tau2`
-// (t2 for RCCD; the CCSD driver passes its own amplitudes and adds T1 terms).
+// (t2 for RCCD; the CCSD residual passes its own amplitudes and adds T1 terms).
 // Produces W.S (ov, symmetrised) and W.Lp / W.Lm (both ladders, packed).
 template <typename T>
 void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {

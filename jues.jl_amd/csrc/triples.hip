@@ -1,5 +1,5 @@
 // Perturbative triples (T) on the device: PerturbativeTriples.compute_pT
-// (CoupledCluster/PerturbativeTriples.jl:35-138) and its call from the CCSD driver
+// (CoupledCluster/PerturbativeTriples.jl:35-138) and its call after the Fock-driven CCSD
 // (AutoRCCSD.jl:294-300).
 //
 // For each occupied triple i >= j >= k the v^3 intermediate W[a,b,c]
