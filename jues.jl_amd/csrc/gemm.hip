@@ -720,6 +720,210 @@ __global__ void splitk_reduce_kernel(const T* __restrict__ W, int S, long M, lon
 
 }  // namespace
 
+// FP32 NN C = alpha A B + beta C for a short, wide product (M <= 64): a 64x256 tile over four
+// wavefronts of 64x64 side by side (gemm32_nn_kernel's wavefront tile), four workgroups per CU.
+// K must be a multiple of 16.  Slices, swizzle and edge clamping as in gemm32_nn_kernel.
+__global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, float alpha, float beta) {
+  constexpr int BM = 64, BN = 256, BK = 16;
+  __shared__ __attribute__((aligned(16))) float As[2][BM * BK];
+  __shared__ __attribute__((aligned(16))) float Bs[2][BN * BK];
+  const int tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
+  const int M = g.M, N = g.N, K = g.K;
+  const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+  const float* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
+  const float* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
+  float* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
+  int tm, tn;
+  if (!tile_coords_bounded((int)blockIdx.x, (int)gridDim.x, g.tiles_m, g.tiles_n, tm, tn)) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  // A share: row am, 4-k chunk ac; B share: rows bn + 64 j (j < 4), 16-byte chunk bc
+  const int am = tid & 63, ac = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int bn = tid >> 2, bc = tid & 3;
+  float ra[4];
+  f4v rb[4];
+  const int a_voff = 4 * (min(m0 + am, M - 1) - m0);
+  const int nlim = N - 1 - n0, ldbi = (int)ldb;
+  const float* At = A + m0;
+  const float* Bt = B + (long)n0 * ldb;
+  auto load = [&](int k0) {
+    const auto rA = make_rsrc(At + (long)k0 * lda);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ra[u] = bload<float>(rA, a_voff, 4 * (4 * ac + u) * (int)lda);
+    const auto rB = make_rsrc(Bt + k0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      rb[j] = __builtin_bit_cast(
+          f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, 4 * (4 * bc + min(bn + 64 * j, nlim) * ldbi), 0, 0));
+  };
+  auto store = [&](int buf) {
+    *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(ac, am)]) = f4v{ra[0], ra[1], ra[2], ra[3]};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = bn + 64 * j;
+      *reinterpret_cast<f4v*>(&Bs[buf][n * BK + 4 * g32_chunk(bc, n)]) = rb[j];
+    }
+  };
+
+  fvec4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
+  const int fr = lane & 15, fk = lane >> 4;
+  const int nk = K / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    f4v bf[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = wn * 64 + j * 16 + fr;
+      bf[j] = *reinterpret_cast<const f4v*>(&Bs[cur][r * BK + 4 * g32_chunk(fk, r)]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = i * 16 + fr;
+      const f4v a0 = *reinterpret_cast<const f4v*>(&As[cur][r * BK + 4 * g32_chunk(fk, r)]);
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j][s2], a0[s2], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + i * 16 + fr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * 64 + j * 16 + 4 * fk + r;
+        if (n < N) {
+          const long off = (long)m + (long)n * ldc;
+          float v = alpha * acc[i][j][r];
+          if (beta != 0.f) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+    }
+  }
+}
+
+// FP64 NN C = alpha A B + beta C for a short, wide product (M <= 64, e.g. the first quarter
+// of the AO->MO transform, M = o): a 64x256 tile over eight wavefronts of 64x32 side by side
+// (the FP64 kernel's wavefront tile), two workgroups per CU.  K must be a multiple of 16.
+// Slices, swizzle and edge clamping as in gemm64_kernel.
+__global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, double alpha, double beta) {
+  constexpr int BM = 64, BN = 256, BK = 16;
+  __shared__ __attribute__((aligned(16))) double As[2][BM * BK];
+  __shared__ __attribute__((aligned(16))) double Bs[2][BN * BK];
+  const int tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
+  const int M = g.M, N = g.N, K = g.K;
+  const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+  const double* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
+  const double* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
+  double* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
+  int tm, tn;
+  if (!tile_coords_bounded((int)blockIdx.x, (int)gridDim.x, g.tiles_m, g.tiles_n, tm, tn)) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  // A share: row am, 16-byte chunk ka (k = 2 ka, 2 ka + 1); B share: rows bn + 64 j, chunk bc
+  const int am = tid & 63, ka = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int bn = tid >> 3, bc = tid & 7;
+  double ra[2];
+  d2v rb[4];
+  const int a_voff = 8 * (min(m0 + am, M - 1) - m0);
+  // B rows past N load row N - 1 (uniform limit; the offsets are recomputed per load rather
+  // than held in registers)
+  const int nlim = N - 1 - n0, ldbi = (int)ldb;
+  const double* At = A + m0;
+  const double* Bt = B + (long)n0 * ldb;
+  auto load = [&](int k0) {
+    const auto rA = make_rsrc(At + (long)k0 * lda);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (2 * ka + u) * (int)lda);
+    const auto rB = make_rsrc(Bt + k0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      rb[j] = __builtin_bit_cast(
+          d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, 8 * (2 * bc + min(bn + 64 * j, nlim) * ldbi), 0, 0));
+  };
+  auto store = [&](int buf) {
+    *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(ka, am)]) = d2v{ra[0], ra[1]};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = bn + 64 * j;
+      *reinterpret_cast<d2v*>(&Bs[buf][n * BK + 2 * g64_chunk(bc, n)]) = rb[j];
+    }
+  };
+
+  dvec4 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = dvec4{0, 0, 0, 0};
+  const int fr = lane & 15, fk = lane >> 4;
+  const int nk = K / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    d2v bf[2][2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int r = wn * 32 + j * 16 + fr;
+      bf[j][0] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+      bf[j][1] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+    }
+    // (A's fragments are read as they are used: with four wavefronts per SIMD the LDS latency
+    // is hidden, and the registers a one-ahead copy would take are not there)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = i * 16 + fr;
+      d2v a0[2];
+      a0[0] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+      a0[1] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j][s2 >> 1][s2 & 1], a0[s2 >> 1][s2 & 1], acc[i][j],
+                                                           0, 0, 0);
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+
+  // acc[i][j][r]: C(m, n) with m = 16 i + lane % 16, n = n0 + 32 wn + 16 j + lane / 16 + 4 r
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + i * 16 + fr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn * 32 + j * 16 + fk + 4 * r;
+        if (n < N) {
+          const long off = (long)m + (long)n * ldc;
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+    }
+  }
+}
+
 static inline long tiles_m128(long x) { return (x + 127) / 128; }
 
 // Whether a product can run on the high-occupancy NN / NT kernels: A m-contiguous, 128x128
@@ -830,6 +1034,49 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   const double flops = 2.0 * M * N * K * batch;
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
+  // Short, wide NN products (M <= 64): gemm32_sk_kernel / gemm64_sk_kernel, 64x256 tiles; a
+  // transposed A (at most 64 x K, the transform's orbital block) is transposed first
+  if (shape == 1 && ta && !tb && lda == K && batch == 1 && K % 16 == 0 &&
+      (long)sizeof(T) * (256 + 64) * (std::max(M, ldb) + 1) < (1L << 31)) {
+    T* At = static_cast<T*>(scratch(ctx, 18, sizeof(T) * (size_t)(M * K)));
+    permute<T, T>(ctx, A, {K, M}, {1, 0}, At, 1.0, 0.0, name);
+    gemm_impl<T>(ctx, false, tb, M, N, K, alpha, At, M, 0, B, ldb, sB, beta, C, ldc, sC, 1, tag, allow_split);
+    return;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (shape == 1 && !ta && !tb && K % 16 == 0 && 4L * (256 + 64) * (std::max(lda, ldb) + 1) < (1L << 31)) {
+      KTimer timer(ctx, name, flops, bytes);
+      for (int b0 = 0; b0 < batch; b0 += 65535) {
+        const int nb = std::min(batch - b0, 65535);
+        GemmSide<float> p = make_side<float>(M, N, K, reinterpret_cast<const float*>(A + (long)b0 * sA), lda, sA,
+                                             reinterpret_cast<const float*>(B + (long)b0 * sB), ldb, sB,
+                                             reinterpret_cast<float*>(C + (long)b0 * sC), ldc, sC);
+        p.tiles_m = (int)((M + 63) / 64);
+        p.tiles_n = (int)((N + 255) / 256);
+        hipLaunchKernelGGL(gemm32_sk_kernel, dim3((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb), dim3(256), 0,
+                           ctx->stream, p, (float)alpha, (float)beta);
+        JUES_HIP(hipGetLastError());
+      }
+      return;
+    }
+  }
+  if constexpr (sizeof(T) == 8) {
+    if (shape == 1 && !ta && !tb && K % 16 == 0 && 8L * (256 + 64) * (std::max(lda, ldb) + 1) < (1L << 31)) {
+      KTimer timer(ctx, name, flops, bytes);
+      for (int b0 = 0; b0 < batch; b0 += 65535) {
+        const int nb = std::min(batch - b0, 65535);
+        GemmSide<double> p = make_side<double>(M, N, K, reinterpret_cast<const double*>(A + (long)b0 * sA), lda, sA,
+                                               reinterpret_cast<const double*>(B + (long)b0 * sB), ldb, sB,
+                                               reinterpret_cast<double*>(C + (long)b0 * sC), ldc, sC);
+        p.tiles_m = (int)((M + 63) / 64);
+        p.tiles_n = (int)((N + 255) / 256);
+        hipLaunchKernelGGL(gemm64_sk_kernel, dim3((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb), dim3(512), 0,
+                           ctx->stream, p, (double)alpha, (double)beta);
+        JUES_HIP(hipGetLastError());
+      }
+      return;
+    }
+  }
   // FP32 NN and NT: gemm32_nn_kernel (four workgroups per CU); FP64: gemm64_kernel (two
   // 512-thread workgroups per CU)
   if (shape == 0 && occ_kernel_ok<T>(ta, K, lda, ldb)) {
