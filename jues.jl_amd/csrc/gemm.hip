@@ -720,11 +720,13 @@ __global__ void splitk_reduce_kernel(const T* __restrict__ W, int S, long M, lon
 
 }  // namespace
 
-// FP32 NN C = alpha A B + beta C for a short, wide product (M <= 64): a 64x256 tile over four
-// wavefronts of 64x64 side by side (gemm32_nn_kernel's wavefront tile), four workgroups per CU.
-// K must be a multiple of 16.  Slices, swizzle and edge clamping as in gemm32_nn_kernel.
+// FP32 NN C = alpha A B + beta C for a short, wide product (M <= 64): a (16 IM)x256 tile over four
+// wavefronts of (16 IM)x64 side by side (gemm32_nn_kernel's wavefront tile for IM = 4), four
+// workgroups per CU.  K must be a multiple of 16.  Slices, swizzle and edge clamping as in
+// gemm32_nn_kernel.
+template <int IM>
 __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, float alpha, float beta) {
-  constexpr int BM = 64, BN = 256, BK = 16;
+  constexpr int BM = 16 * IM, BN = 256, BK = 16;
   __shared__ __attribute__((aligned(16))) float As[2][BM * BK];
   __shared__ __attribute__((aligned(16))) float Bs[2][BN * BK];
   const int tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
@@ -736,8 +738,10 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
   int tm, tn;
   if (!tile_coords_bounded((int)blockIdx.x, (int)gridDim.x, g.tiles_m, g.tiles_n, tm, tn)) return;
   const int m0 = tm * BM, n0 = tn * BN;
-  // A share: row am, 4-k chunk ac; B share: rows bn + 64 j (j < 4), 16-byte chunk bc
-  const int am = tid & 63, ac = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // A share: row am = lane (lanes below BM), 4-k chunk ac = wavefront; B share: rows bn + 64 j
+  // (j < 4), 16-byte chunk bc
+  const int am = lane, ac = __builtin_amdgcn_readfirstlane(wn);
+  const bool a_share = lane < BM;
   const int bn = tid >> 2, bc = tid & 3;
   float ra[4];
   f4v rb[4];
@@ -746,9 +750,11 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
   const float* At = A + m0;
   const float* Bt = B + (long)n0 * ldb;
   auto load = [&](int k0) {
-    const auto rA = make_rsrc(At + (long)k0 * lda);
+    if (a_share) {
+      const auto rA = make_rsrc(At + (long)k0 * lda);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) ra[u] = bload<float>(rA, a_voff, 4 * (4 * ac + u) * (int)lda);
+      for (int u = 0; u < 4; ++u) ra[u] = bload<float>(rA, a_voff, 4 * (4 * ac + u) * (int)lda);
+    }
     const auto rB = make_rsrc(Bt + k0);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -756,7 +762,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
           f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, 4 * (4 * bc + min(bn + 64 * j, nlim) * ldbi), 0, 0));
   };
   auto store = [&](int buf) {
-    *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(ac, am)]) = f4v{ra[0], ra[1], ra[2], ra[3]};
+    if (a_share) *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(ac, am)]) = f4v{ra[0], ra[1], ra[2], ra[3]};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int n = bn + 64 * j;
@@ -764,9 +770,9 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
     }
   };
 
-  fvec4 acc[4][4];
+  fvec4 acc[IM][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
   const int fr = lane & 15, fk = lane >> 4;
@@ -784,7 +790,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
       bf[j] = *reinterpret_cast<const f4v*>(&Bs[cur][r * BK + 4 * g32_chunk(fk, r)]);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < IM; ++i) {
       const int r = i * 16 + fr;
       const f4v a0 = *reinterpret_cast<const f4v*>(&As[cur][r * BK + 4 * g32_chunk(fk, r)]);
 #pragma unroll
@@ -798,7 +804,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
   }
 
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < IM; ++i) {
     const int m = m0 + i * 16 + fr;
     if (m >= M) continue;
 #pragma unroll
@@ -818,11 +824,12 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
 }
 
 // FP64 NN C = alpha A B + beta C for a short, wide product (M <= 64, e.g. the first quarter
-// of the AO->MO transform, M = o): a 64x256 tile over eight wavefronts of 64x32 side by side
-// (the FP64 kernel's wavefront tile), two workgroups per CU.  K must be a multiple of 16.
-// Slices, swizzle and edge clamping as in gemm64_kernel.
+// of the AO->MO transform, M = o): a (16 IM)x256 tile over eight wavefronts of (16 IM)x32 side by
+// side (the FP64 kernel's wavefront tile for IM = 4), two workgroups per CU.  K must be a
+// multiple of 16.  Slices, swizzle and edge clamping as in gemm64_kernel.
+template <int IM>
 __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, double alpha, double beta) {
-  constexpr int BM = 64, BN = 256, BK = 16;
+  constexpr int BM = 16 * IM, BN = 256, BK = 16;
   __shared__ __attribute__((aligned(16))) double As[2][BM * BK];
   __shared__ __attribute__((aligned(16))) double Bs[2][BN * BK];
   const int tid = threadIdx.x, lane = tid & 63, wn = tid >> 6;
@@ -834,8 +841,10 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
   int tm, tn;
   if (!tile_coords_bounded((int)blockIdx.x, (int)gridDim.x, g.tiles_m, g.tiles_n, tm, tn)) return;
   const int m0 = tm * BM, n0 = tn * BN;
-  // A share: row am, 16-byte chunk ka (k = 2 ka, 2 ka + 1); B share: rows bn + 64 j, chunk bc
-  const int am = tid & 63, ka = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // A share: row am = lane (lanes below BM), 16-byte chunk ka = wavefront (k = 2 ka, 2 ka + 1);
+  // B share: rows bn + 64 j, chunk bc
+  const int am = lane, ka = __builtin_amdgcn_readfirstlane(wn);
+  const bool a_share = lane < BM;
   const int bn = tid >> 3, bc = tid & 7;
   double ra[2];
   d2v rb[4];
@@ -846,9 +855,11 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
   const double* At = A + m0;
   const double* Bt = B + (long)n0 * ldb;
   auto load = [&](int k0) {
-    const auto rA = make_rsrc(At + (long)k0 * lda);
+    if (a_share) {
+      const auto rA = make_rsrc(At + (long)k0 * lda);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (2 * ka + u) * (int)lda);
+      for (int u = 0; u < 2; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (2 * ka + u) * (int)lda);
+    }
     const auto rB = make_rsrc(Bt + k0);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -856,7 +867,7 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
           d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, 8 * (2 * bc + min(bn + 64 * j, nlim) * ldbi), 0, 0));
   };
   auto store = [&](int buf) {
-    *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(ka, am)]) = d2v{ra[0], ra[1]};
+    if (a_share) *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(ka, am)]) = d2v{ra[0], ra[1]};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int n = bn + 64 * j;
@@ -864,9 +875,9 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
     }
   };
 
-  dvec4 acc[4][2];
+  dvec4 acc[IM][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = dvec4{0, 0, 0, 0};
   const int fr = lane & 15, fk = lane >> 4;
@@ -887,7 +898,7 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
     // (A's fragments are read as they are used: with four wavefronts per SIMD the LDS latency
     // is hidden, and the registers a one-ahead copy would take are not there)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < IM; ++i) {
       const int r = i * 16 + fr;
       d2v a0[2];
       a0[0] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
@@ -905,7 +916,7 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
 
   // acc[i][j][r]: C(m, n) with m = 16 i + lane % 16, n = n0 + 32 wn + 16 j + lane / 16 + 4 r
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < IM; ++i) {
     const int m = m0 + i * 16 + fr;
     if (m >= M) continue;
 #pragma unroll
@@ -1051,10 +1062,14 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
         GemmSide<float> p = make_side<float>(M, N, K, reinterpret_cast<const float*>(A + (long)b0 * sA), lda, sA,
                                              reinterpret_cast<const float*>(B + (long)b0 * sB), ldb, sB,
                                              reinterpret_cast<float*>(C + (long)b0 * sC), ldc, sC);
-        p.tiles_m = (int)((M + 63) / 64);
+        const int bm = M <= 48 ? 48 : 64;  // 16-row blocks: 3 or 4
+        p.tiles_m = (int)((M + bm - 1) / bm);
         p.tiles_n = (int)((N + 255) / 256);
-        hipLaunchKernelGGL(gemm32_sk_kernel, dim3((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb), dim3(256), 0,
-                           ctx->stream, p, (float)alpha, (float)beta);
+        const dim3 grid((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb);
+        if (bm == 48)
+          hipLaunchKernelGGL(gemm32_sk_kernel<3>, grid, dim3(256), 0, ctx->stream, p, (float)alpha, (float)beta);
+        else
+          hipLaunchKernelGGL(gemm32_sk_kernel<4>, grid, dim3(256), 0, ctx->stream, p, (float)alpha, (float)beta);
         JUES_HIP(hipGetLastError());
       }
       return;
@@ -1068,10 +1083,14 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
         GemmSide<double> p = make_side<double>(M, N, K, reinterpret_cast<const double*>(A + (long)b0 * sA), lda, sA,
                                                reinterpret_cast<const double*>(B + (long)b0 * sB), ldb, sB,
                                                reinterpret_cast<double*>(C + (long)b0 * sC), ldc, sC);
-        p.tiles_m = (int)((M + 63) / 64);
+        const int bm = M <= 48 ? 48 : 64;  // 16-row blocks: 3 or 4
+        p.tiles_m = (int)((M + bm - 1) / bm);
         p.tiles_n = (int)((N + 255) / 256);
-        hipLaunchKernelGGL(gemm64_sk_kernel, dim3((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb), dim3(512), 0,
-                           ctx->stream, p, (double)alpha, (double)beta);
+        const dim3 grid((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb);
+        if (bm == 48)
+          hipLaunchKernelGGL(gemm64_sk_kernel<3>, grid, dim3(512), 0, ctx->stream, p, (double)alpha, (double)beta);
+        else
+          hipLaunchKernelGGL(gemm64_sk_kernel<4>, grid, dim3(512), 0, ctx->stream, p, (double)alpha, (double)beta);
         JUES_HIP(hipGetLastError());
       }
       return;

@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
                                    # skinny tiles (64x256 / 256x64): full tiles, ragged edges, K not a slice multiple
                                    (40, 9000, 256), (64, 8192, 33), (9000, 40, 300), (8195, 17, 64),
                                    # short, wide FP64 products: the 64x256 kernel, a transposed A copied first
-                                   (17, 9000, 64), (33, 10000, 96), (64, 16400, 512), (1, 8192, 16)])
+                                   (17, 9000, 64), (33, 10000, 96), (64, 16400, 512), (1, 8192, 16),
+                                   (33, 9001, 272), (49, 8200, 160)])
 def test_gemm(ctx, prec, tol, ta, tb, M, N, K):
     rng = np.random.default_rng(M * 7 + N * 3 + K)
     A = rng.normal(size=(K, M) if ta == "T" else (M, K))
