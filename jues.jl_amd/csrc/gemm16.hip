@@ -265,17 +265,51 @@ __global__ __launch_bounds__(256) void splitk16_reduce_kernel(const float* __res
 
 constexpr int MAX_PART = 1024;
 
+// 16 bytes of Ti, loaded by one global_load_dwordx4
+template <typename Ti>
+struct alignas(16) Vec16 {
+  Ti x[16 / sizeof(Ti)];
+};
+
+// per-block maxima of |X|.  A contiguous X (ldx == rows) is swept flat, with 16-byte loads
+// when `vec` (X 16-byte aligned), four in flight per thread; otherwise each block takes whole
+// columns.  No per-element index division.
 template <typename Ti>
 __global__ __launch_bounds__(256) void maxabs_kernel(const Ti* __restrict__ X, long rows, long cols, long ldx,
-                                                      double* __restrict__ part) {
-  const long total = rows * cols;
-  double m = 0.0;
-  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256L) {
-    const long r = e % rows, c = e / rows;
-    m = fmax(m, fabs((double)X[r + c * ldx]));
+                                                      int vec, double* __restrict__ part) {
+  Ti m = Ti(0);
+  const long stride = (long)gridDim.x * 256L;
+  if (ldx == rows) {
+    const long total = rows * cols;
+    long done = 0;
+    if (vec) {
+      constexpr int W = 16 / sizeof(Ti);
+      const long nv = total / W;
+      const Vec16<Ti>* Xv = reinterpret_cast<const Vec16<Ti>*>(X);
+      long e = blockIdx.x * 256L + threadIdx.x;
+      for (; e + 3 * stride < nv; e += 4 * stride) {
+        Vec16<Ti> q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = Xv[e + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int w = 0; w < W; ++w) m = fmax(m, fabs(q[u].x[w]));
+      }
+      for (; e < nv; e += stride) {
+        const Vec16<Ti> q = Xv[e];
+#pragma unroll
+        for (int w = 0; w < W; ++w) m = fmax(m, fabs(q.x[w]));
+      }
+      done = nv * W;
+    }
+    for (long e = done + blockIdx.x * 256L + threadIdx.x; e < total; e += stride) m = fmax(m, fabs(X[e]));
+  } else {
+    for (long c = blockIdx.x; c < cols; c += gridDim.x)
+      for (long r = threadIdx.x; r < rows; r += 256) m = fmax(m, fabs(X[r + c * ldx]));
   }
   __shared__ double red[256];
-  red[threadIdx.x] = m;
+  red[threadIdx.x] = (double)m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
@@ -334,6 +368,49 @@ __global__ __launch_bounds__(256) void split_kernel(const Ti* __restrict__ X, lo
   }
 }
 
+// the same planes for tall operands (rows_p / 8 >= 192): each block takes whole columns, each
+// thread 8 consecutive rows of one, read with 16-byte loads when `vec` (X 16-byte aligned and
+// ldx * sizeof(Ti) a multiple of 16) and the 8 rows are inside X.  Columns cols..cols_p-1 and
+// rows rows..rows_p-1 are written as zeros.
+template <typename Ti>
+__global__ __launch_bounds__(256) void split_cols_kernel(const Ti* __restrict__ X, long rows, long cols, long ldx,
+                                                          long rows_p, long cols_p, int vec,
+                                                          const int* __restrict__ sexp, _Float16* __restrict__ P) {
+  constexpr int W = 16 / sizeof(Ti);
+  const int k = sexp[0];
+  const int r8 = (int)(rows_p / 8);
+  for (long c = blockIdx.x; c < cols_p; c += gridDim.x) {
+    _Float16* Pc = P + c * 2 * rows_p;
+    for (int rq = threadIdx.x; rq < r8; rq += 256) {
+      const long r0 = 8L * rq;
+      Ti x[8];
+      if (c < cols && vec && r0 + 8 <= rows) {
+        const Vec16<Ti>* src = reinterpret_cast<const Vec16<Ti>*>(X + c * ldx + r0);
+#pragma unroll
+        for (int u = 0; u < 8 / W; ++u) {
+          const Vec16<Ti> q = src[u];
+#pragma unroll
+          for (int w = 0; w < W; ++w) x[u * W + w] = q.x[w];
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) x[t] = (c < cols && r0 + t < rows) ? X[c * ldx + r0 + t] : Ti(0);
+      }
+      h8 vh, vl;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const Ti y = (Ti)ldexp(x[t], k);
+        const _Float16 h = (_Float16)(float)y;
+        vh[t] = h;
+        vl[t] = (_Float16)(float)(y - (Ti)h);
+      }
+      const long off = (rq / 4) * 64 + (rq % 4) * 8;
+      *reinterpret_cast<h8*>(Pc + off) = vh;
+      *reinterpret_cast<h8*>(Pc + off + 32) = vl;
+    }
+  }
+}
+
 // planes of X^T: X is stored cols x rows (leading dimension ldx), the planes are
 // rows x cols.  64 x 64 tiles pass through LDS so that both the reads of X (along
 // its columns' contiguous index) and the 16-byte plane stores are coalesced.
@@ -373,6 +450,7 @@ __global__ __launch_bounds__(256) void split_t_kernel(const Ti* __restrict__ X, 
 }
 
 int grid_for(long n, int cap) { return (int)std::max(1L, std::min((n + 255) / 256, (long)cap)); }
+int aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
 }  // namespace
 
@@ -409,11 +487,18 @@ void split16(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split16
   } else {
     double* part = static_cast<double*>(scratch(ctx, 3, MAX_PART * sizeof(double)));
     const int nb = grid_for(n, MAX_PART);
-    hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, rows, cols, ldx, part);
+    hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, rows, cols, ldx, aligned16(X),
+                       part);
     hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
   }
-  hipLaunchKernelGGL((split_kernel<Ti>), dim3(grid_for(s.rows_p / 8 * s.cols_p, 131072)), dim3(256), 0, ctx->stream, X,
-                     rows, cols, ldx, s.rows_p, s.cols_p, s.sexp, s.p);
+  if (s.rows_p / 8 >= 192) {
+    const int vec = aligned16(X) && (ldx * (long)sizeof(Ti)) % 16 == 0;
+    hipLaunchKernelGGL((split_cols_kernel<Ti>), dim3((unsigned)std::min(s.cols_p, 65536L)), dim3(256), 0, ctx->stream,
+                       X, rows, cols, ldx, s.rows_p, s.cols_p, vec, s.sexp, s.p);
+  } else {
+    hipLaunchKernelGGL((split_kernel<Ti>), dim3(grid_for(s.rows_p / 8 * s.cols_p, 131072)), dim3(256), 0, ctx->stream,
+                       X, rows, cols, ldx, s.rows_p, s.cols_p, s.sexp, s.p);
+  }
   JUES_HIP(hipGetLastError());
 }
 
@@ -427,7 +512,7 @@ void split16t(jues_ctx* ctx, const Ti* X, long rows, long cols, long ldx, Split1
   KTimer t(ctx, tag ? tag : "split16", 0.0, 2.0 * sizeof(Ti) * n + 4.0 * s.rows_p * s.cols_p);
   const int nb = grid_for(n, MAX_PART);
   // the stored matrix is cols x rows
-  hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, cols, rows, ldx, part);
+  hipLaunchKernelGGL((maxabs_kernel<Ti>), dim3(nb), dim3(256), 0, ctx->stream, X, cols, rows, ldx, aligned16(X), part);
   hipLaunchKernelGGL(split_scale_kernel, dim3(1), dim3(256), 0, ctx->stream, part, nb, s.sexp);
   const dim3 grid((unsigned)((s.rows_p + 63) / 64), (unsigned)(s.cols_p / 64));
   hipLaunchKernelGGL((split_t_kernel<Ti>), grid, dim3(256), 0, ctx->stream, X, rows, cols, ldx, s.rows_p, s.sexp,

@@ -217,10 +217,13 @@ class Context:
         k = self.lib.jues_kernel_stats(self.h, names, ptr(ms), ptr(calls), ptr(fl), ptr(by), cap)
         return {names[i].decode(): dict(ms=ms[i], calls=calls[i], flops=fl[i], bytes=by[i]) for i in range(k)}
 
-    def gemm_test(self, prec, ta, tb, alpha, A, B, beta, Cm):
+    def gemm_test(self, prec, ta, tb, alpha, A, B, beta, Cm, K=None):
+        """C = alpha op(A) op(B) + beta C with leading dimensions A.shape[0], B.shape[0]; an
+        explicit K uses only the first K rows of a transposed A or untransposed B."""
         A, B, Cm = f64(A), f64(B), f64(Cm).copy(order="F")
         M, N = Cm.shape
-        K = A.shape[0] if ta == "T" else A.shape[1]
+        if K is None:
+            K = A.shape[0] if ta == "T" else A.shape[1]
         self.check(self.lib.jues_gemm_test(self.h, prec, ta.encode(), tb.encode(), M, N, K, alpha, ptr(A),
                                            A.shape[0], ptr(B), B.shape[0], beta, ptr(Cm), Cm.shape[0]))
         return Cm

@@ -98,3 +98,19 @@ def test_gemm16_transposed_split(ctx, ta, tb, M, N, K):
     ref = 0.5 * (opA @ opB) + 0.25 * C0
     out = ctx.gemm_test(FP16X3, ta, tb, 0.5, A, B, 0.25, C0)
     assert _normwise(out, ref, opA.T, opB, 0.5, 0.25, C0) < 4e-6
+
+
+@pytest.mark.parametrize("K,ld", [(3000, 3003), (3000, 3004), (2001, 2004), (1600, 1601)])
+def test_gemm16_tall_operands_with_padded_leading_dimension(ctx, K, ld):
+    """Tall operands (K >= 1536 rows) take the column-blocked split: leading dimensions above
+    the row count, odd ones (no 16-byte loads) and a last group of 8 rows cut by the end."""
+    rng = np.random.default_rng(K + ld)
+    M, N = 50, 70
+    A, B, C0 = rng.normal(size=(ld, M)), rng.normal(size=(ld, N)), rng.normal(size=(M, N))
+    A[K:] = np.nan  # rows beyond K are not part of the operands
+    B[K:] = np.nan
+    alpha, beta = 1.1, 0.4
+    ref = alpha * (A[:K].T @ B[:K]) + beta * C0
+    out = ctx.gemm_test(FP16X3, "T", "N", alpha, A, B, beta, C0, K=K)
+    assert np.all(np.isfinite(out))
+    assert _normwise(out, ref, A[:K], B[:K], alpha, beta, C0) < 4e-6
