@@ -10,8 +10,10 @@
 // never formed):
 //   R = <ij|ab> + S[ia,jb] + sum_mn t[mn,ab] W'[mn,ij] + sum_ef t[ij,ef] <ab|ef>
 //   W' = 2 Wmnij - <mn|ij> = <mn|ij> + sum_ef <mn|ef> t[ij,ef]
-//   S = H + H^T with H = U WJ + T WE + (Tx WE)^x + t.Fae^T - Fmi^T.t (ov layout)
-//   WJ = K + K U/2 - Kx T/2,  WE = -J + Kx Tx/2,  U = 2T - Tx
+//   S = H + H^T with H = U WJ + T WE + (Tx WE)^x (ov layout)
+//   WJ = K + K U/2 - Kx T/2,  WE = -J + Kx Tx/2 + D/2,  U = 2T - Tx
+//   D[kc,jb] = delta_kj Fae[b,c] - delta_cb Fmi[k,j]  (T D + (Tx D)^x + transposes = the
+//   Fock-like terms t.Fae^T - Fmi^T.t and their transposes)
 // where K[ia,jb] = (ia|jb), Kx[ia,jb] = (ib|ja), J[ia,jb] = (ij|ab),
 // T[ia,jb] = t[i,j,a,b], Tx[ia,jb] = t[i,j,b,a] and X^x[ia,jb] = X[ib,ja].
 #include "cc.h"
@@ -486,7 +488,7 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   permute<T, T>(ctx, Upp, d4, {2, 0, 1, 3}, Ua, 1.0, 0.0, "cc_prep");     // U[a,m,n,f]
 
   // Fae[a,e] = -sum U[a,(mnf)] K[e,(mnf)];  Fmi[m,i] = sum K[m,(nef)] U[i,(nef)], kept transposed
-  // (Fmi^T[i,m]) so that its product with t below is a plain NN GEMM
+  // (Fmi^T[i,m], the product of two row-major views)
   gemm<T>(ctx, false, true, v, v, oo * v, T(-1), Ua, v, 0, I.Ka.template as<T>(), v, 0, zero, Fae, v, 0, 1, "cc_fock");
   gemm<T>(ctx, false, true, o, o, ov * v, one, Upp, o, 0, I.Kpp.template as<T>(), o, 0, zero, Fmi, o, 0, 1, "cc_fock");
   // particle-particle ladder on the packed (anti)symmetric parts, then the hole-hole
@@ -502,13 +504,14 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, one, WJ);
   permute<T, T>(ctx, I.Jov.template as<T>(), {ov, ov}, {0, 1}, WE, -1.0, 0.0, "cc_prep");
   ring_mm<T>(ctx, S, half, Kx, SP_KX, Ttov, SP_TTOV, one, WE);
+  // Fock-like terms t.Fae^T - Fmi^T.t, carried by WE: with D[kc,jb] = delta_kj Fae[b,c] -
+  // delta_cb Fmi[k,j], T D and (Tx D)^x each give half of them, up to a transpose that
+  // S = H + H^T absorbs
+  fock_dress<T>(ctx, WE, Fae, Fmi, o, v, half);
   ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, H);
   ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
-  // Fock-like terms: H += t[ia j, e] Fae[b,e] ; H -= Fmi[m,i] t[m, a jb]
-  gemm<T>(ctx, false, true, oo * v, v, v, one, Tov, oo * v, 0, Fae, v, 0, one, H, oo * v, 0, 1, "cc_fock");
-  gemm<T>(ctx, false, false, o, ov * v, o, T(-1), Fmi, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
   // H += Y^x ; S = H + H^T (into WJ)
   permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
   JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));

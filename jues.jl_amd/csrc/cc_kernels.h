@@ -78,6 +78,9 @@ void pack_whh(jues_ctx* ctx, const PairTables& P, const T* W, int o, T* WPT, T* 
 // Ttov[i,a,j,b] = t[i,j,b,a], Uov[i,a,j,b] = U[i,j,a,b]
 template <typename T>
 void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, T* Tov, T* Ttov, T* Uov);
+// WE[(k,c),(j,b)] += s (delta_kj Fae[b,c] - delta_cb Fmi[k,j]) (ov layout; FmiT[j,k] = Fmi[k,j])
+template <typename T>
+void fock_dress(jues_ctx* ctx, T* WE, const T* Fae, const T* FmiT, int o, int v, T s);
 // Tq = Tp with its (a,a) columns doubled
 template <typename T>
 void pair_diag2(jues_ctx* ctx, const PairTables& P, const T* Tp, T* Tq);

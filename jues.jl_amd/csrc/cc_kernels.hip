@@ -112,6 +112,33 @@ __global__ __launch_bounds__(256) void pair_diag2_kernel(const T* __restrict__ T
   }
 }
 
+// WE[(k,c),(j,b)] += s (delta_kj Fae[b,c] - delta_cb Fmi[k,j]) on the ov layout (row k + o c,
+// column j + o b), with Fmi held transposed (FmiT[j,k] = Fmi[k,j]).  Threads x < o v^2 take
+// the (k,c,b) entries with j = k (and the Fmi part where c = b); the rest take (k, j != k, b)
+// with c = b, so every entry has one writer.
+template <typename T>
+__global__ __launch_bounds__(256) void fock_dress_kernel(T* __restrict__ WE, const T* __restrict__ Fae,
+                                                         const T* __restrict__ FmiT, int o, int v, T s) {
+  const long ov = (long)o * v, n1 = ov * v, n2 = (long)o * (o - 1) * v;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < n1 + n2; x += (long)gridDim.x * 256L) {
+    if (x < n1) {
+      const int k = (int)(x % o);
+      const long cb = x / o;
+      const int c = (int)(cb % v), b = (int)(cb / v);
+      T d = Fae[b + (long)v * c];
+      if (c == b) d -= FmiT[k + (long)o * k];
+      WE[k + (long)o * c + ov * (k + (long)o * b)] += s * d;
+    } else {
+      const long y = x - n1;
+      const int k = (int)(y % o);
+      const long jb = y / o;
+      const int jj = (int)(jb % (o - 1)), b = (int)(jb / (o - 1));
+      const int j = jj < k ? jj : jj + 1;
+      WE[k + (long)o * b + ov * (j + (long)o * b)] -= s * FmiT[j + (long)o * k];
+    }
+  }
+}
+
 // The RCCD residual's amplitude layouts from t2 (FP64, [i,j,a,b]) in one pass:
 // Tpp = t, Upp = 2t - t^(ij) ([i,j,a,b]); Tov[i,a,j,b] = t[ijab]; Uov[i,a,j,b] = U[ijab];
 // Ttov[i,b,j,a] = t[ijab] (that is t[i,j,b,a] at [i,a,j,b]).  Threads run i fastest, so
@@ -411,6 +438,14 @@ void pair_diag2(jues_ctx* ctx, const PairTables& P, const T* Tp, T* Tq) {
 }
 
 template <typename T>
+void fock_dress(jues_ctx* ctx, T* WE, const T* Fae, const T* FmiT, int o, int v, T s) {
+  const long n = (long)o * v * v + (long)o * (o - 1) * v;
+  KTimer k(ctx, "cc_fock", 0.0, (double)n * 3.0 * sizeof(T));
+  hipLaunchKernelGGL((fock_dress_kernel<T>), dim3(grid256(n)), dim3(256), 0, ctx->stream, WE, Fae, FmiT, o, v, s);
+  JUES_HIP(hipGetLastError());
+}
+
+template <typename T>
 void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, T* Tov, T* Ttov, T* Uov) {
   const long n = (long)o * o * v * v;
   KTimer k(ctx, "cc_prep", 0.0, (double)n * (16.0 + 5.0 * sizeof(T)));
@@ -469,6 +504,8 @@ void divide_d2(jues_ctx* ctx, double* t, const double* eps, int o, int v, double
   JUES_HIP(hipGetLastError());
 }
 
+template void fock_dress<double>(jues_ctx*, double*, const double*, const double*, int, int, double);
+template void fock_dress<float>(jues_ctx*, float*, const float*, const float*, int, int, float);
 template void rccd_layouts<double>(jues_ctx*, const double*, int, int, double*, double*, double*, double*, double*);
 template void rccd_layouts<float>(jues_ctx*, const double*, int, int, float*, float*, float*, float*, float*);
 template void pack_whh<double>(jues_ctx*, const PairTables&, const double*, int, double*, double*);

@@ -5,7 +5,8 @@
 //   tau  = t2 + t1 t1    in the hole-hole and particle-particle ladders,
 //   tau' = t2 + t1 t1/2  in Fae / Fmi,
 //   WJ   = K + K (U - 2 tt^x)/2 - Kx T/2 + T1 terms,
-//   WE   = -J + Kx (Tx + 2 tt^x)/2 + T1 terms,      tt^x[ia,jb] = t1[ib] t1[ja],
+//   WE   = -J + Kx (Tx + 2 tt^x)/2 + T1 terms + D/2, tt^x[ia,jb] = t1[ib] t1[ja],
+//   D[kc,jb] = delta_kj Fae[b,c] - delta_cb Fmi[k,j] (the Fock-like terms, as in RCCD),
 // Wabef's T1 terms become  -sum_m t1[mb] Z[ij,ma] + (ia<->jb)  with
 // Z[ij,ma] = sum_ef tau[ij,ef] (mf|ae); the remaining T1 terms are small
 // contractions done by the TTGT engine.  Integral labels below: occupied
@@ -194,15 +195,15 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   ring_mm<T>(ctx, S, half, Kx, SP_KX, Q, SP_Q, one, WE);
   contract<T>(ctx, -1.0, TN(t1, "jf"), TN(ovvv, "mfbe"), 1.0, TN(WE, "mejb"), "cc_t1");
   contract<T>(ctx, 1.0, TN(t1, "nb"), TN(ooov, "mjne"), 1.0, TN(WE, "mejb"), "cc_t1");
+  // Fock-like terms with Fb / Fj, carried by WE as in RCCD: WE += D/2 with
+  // D[kc,jb] = delta_kj Fb[b,c] - delta_cb Fj[k,j]
+  T* FmiT = bufp_t<T>(W.FmiT, oo);
+  permute<T, T>(ctx, Fmi, {o, o}, {1, 0}, FmiT, 1.0, 0.0, "cc_fock");
+  fock_dress<T>(ctx, WE, Fae, FmiT, o, v, half);
   ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE, SP_WE, zero, H);
   ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
-  // Fock-like terms with Fb / Fj
-  gemm<T>(ctx, false, true, oo * v, v, v, one, Tov, oo * v, 0, Fae, v, 0, one, H, oo * v, 0, 1, "cc_fock");
-  T* FmiT = bufp_t<T>(W.FmiT, oo);
-  permute<T, T>(ctx, Fmi, {o, o}, {1, 0}, FmiT, 1.0, 0.0, "cc_fock");
-  gemm<T>(ctx, false, false, o, ov * v, o, T(-1), FmiT, o, 0, Tov, o, 0, one, H, o, 0, 1, "cc_fock");
   permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
   // T1^2 terms: -t1[ie] t1[ma] (me|jb) and -t1[ie] t1[mb] (mj|ae)
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(Kov, "mejb"), 0.0, TN(Q, "imjb"), "cc_t1");
