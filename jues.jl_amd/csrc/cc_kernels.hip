@@ -164,6 +164,37 @@ __global__ __launch_bounds__(256) void rccd_layouts_kernel(const double* __restr
   }
 }
 
+// The same layouts with the o x o slab t[:,:,a,b] of one (a,b) pair staged in LDS (rows padded to
+// o + 1), so that the exchanged read t[j,i,a,b] comes from LDS and every global read is
+// contiguous.  Dynamic LDS: o (o + 1) doubles.
+template <typename T>
+__global__ __launch_bounds__(256) void rccd_layouts_slab_kernel(const double* __restrict__ t, int o, int v,
+                                                                T* __restrict__ Tpp, T* __restrict__ Upp,
+                                                                T* __restrict__ Tov, T* __restrict__ Ttov,
+                                                                T* __restrict__ Uov) {
+  extern __shared__ double slab[];  // slab[i + (o + 1) j] = t[i,j,a,b]
+  const int oo = o * o, ld = o + 1;
+  const long ov = (long)o * v, vv = (long)v * v;
+  for (long ab = blockIdx.x; ab < vv; ab += gridDim.x) {
+    const int a = (int)(ab % v), b = (int)(ab / v);
+    const double* src = t + ab * oo;
+    __syncthreads();  // the previous pair's readers are done with the slab
+    for (int q = threadIdx.x; q < oo; q += 256) slab[q % o + ld * (q / o)] = src[q];
+    __syncthreads();
+    for (int q = threadIdx.x; q < oo; q += 256) {
+      const int i = q % o, j = q / o;
+      const double tv = slab[i + ld * j];
+      const double u = 2.0 * tv - slab[j + ld * i];
+      Tpp[ab * oo + q] = (T)tv;
+      Upp[ab * oo + q] = (T)u;
+      const long y = i + (long)o * a + ov * (j + (long)o * b);
+      Tov[y] = (T)tv;
+      Uov[y] = (T)u;
+      Ttov[i + (long)o * b + ov * (j + (long)o * a)] = (T)tv;
+    }
+  }
+}
+
 __device__ inline double block_sum256(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   __syncthreads();
@@ -449,8 +480,15 @@ template <typename T>
 void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, T* Tov, T* Ttov, T* Uov) {
   const long n = (long)o * o * v * v;
   KTimer k(ctx, "cc_prep", 0.0, (double)n * (16.0 + 5.0 * sizeof(T)));
-  hipLaunchKernelGGL((rccd_layouts_kernel<T>), dim3(grid256(n)), dim3(256), 0, ctx->stream, t, o, v, Tpp, Upp, Tov,
-                     Ttov, Uov);
+  const size_t slab = (size_t)o * (o + 1) * sizeof(double);
+  if (slab <= 65536) {
+    const long vv = (long)v * v;
+    hipLaunchKernelGGL((rccd_layouts_slab_kernel<T>), dim3((unsigned)std::min(vv, 1L << 20)), dim3(256), slab,
+                       ctx->stream, t, o, v, Tpp, Upp, Tov, Ttov, Uov);
+  } else {
+    hipLaunchKernelGGL((rccd_layouts_kernel<T>), dim3(grid256(n)), dim3(256), 0, ctx->stream, t, o, v, Tpp, Upp, Tov,
+                       Ttov, Uov);
+  }
   JUES_HIP(hipGetLastError());
 }
 
