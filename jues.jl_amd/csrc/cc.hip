@@ -10,7 +10,8 @@
 // never formed):
 //   R = <ij|ab> + S[ia,jb] + sum_mn t[mn,ab] W'[mn,ij] + sum_ef t[ij,ef] <ab|ef>
 //   W' = 2 Wmnij - <mn|ij> = <mn|ij> + sum_ef <mn|ef> t[ij,ef]
-//   S = H + H^T with H = U WJ + T WE + (Tx WE)^x (ov layout)
+//   S = H + H^T with H = U WJ + T WE + (Tx WE)^x (ov layout; the update kernel sums it from
+//   the two products' outputs)
 //   WJ = K + K U/2 - Kx T/2,  WE = -J + Kx Tx/2 + D/2,  U = 2T - Tx
 //   D[kc,jb] = delta_kj Fae[b,c] - delta_cb Fmi[k,j]  (T D + (Tx D)^x + transposes = the
 //   Fock-like terms t.Fae^T - Fmi^T.t and their transposes)
@@ -512,10 +513,13 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
-  // H += Y^x ; S = H + H^T (into WJ)
-  permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
-  JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  permute<T, T>(ctx, H, {ov, ov}, {1, 0}, WJ, 1.0, 1.0, "cc_sym");
+  // S = H + Y^x + (H + Y^x)^T: summed by the update kernel, or here (into WJ) when its LDS slab
+  // would not fit
+  if (!fused_sym(o, sizeof(T))) {
+    permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
+    JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    permute<T, T>(ctx, H, {ov, ov}, {1, 0}, WJ, 1.0, 1.0, "cc_sym");
+  }
 }
 
 template void rccd_contractions<double>(jues_ctx*, CCState&, const double*);
@@ -573,7 +577,12 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
   A.eps = S.eps.as<double>();
   A.Kpp = S.i64->Kpp.as<double>();
   A.Lener = S.Lener.as<double>();
-  A.S = W.WJ.template as<T>();
+  if (fused_sym(o, sizeof(T))) {
+    A.H = W.H.template as<T>();
+    A.Y = W.Y.template as<T>();
+  } else {
+    A.S = W.WJ.template as<T>();
+  }
   A.Lp = W.Lp.template as<T>();
   A.Lm = W.Lm.template as<T>();
   A.extra = S.method == 1 ? W.X2.template as<T>() : nullptr;

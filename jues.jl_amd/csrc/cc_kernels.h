@@ -25,7 +25,9 @@ struct UpdateArgs {
   const double* eps;
   const double* Kpp;    // <ij|ab> (FP64, ijab)
   const double* Lener;  // 2<ij|ab> - <ji|ab> (FP64, ijab)
-  const TR* S;          // ring + Fock-like terms, symmetrised, ov layout [i,a,j,b]
+  const TR* S;          // ring + Fock-like terms, symmetrised, ov layout [i,a,j,b]; or null with:
+  const TR* H;          //   H and Y (ov layout): S[ia,jb] = H[ia,jb] + Y[ib,ja] + H[jb,ia] + Y[ja,ib],
+  const TR* Y;          //   summed by the update kernel itself (fused_sym)
   const TR* Lp;         // packed ladders (pp + hh), symmetric part
   const TR* Lm;         // packed ladders (pp + hh), antisymmetric part
   const TR* extra;      // any further ijab-ordered residual terms (RCCSD), may be null
@@ -89,6 +91,9 @@ template <typename T>
 void pack_v_ea(jues_ctx* ctx, const PairTables& P, const T* X, int v, int f0, int f1, T* Vp, T* Vm);
 template <typename TR>
 void t2_update(jues_ctx* ctx, const UpdateArgs<TR>& A);
+// whether the update kernel sums S = H + H^T + Y^x + (Y^x)^T itself: its o x (o + 1) LDS slab
+// of element size `elem` must fit in 48 KiB
+inline bool fused_sym(int o, size_t elem) { return (size_t)o * (o + 1) * elem <= 48 * 1024; }
 void t1_update(jues_ctx* ctx, const T1UpdateArgs& A, int nblocks);
 void reduce_rows(jues_ctx* ctx, const double* part, long PB, long count, int nq, double* out);
 void diis_solve(jues_ctx* ctx, double* small, const DiisOrder& ord);

@@ -221,11 +221,27 @@ __global__ __launch_bounds__(256) void t2_update_kernel(UpdateArgs<TR> A) {
   double sd[MAX_DIIS];
 #pragma unroll
   for (int k = 0; k < MAX_DIIS; ++k) sd[k] = 0.0;
+  // fused symmetrisation: the transposed terms H[jb,ia] + Y[ja,ib] run along j in memory, so
+  // they are read as contiguous j-runs into an LDS slab[j + (o + 1) i] and taken from there
+  extern __shared__ __attribute__((aligned(16))) unsigned char dyn_lds[];
+  TR* slab = reinterpret_cast<TR*>(dyn_lds);
+  const int ld = o + 1;
+  if (A.H) {
+    for (int q = threadIdx.x; q < (int)oo; q += 256) {
+      const int j = q % o, i = q / o;
+      slab[j + ld * i] = A.H[(long)j + (long)o * b + ov * ((long)i + (long)o * a)] +
+                         A.Y[(long)j + (long)o * a + ov * ((long)i + (long)o * b)];
+    }
+    __syncthreads();
+  }
   for (int ij = threadIdx.x; ij < (int)oo; ij += 256) {
     const int i = ij % o, j = ij / o;
     const long x = (long)ij + oo * ab;
     double R = A.Kpp[x];
     if (A.S) R += (double)A.S[(long)i + (long)o * a + ov * ((long)j + (long)o * b)];
+    if (A.H)
+      R += (double)A.H[(long)i + (long)o * a + ov * ((long)j + (long)o * b)] +
+           (double)A.Y[(long)i + (long)o * b + ov * ((long)j + (long)o * a)] + (double)slab[j + ld * i];
     if (A.Lp) {
       const long pp = i <= j ? (long)i + (long)j * (j + 1) / 2 : (long)j + (long)i * (i + 1) / 2;
       R += (double)A.Lp[pp + A.npo * qp];
@@ -509,8 +525,13 @@ template <typename TR>
 void t2_update(jues_ctx* ctx, const UpdateArgs<TR>& A) {
   const long v2 = (long)A.v * A.v;
   const long n2 = (long)A.o * A.o * v2;
-  KTimer k(ctx, "cc_update", 0.0, (double)n2 * (8.0 * (5 + A.nold) + 4 * sizeof(TR)));
-  hipLaunchKernelGGL((t2_update_kernel<TR>), dim3((unsigned)v2), dim3(256), 0, ctx->stream, A);
+  KTimer k(ctx, "cc_update", 0.0, (double)n2 * (8.0 * (5 + A.nold) + (A.H ? 5 : 1) * sizeof(TR)));
+  size_t lds = 0;
+  if (A.H) {
+    JUES_REQUIRE(A.Y && !A.S && fused_sym(A.o, sizeof(TR)), "t2_update: fused symmetrisation misconfigured");
+    lds = (size_t)A.o * (A.o + 1) * sizeof(TR);
+  }
+  hipLaunchKernelGGL((t2_update_kernel<TR>), dim3((unsigned)v2), dim3(256), lds, ctx->stream, A);
   JUES_HIP(hipGetLastError());
 }
 

@@ -204,7 +204,6 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
-  permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
   // T1^2 terms: -t1[ie] t1[ma] (me|jb) and -t1[ie] t1[mb] (mj|ae)
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(Kov, "mejb"), 0.0, TN(Q, "imjb"), "cc_t1");
   contract<T>(ctx, -1.0, TN(t1, "ma"), TN(Q, "imjb"), 1.0, TN(H, "iajb"), "cc_t1");
@@ -213,9 +212,13 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   // T1-linear terms: t1[ie] (jb|ae) - t1[ma] (mi|jb)
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(ovvv, "jbae"), 1.0, TN(H, "iajb"), "cc_t1");
   contract<T>(ctx, -1.0, TN(t1, "ma"), TN(ooov, "mijb"), 1.0, TN(H, "iajb"), "cc_t1");
-  // S = H + H^T
-  JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  permute<T, T>(ctx, H, {ov, ov}, {1, 0}, WJ, 1.0, 1.0, "cc_sym");
+  // S = H + Y^x + (H + Y^x)^T: summed by the update kernel, or here (into WJ) when its LDS slab
+  // would not fit
+  if (!fused_sym(o, sizeof(T))) {
+    permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
+    JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    permute<T, T>(ctx, H, {ov, ov}, {1, 0}, WJ, 1.0, 1.0, "cc_sym");
+  }
 }
 
 template void rccsd_contractions<double>(jues_ctx*, CCState&, const double*, const double*);
