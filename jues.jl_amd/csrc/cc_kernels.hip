@@ -495,8 +495,9 @@ void fock_dress(jues_ctx* ctx, T* WE, const T* Fae, const T* FmiT, int o, int v,
 template <typename T>
 void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, T* Tov, T* Ttov, T* Uov) {
   const long n = (long)o * o * v * v;
-  KTimer k(ctx, "cc_prep", 0.0, (double)n * (16.0 + 5.0 * sizeof(T)));
   const size_t slab = (size_t)o * (o + 1) * sizeof(double);
+  // t is read once through the LDS slab, twice (direct and exchanged) otherwise
+  KTimer k(ctx, "cc_prep", 0.0, (double)n * ((slab <= 65536 ? 8.0 : 16.0) + 5.0 * sizeof(T)));
   if (slab <= 65536) {
     const long vv = (long)v * v;
     hipLaunchKernelGGL((rccd_layouts_slab_kernel<T>), dim3((unsigned)std::min(vv, 1L << 20)), dim3(256), slab,
@@ -525,7 +526,8 @@ template <typename TR>
 void t2_update(jues_ctx* ctx, const UpdateArgs<TR>& A) {
   const long v2 = (long)A.v * A.v;
   const long n2 = (long)A.o * A.o * v2;
-  KTimer k(ctx, "cc_update", 0.0, (double)n2 * (8.0 * (5 + A.nold) + (A.H ? 5 : 1) * sizeof(TR)));
+  // S (or H and Y, each read twice), Lp, Lm and extra in the residual's compute type
+  KTimer k(ctx, "cc_update", 0.0, (double)n2 * (8.0 * (5 + A.nold) + (A.H ? 7 : 4) * sizeof(TR)));
   size_t lds = 0;
   if (A.H) {
     JUES_REQUIRE(A.Y && !A.S && fused_sym(A.o, sizeof(TR)), "t2_update: fused symmetrisation misconfigured");
