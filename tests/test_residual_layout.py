@@ -127,3 +127,86 @@ def test_fock_terms_are_carried_by_we():
     t = _amplitudes(s.o, s.v, 105)
     ref = O.rccd_residual_tensor(t, I)
     assert np.max(np.abs(_device_residual(t, I, fock_in_we=False) - ref)) > 1e-4 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------- RCCSD
+
+def _device_rccsd(t1, t2, B, Co, Cv):
+    """The device's RCCSD residual sequence (mRCCSD equations), restated with chemist blocks
+    g_xxxx[p,q,r,s] = (pq|rs): tau = t2 + t1 t1, tau' = t2 + t1 t1/2; Fae / Fmi from tau' plus
+    their T1 terms, Fme; the T1 residual; Fb = Fae - t1.Fme/2, Fj = Fmi + t1.Fme/2; both ladders
+    on tau (W' with its T1 terms); Wabef's T1 part -sum_m t1[mb] Z[ij,ma] + (ia<->jb); the ring
+    with U2 = U - 2 tt^x, Q = Tx + 2 tt^x and the T1 terms of WJ / WE, the Fock-like terms in
+    WE; the T1^2 and T1-linear terms; S summed as the update kernel does."""
+    g = lambda a, b, c, d: O.mo_from_factors(B, a, b, c, d)
+    ovov, oovv, ovvv, ooov = g(Co, Cv, Co, Cv), g(Co, Co, Cv, Cv), g(Co, Cv, Cv, Cv), g(Co, Co, Co, Cv)
+    oooo_p = g(Co, Co, Co, Co).transpose(0, 2, 1, 3)  # <mn|ij>
+    vvvv_p = g(Cv, Cv, Cv, Cv).transpose(0, 2, 1, 3)  # <ab|ef>
+    o, v = t1.shape
+    es = np.einsum
+    mm = lambda A, Bm: es("iakc,kcjb->iajb", A, Bm)
+    Kpp = ovov.transpose(0, 2, 1, 3)  # <ij|ab> = (ia|jb), [i,j,a,b]
+    Kov = ovov                        # (ia|jb), [i,a,j,b]
+    Kx = ovov.transpose(0, 3, 2, 1)   # (ib|ja), [i,a,j,b]
+    Jov = oovv.transpose(0, 2, 1, 3)  # (ij|ab), [i,a,j,b]
+    tt = es("ia,jb->ijab", t1, t1)
+    tau, taup = t2 + tt, t2 + 0.5 * tt
+    ttx = es("ib,ja->iajb", t1, t1)   # tt^x[ia,jb] = t1[ib] t1[ja]
+    Upp = 2 * taup - taup.transpose(1, 0, 2, 3)
+    Tov = t2.transpose(0, 2, 1, 3)
+    Ttov = es("ijba->iajb", t2)
+    Uov = 2 * Tov - Ttov
+    Fae = -es("mnef,mnaf->ae", Kpp, Upp)
+    Fmi = es("mnef,inef->mi", Kpp, Upp)
+    Fae += 2 * es("mf,mfae->ae", t1, ovvv) - es("mf,maef->ae", t1, ovvv)
+    Fmi += 2 * es("ne,mine->mi", t1, ooov) - es("ne,nime->mi", t1, ooov)
+    Fme = 2 * es("nf,menf->me", t1, Kov) - es("nf,nemf->me", t1, Kov)
+    n1 = (es("ie,ae->ia", t1, Fae) - es("ma,mi->ia", t1, Fmi) + es("me,iame->ia", Fme, Uov)
+          + 2 * es("me,iame->ia", t1, Kov) - es("me,maie->ia", t1, Jov)
+          - 2 * es("mane,mine->ia", Tov, ooov) + es("mane,nime->ia", Tov, ooov)
+          + 2 * es("iemf,mfae->ia", Tov, ovvv) - es("iemf,meaf->ia", Tov, ovvv))
+    Fae = Fae - 0.5 * es("mb,me->be", t1, Fme)
+    Fmi = Fmi + 0.5 * es("je,me->mj", t1, Fme)
+    # ladders on tau
+    Wp = oooo_p + es("je,mine->mnij", t1, ooov) + es("ie,njme->mnij", t1, ooov)
+    Wp = Wp + es("mnef,ijef->mnij", Kpp, tau)
+    L = es("ijef,abef->ijab", tau, vvvv_p) + es("mnab,mnij->ijab", tau, Wp)
+    Z = es("ijef,mfae->ijma", tau, ovvv)
+    Q = -es("mb,ijma->ijab", t1, Z)
+    X2 = Q + Q.transpose(1, 0, 3, 2)
+    # ring
+    Qr = Ttov + 2 * ttx
+    U2 = Uov - 2 * ttx
+    WJ = Kov + 0.5 * mm(Kov, U2) - 0.5 * mm(Kx, Tov)
+    WJ = WJ + es("jf,mebf->mejb", t1, ovvv) - es("nb,njme->mejb", t1, ooov)
+    WE = -Jov + 0.5 * mm(Kx, Qr)
+    WE = WE - es("jf,mfbe->mejb", t1, ovvv) + es("nb,mjne->mejb", t1, ooov)
+    D = np.zeros((o, v, o, v))
+    for k in range(o):
+        D[k, :, k, :] += Fae.T
+    for b in range(v):
+        D[:, b, :, b] -= Fmi
+    WE = WE + 0.5 * D
+    H = mm(Tov, WE) + mm(Uov, WJ)
+    Y = mm(Ttov, WE)
+    Qa = es("ie,mejb->imjb", t1, Kov)
+    H = H - es("ma,imjb->iajb", t1, Qa)
+    Qb = es("ie,maje->imaj", t1, Jov)
+    H = H - es("mb,imaj->iajb", t1, Qb)
+    H = H + es("ie,jbae->iajb", t1, ovvv) - es("ma,mijb->iajb", t1, ooov)
+    S = H + Y.transpose(0, 3, 2, 1) + H.transpose(2, 3, 0, 1) + Y.transpose(2, 1, 0, 3)
+    n2 = Kpp + S.transpose(0, 2, 1, 3) + L + X2
+    return n1, n2
+
+
+@pytest.mark.parametrize("n,o,naux,seed", [(9, 3, 14, 8), (8, 2, 12, 9)])
+def test_device_rccsd_sequence_matches_oracle(n, o, naux, seed):
+    s = synthetic.make_system(n, o, n - o, naux, seed, name="layout")
+    I = O.rccsd_integrals_from_factors(s.B, s.Co, s.Cv)
+    rng = np.random.default_rng(seed + 200)
+    t1 = rng.normal(size=(s.o, s.v)) * 0.05
+    t2 = _amplitudes(s.o, s.v, seed + 300)
+    r1, r2 = O.rccsd_update(t1, t2, I)
+    d1, d2 = _device_rccsd(t1, t2, s.B, s.Co, s.Cv)
+    assert np.max(np.abs(d1 - r1)) < 1e-12 * max(1.0, np.max(np.abs(r1)))
+    assert np.max(np.abs(d2 - r2)) < 1e-12 * max(1.0, np.max(np.abs(r2)))
