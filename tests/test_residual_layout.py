@@ -210,3 +210,45 @@ def test_device_rccsd_sequence_matches_oracle(n, o, naux, seed):
     d1, d2 = _device_rccsd(t1, t2, s.B, s.Co, s.Cv)
     assert np.max(np.abs(d1 - r1)) < 1e-12 * max(1.0, np.max(np.abs(r1)))
     assert np.max(np.abs(d2 - r2)) < 1e-12 * max(1.0, np.max(np.abs(r2)))
+
+
+# ---------------------------------------------------------------- recomputed ladder
+
+@pytest.mark.parametrize("nf", [1, 2, 4, 6])
+def test_recomputed_ladder_slabs_equal_stored_ladder(nf):
+    """The ladder with <ab|ef> rebuilt from the factors one slab of f at a time: per slab,
+    X[e,a,b,f-f0] = sum_P Lt[ea,P] L[bf,P] = (ae|bf) (L[ae,P] the MO factors, rows a + v e;
+    Lt the same with a and e exchanged), its rows of V+- (pairs e<=f / e<f with f in the slab)
+    are ((ae|bf) +- (af|be))/2, and they are the contiguous K-range f0(f0+1)/2 ... of the
+    packed product T+- V+-.  Summed over slabs this is the stored-V ladder.  (The factors of
+    real orbitals are symmetric in a, e, so L and Lt hold the same numbers here: the test pins
+    the slab offsets and the packing, not which of the two the device reads.)"""
+    s = synthetic.make_system(9, 3, 6, 14, 10, name="layout")
+    o, v, B, Cv = s.o, s.v, s.B, s.Cv
+    t = _amplitudes(o, v, 410)
+    vvvv = O.mo_from_factors(B, Cv, Cv, Cv, Cv).transpose(0, 2, 1, 3)  # <ab|ef>
+    ole, olt, vle, vlt = _pairs(o, False), _pairs(o, True), _pairs(v, False), _pairs(v, True)
+    Tp = np.array([[t[ij + ef] if ef[0] == ef[1] else t[ij + ef] + t[ij + ef[::-1]] for ef in vle] for ij in ole])
+    Tm = np.array([[t[ij + ef] - t[ij + ef[::-1]] for ef in vlt] for ij in olt])
+    Vp = np.array([[0.5 * (vvvv[ab + ef] + vvvv[ab + ef[::-1]]) for ab in vle] for ef in vle])
+    Vm = np.array([[0.5 * (vvvv[ab + ef] - vvvv[ab + ef[::-1]]) for ab in vlt] for ef in vlt])
+    L3 = np.einsum("ma,mnP,ne->aeP", Cv, B, Cv)
+    naux = L3.shape[2]
+    L = L3.reshape(v * v, naux, order="F")                      # row a + v e
+    Lt = L3.transpose(1, 0, 2).reshape(v * v, naux, order="F")  # row e + v a holds L[a,e]
+    Lp = np.zeros((len(ole), len(vle)))
+    Lm = np.zeros((len(olt), len(vlt)))
+    for f0 in range(0, v, nf):
+        f1 = min(v, f0 + nf)
+        X = (Lt @ L[v * f0:v * f1].T).reshape(v, v, v, f1 - f0, order="F")  # X[e,a,b,f-f0]
+        r0p, r1p = f0 * (f0 + 1) // 2, f1 * (f1 + 1) // 2
+        r0m, r1m = f0 * (f0 - 1) // 2, f1 * (f1 - 1) // 2
+        Vps = np.array([[0.5 * (X[e, a, b, f - f0] + X[e, b, a, f - f0]) for (a, b) in vle]
+                        for (e, f) in vle[r0p:r1p]])
+        Lp += Tp[:, r0p:r1p] @ Vps
+        if r1m > r0m:
+            Vms = np.array([[0.5 * (X[e, a, b, f - f0] - X[e, b, a, f - f0]) for (a, b) in vlt]
+                            for (e, f) in vlt[r0m:r1m]])
+            Lm += Tm[:, r0m:r1m] @ Vms
+    assert np.max(np.abs(Lp - Tp @ Vp)) < 1e-12 * np.max(np.abs(Tp @ Vp))
+    assert np.max(np.abs(Lm - Tm @ Vm)) < 1e-12 * np.max(np.abs(Tm @ Vm))
