@@ -155,10 +155,16 @@ jues_status jues_tei_transform(jues_ctx* ctx, int prec,
 
 /* ---- IntegralTransformation.get_eri (Backend/IntegralTransformation.jl:38-101) --
  * eri_string: 4 chars from {o,O,v,V}; notation: "phys" (default when NULL) or "chem".
- * Cocc (n x o), Cvir (n x v); fcn drops the first fcn occupied columns.  out has the
- * dimensions the string names, in the requested notation. */
+ * As the reference (:52-68), upper-case letters take the alpha orbitals, O -> Cao
+ * (n x nalpha) and V -> Cav (n x nvira), lower-case letters the beta ones, o -> Cbo
+ * (n x nbeta) and v -> Cbv (n x nvirb); for a restricted reference pass the alpha arrays
+ * twice.  A block the string does not name may be NULL.  fcn drops the first fcn
+ * columns of either occupied block.  out has the dimensions the letters name (nalpha -
+ * fcn, nbeta - fcn, nvira or nvirb), in the requested notation.  prec: JUES_FP64 or
+ * JUES_FP32_MIXED (quarter GEMMs in FP32). */
 jues_status jues_get_eri(jues_ctx* ctx, int prec, const char* eri_string, const char* notation,
-                         const double* Cocc, int o, const double* Cvir, int v, int n, int fcn,
+                         const double* Cao, int nalpha, const double* Cav, int nvira,
+                         const double* Cbo, int nbeta, const double* Cbv, int nvirb, int n, int fcn,
                          double* out);
 
 /* ---- MollerPlesset.do_rmp2 (MollerPlesset/RMP2.jl:11-45) ----------------------
@@ -168,10 +174,14 @@ jues_status jues_rmp2(jues_ctx* ctx, int prec, const double* Cocc, int o, const 
 
 /* MP2 adjoint (no reference counterpart; the reverse rule of jues_rmp2 for a
  * ChainRulesCore rrule).  C is the full n x n matrix [Cocc Cvir] (o occupied
- * columns first); eps is held fixed.  dE_dC (n x n) and dE_dg (n^4, only with a
- * dense AO tensor) may be NULL.  Every entry of C and g is an independent variable. */
+ * columns first), eps the n orbital energies (occupied first).  Outputs, each may be
+ * NULL: dE_dC (n x n), dE_dg (n^4, only with a dense AO tensor) and dE_deps (n).
+ * Every entry of C, g and eps is an independent variable.  With AO factors the
+ * gradient takes the factor route (dL = 2 G L through L = Co^T B_P Cv; no n^2 o v
+ * intermediate); with a dense AO tensor the half-transforms are recomputed. */
 jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, int o, int n,
-                           const double* eps, double* E, double* dE_dC, double* dE_dg);
+                           const double* eps, double* E, double* dE_dC, double* dE_dg,
+                           double* dE_deps);
 
 /* ---- CoupledCluster.{RCCD,mRCCD}.do_rccd (RCCD.jl:33-83, mRCCD.jl:38-118) ------
  * Converges the closed-shell CCD equations of RCCD.jl:134-303.  T2_inout (o x o x v x v,

@@ -141,25 +141,35 @@ jues_status jues_tei_transform(jues_ctx* ctx, int prec, const double* C1, int d1
   JUES_API_END(ctx)
 }
 
-jues_status jues_get_eri(jues_ctx* ctx, int prec, const char* eri_string, const char* notation, const double* Cocc,
-                         int o, const double* Cvir, int v, int n, int fcn, double* out) {
+jues_status jues_get_eri(jues_ctx* ctx, int prec, const char* eri_string, const char* notation, const double* Cao,
+                         int nalpha, const double* Cav, int nvira, const double* Cbo, int nbeta, const double* Cbv,
+                         int nvirb, int n, int fcn, double* out) {
   JUES_API_BEGIN(ctx)
   JUES_REQUIRE(eri_string && std::strlen(eri_string) == 4,
                std::string("Invalid string given to get_eri: ") + (eri_string ? eri_string : "(null)"));
   const std::string note = notation ? notation : "phys";
   JUES_REQUIRE(note == "phys" || note == "chem", "get_eri: notation must be \"phys\" or \"chem\"");
-  JUES_REQUIRE(Cocc && Cvir && out && o > 0 && v > 0 && n > 0, "get_eri: bad arguments");
-  JUES_REQUIRE(fcn >= 0 && fcn < o, "get_eri: bad fcn");
+  JUES_REQUIRE(out, "get_eri: null output");
+  JUES_REQUIRE(n == ctx->n && n > 0, "get_eri: n does not match the AO integrals");
+  JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED, "get_eri: bad precision");
+  JUES_REQUIRE(fcn >= 0, "get_eri: bad fcn");
   std::string s(eri_string);
   for (char c : s) JUES_REQUIRE(std::strchr("oOvV", c), "Invalid string given to get_eri: " + s);
   std::string chem = s;
   if (note == "phys") chem = std::string{s[0], s[2], s[1], s[3]};
+  // IntegralTransformation.jl:52-68: O -> Cao (nalpha), o -> Cbo (nbeta), V -> Cav (nvira),
+  // v -> Cbv (nvirb); fcn drops the first fcn columns of either occupied block
   const double* Cs[4];
   int ds[4];
   for (int k = 0; k < 4; ++k) {
-    const bool occ = chem[k] == 'o' || chem[k] == 'O';
-    Cs[k] = occ ? Cocc + (size_t)fcn * n : Cvir;
-    ds[k] = occ ? o - fcn : v;
+    const char c = chem[k];
+    const double* C = c == 'O' ? Cao : c == 'o' ? Cbo : c == 'V' ? Cav : Cbv;
+    const int d = c == 'O' ? nalpha : c == 'o' ? nbeta : c == 'V' ? nvira : nvirb;
+    const bool occ = c == 'o' || c == 'O';
+    JUES_REQUIRE(C && d > 0, std::string("get_eri: no orbitals for '") + c + "'");
+    JUES_REQUIRE(!occ || fcn < d, "get_eri: bad fcn");
+    Cs[k] = occ ? C + (size_t)fcn * n : C;
+    ds[k] = occ ? d - fcn : d;
   }
   const size_t total = (size_t)ds[0] * ds[1] * ds[2] * ds[3];
   DevBuf r64(total * sizeof(double));
@@ -206,7 +216,7 @@ jues_status jues_rmp2(jues_ctx* ctx, int prec, const double* Cocc, int o, const 
     tei_transform_dev<float>(ctx, co.p, o, cv.p, v, co.p, o, cv.p, v, K.as<float>());
     *E = mp2_energy_dev<float>(ctx, K.as<float>(), e.p, o, v);
   }
-  JUES_REQUIRE(std::isfinite(*E), "jues_rmp2: non-finite energy");
+  if (!std::isfinite(*E)) throw Error(JUES_NONFINITE, "jues_rmp2: non-finite energy (a zero denominator?)");
   JUES_API_END(ctx)
 }
 

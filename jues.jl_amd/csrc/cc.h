@@ -19,8 +19,11 @@ struct CCInts {
   DevBuf Vp, Vm;  // packed pp-ladder integrals
   DevBuf KP, KM;  // packed hole-hole ladder integrals (pair sums / differences of <kl|cd>)
   DevBuf ooov_c;  // (ij|ka) at [i,j,k,a]      (RCCSD)
-  DevBuf ovvv_c;  // (ia|bc) at [i,a,b,c]      (RCCSD)
-  DevBuf Lvv;     // MO factors L[ae,P] (v^2 x naux) when the ladder is recomputed
+  DevBuf ovvv_c;  // (ia|bc) at [i,a,b,c]      (RCCSD with dense AO integrals)
+  // RCCSD with AO factors: (ia|bc) = sum_P Lov[i,a,P] Lvv[b,c,P] is never stored; the
+  // o v^3 terms contract these factors (LovT[a,i,P] = Lov[i,a,P], Lovp[i,P,a] = Lov[i,a,P])
+  DevBuf Lov, LovT, Lovp;
+  DevBuf Lvv;     // MO factors L[ae,P] (v^2 x naux) when the ladder is recomputed or (ia|bc) factored
   DevBuf Lvvt;    // the same with a and e exchanged: L[ea,P]
   // off-diagonal Fock blocks of a non-canonical reference (AutoRCCSD.jl:215-232):
   // f_mi at [m,i], f_ae at [a,e] (diagonals zero), f_ia at [i,a]
@@ -34,6 +37,14 @@ struct CCWork {
   DevBuf Xs, Vps, Vms;  // recomputed ladder: (ae|bf) slab and its packed V+/V- rows
   // RCCSD
   DevBuf t1, Fme, X2, Z, Q, n1, U2, Tau, FmiT;
+  // RCCSD o v^3 terms through the factors: B1[m,P,e] / Y2[m,e,P] = sum_f t1[m,f] Lvv[e,f,P],
+  // Y3[m,j,P] = sum_f Lov[m,f,P] t1[j,f], W1[i,e,P], W2[i,f,P], g[P]; Zs / Zsp: slabs of (mf|ae)
+  DevBuf B1, Y2, Y3, W1, W2, gP, Zs, Zsp;
+  // where the last residual evaluation left H, Y (ov layout) and, when the update kernel does
+  // not symmetrise them itself, S = H + Y^x + (H + Y^x)^T (the buffers may be shared)
+  T* out_H = nullptr;
+  T* out_Y = nullptr;
+  T* out_S = nullptr;
 };
 
 // Split-FP16 operand planes of the fast stage.  Slots below SPLIT_CONST hold
@@ -69,7 +80,10 @@ struct CCState {
   DevBuf vals, errs, small;  // DIIS storage
   std::vector<int> active;   // DIIS slots, oldest first
   DevBuf R1;       // FP64 T1 residual numerator (RCCSD)
-  DevBuf tau64, taup64, ttx64;  // FP64 helpers (RCCSD)
+  bool ovvv_factored = false;  // RCCSD: the (ia|bc) terms through the factors (no o v^3 block)
+  DevBuf Co, Cv;               // device orbitals (frozen core removed), to rebuild FP64 blocks
+  bool i64_thin = false;       // FP64 blocks other than <ij|ab> released during a fast stage
+  int last_stage = -1;         // precision of the last cc_run stage
   std::unique_ptr<CCInts<double>> i64;
   std::unique_ptr<CCInts<float>> i32;
   std::unique_ptr<CCWork<double>> w64;
@@ -109,9 +123,10 @@ void ring_mm(jues_ctx* ctx, CCState& S, T alpha, const T* A, int slotA, const T*
 // Wp: [m,n,i,j], oooo plus any T1 terms.
 template <typename T>
 void hh_mm(jues_ctx* ctx, CCState& S, const T* Wp, const T* Tp, const T* Tm, T* Lp, T* Lm);
-// Packed pp-ladder Lp/Lm from FP64 amplitudes `amp` (ijab); Tp/Tm are workspaces.
+// Packed pp-ladder Lp/Lm from FP64 amplitudes `amp` (ijab), or of tau = amp + t1 t1 when t1
+// (o x v) is given; Tp/Tm are workspaces.
 template <typename T>
-void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm);
+void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm, const double* t1 = nullptr);
 
 template <typename T>
 void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2);

@@ -121,13 +121,17 @@ bool ladder_recompute_mode(jues_ctx* ctx, const CCState& S, int mode, int prec) 
 // ---------------------------------------------------------------------------
 // integral setup
 // ---------------------------------------------------------------------------
-void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv) {
+// FP64 MO integral blocks.  With rebuild set only the blocks of S.i64 are (re)made, from the
+// orbitals kept in S; the energy weights and the guess are left alone.
+void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv, bool rebuild = false) {
   const int o = S.o, v = S.v;
   const long ov = (long)o * v, N2 = ov * ov, oo = (long)o * o;
   auto& I = *S.i64;
   double* Kov = bufp<double>(I.Kov, N2);
-  double* Jc = static_cast<double*>(scratch(ctx, 7, N2 * sizeof(double)));
-  double* Oc = static_cast<double*>(scratch(ctx, 8, oo * oo * sizeof(double)));
+  // setup-only blocks, freed on return: (ij|ab) at [i,j,a,b] and (mi|nj) at [m,i,n,j]
+  DevBuf Jc_buf(N2 * sizeof(double)), Oc_buf(oo * oo * sizeof(double));
+  double* Jc = Jc_buf.as<double>();
+  double* Oc = Oc_buf.as<double>();
   if (ctx->gao.p) {
     tei_transform_dev<double>(ctx, Co, o, Cv, v, Co, o, Cv, v, Kov);   // (ia|jb) -> [i,a,j,b]
     tei_transform_dev<double>(ctx, Co, o, Co, o, Cv, v, Cv, v, Jc);    // (ij|ab) -> [i,j,a,b]
@@ -145,13 +149,18 @@ void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv
     gemm<double>(ctx, false, true, oo, oo, naux, 1.0, Loo.as<double>(), oo, 0, Loo.as<double>(), oo, 0, 0.0, Oc, oo,
                  0, 1, "setup_ints");
     if (S.method == 1) {
-      // (ij|ka) -> ooov chem [i,j,k,a];  (ia|bc) -> ovvv chem [i,a,b,c]
+      // (ij|ka) -> ooov chem [i,j,k,a]; (ia|bc) stays factored: Lov, its two permutations and Lvv
       double* ooov = bufp<double>(I.ooov_c, oo * ov);
-      double* ovvv = bufp<double>(I.ovvv_c, ov * v * v);
       gemm<double>(ctx, false, true, oo, ov, naux, 1.0, Loo.as<double>(), oo, 0, Lov.as<double>(), ov, 0, 0.0, ooov,
                    oo, 0, 1, "setup_ints");
-      gemm<double>(ctx, false, true, ov, (long)v * v, naux, 1.0, Lov.as<double>(), ov, 0, Lvv.as<double>(),
-                   (long)v * v, 0, 0.0, ovvv, ov, 0, 1, "setup_ints");
+      S.ovvv_factored = true;
+      permute<double, double>(ctx, Lov.as<double>(), {o, v, naux}, {1, 0, 2}, bufp<double>(I.LovT, ov * naux), 1.0,
+                              0.0, "setup_ints");
+      permute<double, double>(ctx, Lov.as<double>(), {o, v, naux}, {0, 2, 1}, bufp<double>(I.Lovp, ov * naux), 1.0,
+                              0.0, "setup_ints");
+      sync(ctx);
+      I.Lov = std::move(Lov);
+      if (!S.ladder_recompute) I.Lvv = std::move(Lvv);
     }
   }
   if (ctx->gao.p && S.method == 1) {
@@ -167,11 +176,13 @@ void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv
   pack_khh(ctx, S.P, I.Kpp.as<double>(), o, v, bufp<double>(I.KP, S.P.npo * S.P.npv),
            bufp<double>(I.KM, std::max(1L, S.P.nmo * S.P.nmv)));
   // energy weights 2<ij|ab> - <ji|ab>
-  double* Le = bufp<double>(S.Lener, N2);
-  permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {0, 1, 2, 3}, Le, 2.0, 0.0);
-  permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {1, 0, 2, 3}, Le, -1.0, 1.0);
+  if (!rebuild) {
+    double* Le = bufp<double>(S.Lener, N2);
+    permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {0, 1, 2, 3}, Le, 2.0, 0.0);
+    permute<double, double>(ctx, I.Kpp.as<double>(), {o, o, v, v}, {1, 0, 2, 3}, Le, -1.0, 1.0);
+  }
   // guess helpers: (ij|ab) in ijab order
-  if (S.opts.guess == JUES_GUESS_OVOV) {
+  if (!rebuild && S.opts.guess == JUES_GUESS_OVOV) {
     double* g = bufp<double>(S.guess_buf, N2);
     JUES_HIP(hipMemcpyAsync(g, Jc, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   }
@@ -203,6 +214,9 @@ void cc_make_ints32(jues_ctx* ctx, CCState& S) {
   // Vp/Vm and KP/KM are converted by the first FP32 (non-split) ladder products
   convert_buf<float>(ctx, a.ooov_c, b.ooov_c);
   convert_buf<float>(ctx, a.ovvv_c, b.ovvv_c);
+  convert_buf<float>(ctx, a.Lov, b.Lov);
+  convert_buf<float>(ctx, a.LovT, b.LovT);
+  convert_buf<float>(ctx, a.Lovp, b.Lovp);
   convert_buf<float>(ctx, a.Lvv, b.Lvv);
   convert_buf<float>(ctx, a.Lvvt, b.Lvvt);
   convert_buf<float>(ctx, a.fOO, b.fOO);
@@ -254,6 +268,9 @@ const Split16& split_operand(jues_ctx* ctx, CCState& S, int slot, const Ti* X, l
   SplitState& Z = split_state(S);
   SplitState::Entry& e = Z.e[slot];
   const long stamp = slot < SPLIT_CONST ? (long)S.iter : -1;
+  // a constant operand, once split, stays valid for the stage (its FP64 source may since have
+  // been released to make room)
+  if (stamp == -1 && e.stamp == -1 && e.src) return e.buf.s;
   if (e.src != static_cast<const void*>(X) || e.stamp != stamp) {
     const SplitState::Entry& tov = Z.e[SP_TOV];
     const int* same = slot == SP_TTOV && !trans && tov.stamp == stamp && tov.src ? tov.buf.s.sexp : nullptr;
@@ -293,13 +310,14 @@ void ring_mm<float>(jues_ctx* ctx, CCState& S, float alpha, const float* A, int 
 // sum_P L[ea,P] L[bf,P] = (ae|bf); its rows of V+/V- (pairs e<=f / e<f with f in the slab)
 // are a contiguous K-range of the ladder product, accumulated into Lp / Lm.  V+/V- never exist.
 template <typename T>
-static void ladder_recompute_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm) {
+static void ladder_recompute_mm(jues_ctx* ctx, CCState& S, const double* amp, const double* t1, T* Tp, T* Tm, T* Lp,
+                                T* Lm) {
   const PairTables& P = S.P;
   const int v = S.v;
   const long v2 = (long)v * v, naux = ctx->naux;
   const bool minus = P.nmo > 0 && P.nmv > 0;
   CCWork<T>& W = work<T>(S);
-  pack_t<T>(ctx, P, amp, S.o, v, Tp, Tm);
+  pack_t<T>(ctx, P, amp, S.o, v, Tp, Tm, t1);
   // slabs of about 4 GiB of (ae|bf) (JUES_LADDER_SLAB_BYTES overrides, e.g. to test many slabs)
   long slab_bytes = 4L << 30;
   if (const char* env = std::getenv("JUES_LADDER_SLAB_BYTES")) slab_bytes = std::max(1L, std::atol(env));
@@ -352,9 +370,9 @@ static void ladder_recompute_mm(jues_ctx* ctx, CCState& S, const double* amp, T*
 }
 
 template <typename T>
-void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm) {
+void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp, T* Lm, const double* t1) {
   if (S.ladder_recompute) {
-    ladder_recompute_mm<T>(ctx, S, amp, Tp, Tm, Lp, Lm);
+    ladder_recompute_mm<T>(ctx, S, amp, t1, Tp, Tm, Lp, Lm);
     return;
   }
   const PairTables& P = S.P;
@@ -363,7 +381,7 @@ void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp
     if (S.split_on) {
       // Lp[(ij),(ab)] = sum_(ef) Tp[(ij),(ef)] Vp[(ef),(ab)]: the planes of Tp^T are the
       // k-contiguous left operand; Vp's planes come straight from the FP64 integrals
-      pack_t<float>(ctx, P, amp, S.o, S.v, Tp, Tm);
+      pack_t<float>(ctx, P, amp, S.o, S.v, Tp, Tm, t1);
       const Split16& tp = split_operand<float>(ctx, S, SP_TPT, Tp, P.npv, P.npo, true);
       const Split16& vp = split_operand<double>(ctx, S, SP_VP, S.i64->Vp.as<double>(), P.npv, P.npv);
       gemm16_tn(ctx, P.npo, P.npv, 1.0f, tp, vp, 0.0f, Lp, P.npo, "cc_pp_ladder");
@@ -381,7 +399,7 @@ void ladder_mm(jues_ctx* ctx, CCState& S, const double* amp, T* Tp, T* Tm, T* Lp
     }
   }
   CCInts<T>& I = ints<T>(S);
-  pack_t<T>(ctx, P, amp, S.o, S.v, Tp, Tm);
+  pack_t<T>(ctx, P, amp, S.o, S.v, Tp, Tm, t1);
   // (one launch per product: at over six rounds of workgroups each, a shared launch measured no faster)
   gemm<T>(ctx, false, false, P.npo, P.npv, P.npv, T(1), Tp, P.npo, 0, I.Vp.template as<T>(), P.npv, 0, T(0), Lp,
           P.npo, 0, 1, "cc_pp_ladder");
@@ -451,8 +469,9 @@ void hh_mm(jues_ctx* ctx, CCState& S, const T* Wp, const T* Tp, const T* Tm, T* 
 template void hh_mm<double>(jues_ctx*, CCState&, const double*, const double*, const double*, double*, double*);
 template void hh_mm<float>(jues_ctx*, CCState&, const float*, const float*, const float*, float*, float*);
 
-template void ladder_mm<double>(jues_ctx*, CCState&, const double*, double*, double*, double*, double*);
-template void ladder_mm<float>(jues_ctx*, CCState&, const double*, float*, float*, float*, float*);
+template void ladder_mm<double>(jues_ctx*, CCState&, const double*, double*, double*, double*, double*,
+                                const double*);
+template void ladder_mm<float>(jues_ctx*, CCState&, const double*, float*, float*, float*, float*, const double*);
 
 // Doubles-residual contractions that depend only on the doubles amplitudes `tau2`
 // (t2 for RCCD; the CCSD residual passes its own amplitudes and adds T1 terms).
@@ -520,6 +539,9 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
     JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
     permute<T, T>(ctx, H, {ov, ov}, {1, 0}, WJ, 1.0, 1.0, "cc_sym");
   }
+  W.out_H = H;
+  W.out_Y = Y;
+  W.out_S = WJ;
 }
 
 template void rccd_contractions<double>(jues_ctx*, CCState&, const double*);
@@ -578,10 +600,10 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
   A.Kpp = S.i64->Kpp.as<double>();
   A.Lener = S.Lener.as<double>();
   if (fused_sym(o, sizeof(T))) {
-    A.H = W.H.template as<T>();
-    A.Y = W.Y.template as<T>();
+    A.H = W.out_H;
+    A.Y = W.out_Y;
   } else {
-    A.S = W.WJ.template as<T>();
+    A.S = W.out_S;
   }
   A.Lp = W.Lp.template as<T>();
   A.Lm = W.Lm.template as<T>();
@@ -708,6 +730,8 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   S.ladder_recompute = ladder_recompute_mode(ctx, S, opts.ladder, prec);
   S.i64.reset(new CCInts<double>());
   cc_build_ints(ctx, S, co.p, cv.p);
+  S.Co = std::move(co.buf);  // kept to rebuild released FP64 blocks
+  S.Cv = std::move(cv.buf);
   if (method == 1) cc_build_ccsd_ints(ctx, S);
   if (fock) {
     // fock is the (o + v) x (o + v) MO Fock matrix over [Cocc | Cvir]; frozen core dropped
@@ -778,6 +802,31 @@ double cc_energy(jues_ctx* ctx, CCState& S) {
   return S.lambda * E;
 }
 
+// In a fast stage the FP64 integral blocks are read only by the update kernel (<ij|ab>) once
+// the stage's FP32 copies and split planes exist, so when device memory is short (below 15% of
+// the device free after the stage's first iteration, or JUES_THIN_FP64_INTS=1) the others are
+// released; the next stage switch rebuilds them from the orbitals kept in S.
+static void thin_fp64_ints(jues_ctx* ctx, CCState& S) {
+  if (S.i64_thin) return;
+  size_t free_b = 0, total_b = 0;
+  JUES_HIP(hipMemGetInfo(&free_b, &total_b));
+  const char* env = std::getenv("JUES_THIN_FP64_INTS");
+  const bool force = env && std::atoi(env) != 0;
+  if (!force && (double)free_b >= 0.15 * (double)total_b) return;
+  sync(ctx);
+  CCInts<double>& I = *S.i64;
+  for (DevBuf* b : {&I.Kov, &I.Ktov, &I.Jov, &I.Ka, &I.oooo, &I.Vp, &I.Vm, &I.KP, &I.KM, &I.ooov_c, &I.ovvv_c,
+                    &I.Lov, &I.LovT, &I.Lovp, &I.Lvv, &I.Lvvt})
+    b->release();
+  S.i64_thin = true;
+}
+
+static void ensure_fp64_ints(jues_ctx* ctx, CCState& S) {
+  if (!S.i64_thin) return;
+  cc_build_ints(ctx, S, S.Co.as<double>(), S.Cv.as<double>(), true);
+  S.i64_thin = false;
+}
+
 void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_conv, jues_iter_info* last,
             bool* converged, jues_iter_cb cb, void* user) {
   *converged = false;
@@ -794,6 +843,9 @@ void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_
     sync(ctx);
     S.w64.reset();
   }
+  // a stage switch rebuilds released FP64 blocks (the new stage's copies and splits are made from them)
+  if (stage_prec != S.last_stage) ensure_fp64_ints(ctx, S);
+  S.last_stage = stage_prec;
   if (stage_prec != JUES_FP64) cc_make_ints32(ctx, S);
   S.split_on = stage_prec == JUES_FP16X3_MIXED;
   int stall = 0;
@@ -801,8 +853,12 @@ void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_
   static const char* stage_name[3] = {"fp64", "fp32", "fp16x3"};
   for (int k = 0; k < nsteps; ++k) {
     jues_iter_info info{};
-    if (stage_prec != JUES_FP64) cc_iteration<float>(ctx, S, &info);
-    else cc_iteration<double>(ctx, S, &info);
+    if (stage_prec != JUES_FP64) {
+      cc_iteration<float>(ctx, S, &info);
+      thin_fp64_ints(ctx, S);
+    } else {
+      cc_iteration<double>(ctx, S, &info);
+    }
     if (cb) cb(user, &info);
     if (S.opts.verbose)
       std::fprintf(stderr, "  cc iter %3d %s E=% .15f dE=% .3e |dt|=%.3e diis=%d %.1f ms\n", info.iter,

@@ -66,8 +66,10 @@ struct DiisOrder {
   int old_slots[MAX_DIIS];  // their slots, in the order their dots were accumulated
 };
 
+// packed T+/T- of t, or of tau = t + t1 t1 (tau[ijef] = t[ijef] + t1[i,e] t1[j,f]) when t1 is given
 template <typename T>
-void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm);
+void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm,
+            const double* t1 = nullptr);
 template <typename Ts, typename Td>
 void pack_v(jues_ctx* ctx, const PairTables& P, const Ts* X, int v, int f0, int f1, Td* Vp, Td* Vm,
             bool slab = false);

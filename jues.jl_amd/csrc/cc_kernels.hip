@@ -13,11 +13,14 @@ __device__ inline long tri_lt(int p, int q) { return (long)p + (long)q * (q - 1)
 
 // Tp[(i<=j),(e<=f)] = t[ijef] + t[ijfe] (e<f), t[ijee] (e=f)
 // Tm[(i<j),(e<f)]   = t[ijef] - t[ijfe]
+// With t1 non-null the packed amplitudes are those of tau = t + t1 t1,
+// tau[ijef] = t[ijef] + t1[i,e] t1[j,f] (the CCSD ladders), formed on the fly.
 // With ROWDIAG the single term is taken on the row diagonal instead: for the hole-hole
 // ladder integrals K[kl,cd] = <kl|cd>, KP[(k<=l),(c<=d)] = K[klcd] + K[kldc] (k<l),
 // K[kkcd] (k=l) and KM[(k<l),(c<d)] = K[klcd] - K[kldc].
 template <typename T, bool ROWDIAG = false>
-__global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ t, int o, int v,
+__global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ t, const double* __restrict__ t1,
+                                                     int o, int v,
                                                      const int2* __restrict__ pairs_o_le, const int2* __restrict__ pairs_v_le,
                                                      long npo, long npv, T* __restrict__ Tp,
                                                      const int2* __restrict__ pairs_o_lt, const int2* __restrict__ pairs_v_lt,
@@ -25,22 +28,20 @@ __global__ __launch_bounds__(256) void pack_t_kernel(const double* __restrict__ 
   const long oo = (long)o * o;
   const long tot_p = npo * npv, tot_m = nmo * nmv;
   for (long x = blockIdx.x * 256L + threadIdx.x; x < tot_p + tot_m; x += (long)gridDim.x * 256L) {
-    if (x < tot_p) {
-      const long r = x % npo, c = x / npo;
-      const int2 ij = pairs_o_le[r], ef = pairs_v_le[c];
-      const long base = ij.x + (long)o * ij.y;
-      const double a = t[base + oo * (ef.x + (long)v * ef.y)];
-      const double b = t[base + oo * (ef.y + (long)v * ef.x)];
-      Tp[x] = (T)((ROWDIAG ? ij.x == ij.y : ef.x == ef.y) ? a : a + b);
-    } else {
-      const long y = x - tot_p;
-      const long r = y % nmo, c = y / nmo;
-      const int2 ij = pairs_o_lt[r], ef = pairs_v_lt[c];
-      const long base = ij.x + (long)o * ij.y;
-      const double a = t[base + oo * (ef.x + (long)v * ef.y)];
-      const double b = t[base + oo * (ef.y + (long)v * ef.x)];
-      Tm[y] = (T)(a - b);
+    const bool plus = x < tot_p;
+    const long y = plus ? x : x - tot_p;
+    const long nr = plus ? npo : nmo;
+    const long r = y % nr, c = y / nr;
+    const int2 ij = plus ? pairs_o_le[r] : pairs_o_lt[r], ef = plus ? pairs_v_le[c] : pairs_v_lt[c];
+    const long base = ij.x + (long)o * ij.y;
+    double a = t[base + oo * (ef.x + (long)v * ef.y)];
+    double b = t[base + oo * (ef.y + (long)v * ef.x)];
+    if (t1) {
+      a += t1[ij.x + (long)o * ef.x] * t1[ij.y + (long)o * ef.y];
+      b += t1[ij.x + (long)o * ef.y] * t1[ij.y + (long)o * ef.x];
     }
+    if (plus) Tp[y] = (T)((ROWDIAG ? ij.x == ij.y : ef.x == ef.y) ? a : a + b);
+    else Tm[y] = (T)(a - b);
   }
 }
 
@@ -450,10 +451,10 @@ int grid256(long n) { return (int)std::max(1L, std::min((n + 255) / 256, 131072L
 }  // namespace
 
 template <typename T>
-void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm) {
+void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T* Tp, T* Tm, const double* t1) {
   const long tot = P.npo * P.npv + P.nmo * P.nmv;
   KTimer k(ctx, "cc_pack_t", 0.0, (double)tot * (16.0 + sizeof(T)));
-  hipLaunchKernelGGL((pack_t_kernel<T>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, t, o, v, P.o_le, P.v_le,
+  hipLaunchKernelGGL((pack_t_kernel<T>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, t, t1, o, v, P.o_le, P.v_le,
                      P.npo, P.npv, Tp, P.o_lt, P.v_lt, P.nmo, P.nmv, Tm);
   JUES_HIP(hipGetLastError());
 }
@@ -461,8 +462,8 @@ void pack_t(jues_ctx* ctx, const PairTables& P, const double* t, int o, int v, T
 void pack_khh(jues_ctx* ctx, const PairTables& P, const double* K, int o, int v, double* KP, double* KM) {
   const long tot = P.npo * P.npv + P.nmo * P.nmv;
   KTimer k(ctx, "cc_pack_k", 0.0, (double)tot * 24.0);
-  hipLaunchKernelGGL((pack_t_kernel<double, true>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, K, o, v, P.o_le,
-                     P.v_le, P.npo, P.npv, KP, P.o_lt, P.v_lt, P.nmo, P.nmv, KM);
+  hipLaunchKernelGGL((pack_t_kernel<double, true>), dim3(grid256(tot)), dim3(256), 0, ctx->stream, K, nullptr, o, v,
+                     P.o_le, P.v_le, P.npo, P.npv, KP, P.o_lt, P.v_lt, P.nmo, P.nmv, KM);
   JUES_HIP(hipGetLastError());
 }
 
@@ -573,8 +574,8 @@ template void pack_whh<double>(jues_ctx*, const PairTables&, const double*, int,
 template void pack_whh<float>(jues_ctx*, const PairTables&, const float*, int, float*, float*);
 template void pair_diag2<double>(jues_ctx*, const PairTables&, const double*, double*);
 template void pair_diag2<float>(jues_ctx*, const PairTables&, const float*, float*);
-template void pack_t<double>(jues_ctx*, const PairTables&, const double*, int, int, double*, double*);
-template void pack_t<float>(jues_ctx*, const PairTables&, const double*, int, int, float*, float*);
+template void pack_t<double>(jues_ctx*, const PairTables&, const double*, int, int, double*, double*, const double*);
+template void pack_t<float>(jues_ctx*, const PairTables&, const double*, int, int, float*, float*, const double*);
 template <typename T>
 void pack_v_ea(jues_ctx* ctx, const PairTables& P, const T* X, int v, int f0, int f1, T* Vp, T* Vm) {
   const long nrp = (long)f1 * (f1 + 1) / 2 - (long)f0 * (f0 + 1) / 2;

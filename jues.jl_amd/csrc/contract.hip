@@ -44,7 +44,8 @@ void add_perm(jues_ctx* ctx, double alpha, const Tens<Ti>& A, double beta, const
 
 template <typename T>
 void contract(jues_ctx* ctx, double alpha, const Tens<T>& A, const Tens<T>& B, double beta, const Tens<T>& C,
-              const char* tag) {
+              const char* tag, T* work, long work_elems) {
+
   JUES_REQUIRE(A.idx.size() == A.dims.size() && B.idx.size() == B.dims.size() && C.idx.size() == C.dims.size(),
                "contract: rank mismatch");
   std::string fa_A, fb_B, kA, kB;  // groups in operand order
@@ -93,6 +94,20 @@ void contract(jues_ctx* ctx, double alpha, const Tens<T>& A, const Tens<T>& B, d
           if (p.cost < best.cost) best = p;
         }
   const Plan& p = best;
+  const std::string direct = p.swapC ? p.fb + p.fa : p.fa + p.fb;
+  const bool a_perm = !(A.idx == p.fa + p.kk || A.idx == p.kk + p.fa);
+  const bool b_perm = !(B.idx == p.kk + p.fb || B.idx == p.fb + p.kk);
+  const bool c_direct = C.idx == direct;
+  // the largest permuted copy (operand or product) goes to the caller's workspace when it
+  // fits there, the others to the context's scratch slots
+  const long sz[3] = {a_perm ? A.size() : 0, b_perm ? B.size() : 0, c_direct ? 0 : C.size()};
+  int to_work = -1;
+  for (int q = 0; q < 3; ++q)
+    if (work && sz[q] > 0 && sz[q] <= work_elems && (to_work < 0 || sz[q] > sz[to_work])) to_work = q;
+  auto temp = [&](int q, int slot) -> T* {
+    if (q == to_work) return work;
+    return static_cast<T*>(scratch(ctx, slot, sizeof(T) * (size_t)sz[q]));
+  };
 
   // operand A as op(A) (M x K)
   const T* Ap = A.p;
@@ -100,7 +115,7 @@ void contract(jues_ctx* ctx, double alpha, const Tens<T>& A, const Tens<T>& B, d
   if (A.idx == p.fa + p.kk) ta = false;
   else if (A.idx == p.kk + p.fa) ta = true;
   else {
-    T* tmp = static_cast<T*>(scratch(ctx, 10, sizeof(T) * (size_t)A.size()));
+    T* tmp = temp(0, 10);
     permute<T, T>(ctx, A.p, A.dims, perm_to(A.idx, p.fa + p.kk), tmp, 1.0, 0.0, "contract_perm");
     Ap = tmp;
     ta = false;
@@ -110,16 +125,14 @@ void contract(jues_ctx* ctx, double alpha, const Tens<T>& A, const Tens<T>& B, d
   if (B.idx == p.kk + p.fb) tb = false;
   else if (B.idx == p.fb + p.kk) tb = true;
   else {
-    T* tmp = static_cast<T*>(scratch(ctx, 11, sizeof(T) * (size_t)B.size()));
+    T* tmp = temp(1, 11);
     permute<T, T>(ctx, B.p, B.dims, perm_to(B.idx, p.kk + p.fb), tmp, 1.0, 0.0, "contract_perm");
     Bp = tmp;
     tb = false;
   }
   const long lda = ta ? K : M, ldb = tb ? N : K;
-  const std::string direct = p.swapC ? p.fb + p.fa : p.fa + p.fb;
-  const bool c_direct = C.idx == direct;
   T* Cp = C.p;
-  if (!c_direct) Cp = static_cast<T*>(scratch(ctx, 12, sizeof(T) * (size_t)C.size()));
+  if (!c_direct) Cp = temp(2, 12);
   const T al = c_direct ? (T)alpha : T(1), be = c_direct ? (T)beta : T(0);
   if (!p.swapC) {
     gemm<T>(ctx, ta, tb, M, N, K, al, Ap, lda, 0, Bp, ldb, 0, be, Cp, M, 0, 1, tag);
@@ -137,9 +150,9 @@ void contract(jues_ctx* ctx, double alpha, const Tens<T>& A, const Tens<T>& B, d
 }
 
 template void contract<double>(jues_ctx*, double, const Tens<double>&, const Tens<double>&, double,
-                               const Tens<double>&, const char*);
+                               const Tens<double>&, const char*, double*, long);
 template void contract<float>(jues_ctx*, double, const Tens<float>&, const Tens<float>&, double,
-                              const Tens<float>&, const char*);
+                              const Tens<float>&, const char*, float*, long);
 template void add_perm<double, double>(jues_ctx*, double, const Tens<double>&, double, const Tens<double>&,
                                        const char*);
 template void add_perm<float, float>(jues_ctx*, double, const Tens<float>&, double, const Tens<float>&,

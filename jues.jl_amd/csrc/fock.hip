@@ -52,7 +52,7 @@ jues_status jues_ump2(jues_ctx* ctx, int prec, const double* Cao, int na, const 
   JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED, "jues_ump2: bad precision");
   *E = prec == JUES_FP64 ? ump2_dev<double>(ctx, Cao, na, Cav, va, Cbo, nb, Cbv, vb, epsa, epsb)
                          : ump2_dev<float>(ctx, Cao, na, Cav, va, Cbo, nb, Cbv, vb, epsa, epsb);
-  JUES_REQUIRE(std::isfinite(*E), "jues_ump2: non-finite energy");
+  if (!std::isfinite(*E)) throw Error(JUES_NONFINITE, "jues_ump2: non-finite energy (a zero denominator?)");
   JUES_API_END(ctx)
 }
 

@@ -182,10 +182,12 @@ struct Tens {
   std::vector<long> dims;
   long size() const { long s = 1; for (long d : dims) s *= d; return s; }
 };
-// C[idxC] = alpha * sum A[idxA] B[idxB] + beta * C  (any index order; TTGT on the device)
+// C[idxC] = alpha * sum A[idxA] B[idxB] + beta * C  (any index order; TTGT on the device).
+// A permuted copy of an operand or of the product goes to `work` (work_elems elements) when
+// given and large enough, else to the context's scratch.
 template <typename T>
 void contract(jues_ctx* ctx, double alpha, const Tens<T>& A, const Tens<T>& B, double beta,
-              const Tens<T>& C, const char* tag = nullptr);
+              const Tens<T>& C, const char* tag = nullptr, T* work = nullptr, long work_elems = 0);
 // C[idxC] = alpha * P(A) + beta * C
 template <typename Ti, typename To>
 void add_perm(jues_ctx* ctx, double alpha, const Tens<Ti>& A, double beta, const Tens<To>& C,
@@ -207,6 +209,9 @@ void tei_transform_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, 
 // MO factors L[p,q,P] = (C1^T B_P C2)[p,q]  (needs factors)
 template <typename T>
 void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* L);
+// the same with the half-transformed factors in `work` (n * min(d1, d2) * naux elements)
+template <typename T>
+void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* L, T* work);
 // RMP2 energy from K[i,a,j,b] = (ia|jb) (device, o x v x o x v) and eps (device FP64, n)
 template <typename T>
 double mp2_energy_dev(jues_ctx* ctx, const T* K, const double* eps, int o, int v);

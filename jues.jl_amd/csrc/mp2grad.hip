@@ -12,8 +12,18 @@
 //   with the half-transformed integrals recomputed (never stored between calls):
 //     H12[mu,nu,j,b] = sum g[mu,nu,lam,sig] Co[lam,j] Cv[sig,b]
 //     H34[i,a,lam,sig] = sum g[mu,nu,lam,sig] Co[mu,i] Cv[nu,a].
-// No symmetry of g is assumed.  With factors, H12 = B . L^T and H34 = L . B^T
-// with L[i,a,P] = (Co^T B_P Cv)[i,a].
+// No symmetry of g is assumed.
+//
+// Factor route ((mu nu|lam sig) = sum_P B[mu,nu,P] B[lam,sig,P]): K = L L^T with
+// L[i,a,P] = (Co^T B_P Cv)[i,a], so K and G are symmetric (ov) x (ov) matrices and
+//   dL = dE/dL = (G + G^T) L = 2 G L,
+//   dCo[mu,i] = sum_{a,P} dL[i,a,P] (B_P Cv)[mu,a],  dCv[nu,a] = sum_{i,P} dL[i,a,P] (B_P^T Co)[nu,i];
+// no n^2 ov half-transform is formed (no symmetry of B_P in (mu,nu) is assumed).
+//
+// Orbital energies: with W = K (2K - K_ibja) / D^2 every eps enters D through two slots,
+//   dE/de_k = -(sum_{a,jb} W[k,a,jb] + sum_{ia,b} W[ia,k,b])          (occupied k)
+//   dE/de_{o+c} = sum_{i,jb} W[i,c,jb] + sum_{ia,j} W[ia,j,c]          (virtual c),
+// i.e. r = row sums + column sums of W as an (ov) x (ov) matrix, then sums of r over a or i.
 #include "api_util.h"
 #include "jues_internal.h"
 
@@ -22,17 +32,19 @@
 namespace jues {
 namespace {
 
-// G[i,a,j,b] = 2(2K - Kx)/D and per-column energy partials (one block per (j,b))
+// G[i,a,j,b] = 2(2K - Kx)/D and per-column energy partials (one block per (j,b)); with W
+// non-null also W = K(2K - Kx)/D^2 and its column sums wcol[jb]
 template <typename T>
 __global__ __launch_bounds__(256) void mp2_g_kernel(const T* __restrict__ K, const double* __restrict__ eps, int o,
-                                                    int v, T* __restrict__ G, double* __restrict__ part) {
-  __shared__ double sh[4];
+                                                    int v, T* __restrict__ G, double* __restrict__ part,
+                                                    double* __restrict__ W, double* __restrict__ wcol) {
+  __shared__ double sh[8];
   const int ov = o * v;
   const int jb = blockIdx.x;
   const int j = jb % o, b = jb / o;
   const double ejb = eps[j] - eps[o + b];
   const long col = (long)jb * ov;
-  double s = 0.0;
+  double s = 0.0, sw = 0.0;
   for (int ia = threadIdx.x; ia < ov; ia += 256) {
     const int i = ia % o, a = ia / o;
     const double k = (double)K[col + ia];
@@ -40,11 +52,25 @@ __global__ __launch_bounds__(256) void mp2_g_kernel(const T* __restrict__ K, con
     const double d = eps[i] - eps[o + a] + ejb;
     s += k * (2.0 * k - kx) / d;
     G[col + ia] = (T)(2.0 * (2.0 * k - kx) / d);
+    if (W) {
+      const double w = k * (2.0 * k - kx) / (d * d);
+      W[col + ia] = w;
+      sw += w;
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_down(s, off, 64);
+    sw += __shfl_down(sw, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = s;
+    sh[4 + (threadIdx.x >> 6)] = sw;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) part[jb] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  if (threadIdx.x == 0) {
+    part[jb] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    if (W) wcol[jb] = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+  }
 }
 
 __global__ __launch_bounds__(256) void sum_fixed_kernel2(const double* __restrict__ part, int count,
@@ -61,52 +87,148 @@ __global__ __launch_bounds__(256) void sum_fixed_kernel2(const double* __restric
   if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// rowpart[c, ia] = sum over the columns jb of chunk c of W[ia, jb] (coalesced along ia)
+__global__ __launch_bounds__(256) void mp2_w_rowsum_kernel(const double* __restrict__ W, long ov, long chunk,
+                                                           double* __restrict__ rowpart) {
+  const long ia = blockIdx.x * 256L + threadIdx.x;
+  if (ia >= ov) return;
+  const long jb0 = blockIdx.y * chunk, jb1 = std::min(ov, jb0 + chunk);
+  double s = 0.0;
+  for (long jb = jb0; jb < jb1; ++jb) s += W[ia + ov * jb];
+  rowpart[blockIdx.y * ov + ia] = s;
+}
+
+// r[ia] = wcol[ia] + sum_c rowpart[c, ia]  (fixed order)
+__global__ __launch_bounds__(256) void mp2_w_r_kernel(const double* __restrict__ rowpart, int nchunk,
+                                                      const double* __restrict__ wcol, long ov,
+                                                      double* __restrict__ r) {
+  const long ia = blockIdx.x * 256L + threadIdx.x;
+  if (ia >= ov) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; ++c) s += rowpart[c * ov + ia];
+  r[ia] = wcol[ia] + s;
+}
+
+// deps[k] = -sum_a r[k,a] (occupied k), deps[o+c] = sum_i r[i,c] (virtual c)
+__global__ __launch_bounds__(256) void mp2_deps_kernel(const double* __restrict__ r, int o, int v,
+                                                       double* __restrict__ deps) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= o + v) return;
+  double s = 0.0;
+  if (p < o) {
+    for (int a = 0; a < v; ++a) s += r[p + (long)o * a];
+    deps[p] = -s;
+  } else {
+    const long c = p - o;
+    for (int i = 0; i < o; ++i) s += r[i + o * c];
+    deps[p] = s;
+  }
+}
+
+// G = dE/dK from K, the energy into *E (host) and, with deps non-null, dE/deps (device, o+v)
 template <typename T>
-void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const double* eps, double* E, T* dC,
-                  T* dg) {
-  const long n = ctx->n, n2 = n * n, n3 = n2 * n;
+void mp2_g(jues_ctx* ctx, const T* K, const double* eps, int o, int v, T* G, double* E, double* deps) {
   const long ov = (long)o * v, N = ov * ov;
-  // K = (ia|jb), G, energy
-  DevBuf Kb(N * sizeof(T)), Gb(N * sizeof(T)), part((ov + 1) * sizeof(double));
-  T* K = Kb.as<T>();
-  T* G = Gb.as<T>();
-  tei_transform_dev<T>(ctx, Co, o, Cv, v, Co, o, Cv, v, K);
+  DevBuf part((ov + 1) * sizeof(double));
+  DevBuf Wb(deps ? N * sizeof(double) : 0), wcol(deps ? ov * sizeof(double) : 0);
   {
-    KTimer t(ctx, "mp2_grad_g", 0.0, (double)N * sizeof(T) * 3);
+    KTimer t(ctx, "mp2_grad_g", 0.0, (double)N * (sizeof(T) * 3 + (deps ? 8 : 0)));
     hipLaunchKernelGGL((mp2_g_kernel<T>), dim3((unsigned)ov), dim3(256), 0, ctx->stream, K, eps, o, v, G,
-                       part.as<double>());
+                       part.as<double>(), deps ? Wb.as<double>() : nullptr, deps ? wcol.as<double>() : nullptr);
     hipLaunchKernelGGL(sum_fixed_kernel2, dim3(1), dim3(256), 0, ctx->stream, part.as<double>(), (int)ov,
                        part.as<double>() + ov);
     JUES_HIP(hipGetLastError());
   }
+  if (deps) {
+    KTimer t(ctx, "mp2_grad_eps", 0.0, (double)N * 8);
+    const long chunk = 128, nchunk = (ov + chunk - 1) / chunk;
+    DevBuf rowpart((size_t)(nchunk * ov) * sizeof(double)), r(ov * sizeof(double));
+    const unsigned gx = (unsigned)((ov + 255) / 256);
+    hipLaunchKernelGGL(mp2_w_rowsum_kernel, dim3(gx, (unsigned)nchunk), dim3(256), 0, ctx->stream, Wb.as<double>(),
+                       ov, chunk, rowpart.as<double>());
+    hipLaunchKernelGGL(mp2_w_r_kernel, dim3(gx), dim3(256), 0, ctx->stream, rowpart.as<double>(), (int)nchunk,
+                       wcol.as<double>(), ov, r.as<double>());
+    hipLaunchKernelGGL(mp2_deps_kernel, dim3((unsigned)((o + v + 255) / 256)), dim3(256), 0, ctx->stream,
+                       r.as<double>(), o, v, deps);
+    JUES_HIP(hipGetLastError());
+    sync(ctx);
+  }
   d2h(ctx, E, part.as<double>() + ov, sizeof(double));
+}
+
+// Factor route of the adjoint: dE/dC from L = Co^T B_P Cv without any n^2 ov intermediate.
+template <typename T>
+void mp2_grad_factors_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const double* eps, double* E, T* dC,
+                          double* deps) {
+  const long n = ctx->n, naux = ctx->naux;
+  const long ov = (long)o * v, N = ov * ov;
+  const T* B = ao_factors<T>(ctx);
+  const T one = T(1), zero = T(0);
+  DevBuf Lb(ov * naux * sizeof(T)), dLb(ov * naux * sizeof(T));
+  T* L = Lb.as<T>();
+  T* dL = dLb.as<T>();
+  mo_factors_dev<T>(ctx, Co, o, Cv, v, L);
+  {
+    DevBuf Kb(N * sizeof(T)), Gb(N * sizeof(T));
+    T* K = Kb.as<T>();
+    T* G = Gb.as<T>();
+    gemm<T>(ctx, false, true, ov, ov, naux, one, L, ov, 0, L, ov, 0, zero, K, ov, 0, 1, "mp2grad_k");
+    mp2_g<T>(ctx, K, eps, o, v, G, E, deps);
+    Kb.release();
+    // dL = (G + G^T) L = 2 G L (G is symmetric: K = L L^T is)
+    gemm<T>(ctx, false, false, ov, naux, ov, T(2), G, ov, 0, L, ov, 0, zero, dL, ov, 0, 1, "mp2grad_dl");
+  }
+  Lb.release();
+  T* dCo = dC;          // n x o (leading dimension n)
+  T* dCv = dC + n * o;  // n x v
+  {
+    // Z[mu,a,P] = (B_P Cv)[mu,a];  dCo = Z (n x v naux) . dL^T (v naux x o)
+    DevBuf Z(n * v * naux * sizeof(T));
+    gemm<T>(ctx, false, false, n, v, n, one, B, n, n * n, Cv, n, 0, zero, Z.as<T>(), n, n * v, (int)naux,
+            "mp2grad_bc");
+    gemm<T>(ctx, false, true, n, o, v * naux, one, Z.as<T>(), n, 0, dL, o, 0, zero, dCo, n, 0, 1, "mp2grad");
+  }
+  {
+    // Wt[nu,i,P] = (B_P^T Co)[nu,i];  dLp[i,P,a] = dL[i,a,P];  dCv = Wt (n x o naux) . dLp (o naux x v)
+    DevBuf Wt(n * o * naux * sizeof(T)), dLp(ov * naux * sizeof(T));
+    gemm<T>(ctx, true, false, n, o, n, one, B, n, n * n, Co, n, 0, zero, Wt.as<T>(), n, n * o, (int)naux,
+            "mp2grad_bc");
+    permute<T, T>(ctx, dL, {o, v, naux}, {0, 2, 1}, dLp.as<T>(), 1.0, 0.0, "mp2grad_perm");
+    gemm<T>(ctx, false, false, n, v, o * naux, one, Wt.as<T>(), n, 0, dLp.as<T>(), o * naux, 0, zero, dCv, n, 0, 1,
+            "mp2grad");
+  }
+}
+
+// Dense route of the adjoint (the AO tensor g is on the device); dg may be null.
+template <typename T>
+void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const double* eps, double* E, T* dC,
+                  T* dg, double* deps) {
+  const long n = ctx->n, n2 = n * n, n3 = n2 * n;
+  const long ov = (long)o * v, N = ov * ov;
+  // K = (ia|jb), G, energy
+  DevBuf Kb(N * sizeof(T)), Gb(N * sizeof(T));
+  T* K = Kb.as<T>();
+  T* G = Gb.as<T>();
+  tei_transform_dev<T>(ctx, Co, o, Cv, v, Co, o, Cv, v, K);
+  mp2_g<T>(ctx, K, eps, o, v, G, E, deps);
   Kb.release();
   const T one = T(1), zero = T(0);
   // half-transformed integrals, recomputed here
   DevBuf H12b(n2 * ov * sizeof(T)), H34b(n2 * ov * sizeof(T));
   T* H12 = H12b.as<T>();  // [mu,nu,j,b]
   T* H34 = H34b.as<T>();  // [i,a,lam,sig]
-  if (ctx->gao.p) {
-    const T* g = ao_dense<T>(ctx);
-    {
-      DevBuf Y(n3 * v * sizeof(T));  // Y[mu,nu,lam,b] = sum_sig g Cv[sig,b]
-      gemm<T>(ctx, false, false, n3, v, n, one, g, n3, 0, Cv, n, 0, zero, Y.as<T>(), n3, 0, 1, "mp2grad_half");
-      gemm<T>(ctx, false, false, n2, o, n, one, Y.as<T>(), n2, n3, Co, n, 0, zero, H12, n2, n2 * o, v,
-              "mp2grad_half");
-    }
-    {
-      DevBuf Z(o * n3 * sizeof(T));  // Z[i,nu,lam,sig] = sum_mu Co[mu,i] g
-      gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z.as<T>(), o, 0, 1, "mp2grad_half");
-      gemm<T>(ctx, false, false, o, v, n, one, Z.as<T>(), o, (long)o * n, Cv, n, 0, zero, H34, o, ov, (int)n2,
-              "mp2grad_half");
-    }
-  } else {
-    const T* B = ao_factors<T>(ctx);
-    const long naux = ctx->naux;
-    DevBuf L(ov * naux * sizeof(T));
-    mo_factors_dev<T>(ctx, Co, o, Cv, v, L.as<T>());
-    gemm<T>(ctx, false, true, n2, ov, naux, one, B, n2, 0, L.as<T>(), ov, 0, zero, H12, n2, 0, 1, "mp2grad_half");
-    gemm<T>(ctx, false, true, ov, n2, naux, one, L.as<T>(), ov, 0, B, n2, 0, zero, H34, ov, 0, 1, "mp2grad_half");
+  const T* g = ao_dense<T>(ctx);
+  {
+    DevBuf Y(n3 * v * sizeof(T));  // Y[mu,nu,lam,b] = sum_sig g Cv[sig,b]
+    gemm<T>(ctx, false, false, n3, v, n, one, g, n3, 0, Cv, n, 0, zero, Y.as<T>(), n3, 0, 1, "mp2grad_half");
+    gemm<T>(ctx, false, false, n2, o, n, one, Y.as<T>(), n2, n3, Co, n, 0, zero, H12, n2, n2 * o, v,
+            "mp2grad_half");
+  }
+  {
+    DevBuf Z(o * n3 * sizeof(T));  // Z[i,nu,lam,sig] = sum_mu Co[mu,i] g
+    gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z.as<T>(), o, 0, 1, "mp2grad_half");
+    gemm<T>(ctx, false, false, o, v, n, one, Z.as<T>(), o, (long)o * n, Cv, n, 0, zero, H34, o, ov, (int)n2,
+            "mp2grad_half");
   }
   T* dCo = dC;           // n x o  (columns 0..o-1 of dC, leading dimension n)
   T* dCv = dC + n * o;   // n x v
@@ -174,7 +296,7 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
 using namespace jues;
 
 extern "C" jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, int o, int n, const double* eps,
-                                      double* E, double* dE_dC, double* dE_dg) {
+                                      double* E, double* dE_dC, double* dE_dg, double* dE_deps) {
   JUES_API_BEGIN(ctx)
   JUES_REQUIRE(C && eps && E && o > 0 && o < n, "jues_rmp2_grad: bad arguments");
   JUES_REQUIRE(n == ctx->n && n > 0, "jues_rmp2_grad: n does not match the AO integrals");
@@ -182,28 +304,36 @@ extern "C" jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, 
   JUES_REQUIRE(prec == JUES_FP64 || prec == JUES_FP32_MIXED, "jues_rmp2_grad: bad precision");
   const int v = n - o;
   DeviceArray<double> e(ctx, eps, (size_t)n);
+  DevBuf de(dE_deps ? (size_t)n * sizeof(double) : 0);
+  double* deps = dE_deps ? de.as<double>() : nullptr;
   const size_t nn = (size_t)n * n, n4 = nn * nn;
-  if (prec == JUES_FP64) {
-    DeviceArray<double> c(ctx, C, nn);
-    DevBuf dC(nn * sizeof(double));
-    DevBuf dg(dE_dg ? n4 * sizeof(double) : 0);
-    mp2_grad_dev<double>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<double>(),
-                         dE_dg ? dg.as<double>() : nullptr);
-    if (dE_dC) d2h(ctx, dE_dC, dC.p, nn * sizeof(double));
-    if (dE_dg) d2h(ctx, dE_dg, dg.p, n4 * sizeof(double));
-  } else {
-    DeviceArray<float> c(ctx, C, nn);
-    DevBuf dC(nn * sizeof(float)), dC64(nn * sizeof(double));
-    DevBuf dg(dE_dg ? n4 * sizeof(float) : 0);
-    mp2_grad_dev<float>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<float>(), dE_dg ? dg.as<float>() : nullptr);
-    convert<float, double>(ctx, dC.as<float>(), dC64.as<double>(), (long)nn);
-    if (dE_dC) d2h(ctx, dE_dC, dC64.p, nn * sizeof(double));
-    if (dE_dg) {
-      DevBuf dg64(n4 * sizeof(double));
-      convert<float, double>(ctx, dg.as<float>(), dg64.as<double>(), (long)n4);
-      d2h(ctx, dE_dg, dg64.p, n4 * sizeof(double));
-    }
-  }
-  JUES_REQUIRE(std::isfinite(*E), "jues_rmp2_grad: non-finite energy");
+  // dE/dg needs the dense route; otherwise factors take the factor route
+  const bool factor_route = !ctx->gao.p;
+  auto run = [&](auto zero) {
+    using T = decltype(zero);
+    DeviceArray<T> c(ctx, C, nn);
+    DevBuf dC(nn * sizeof(T));
+    DevBuf dg(dE_dg ? n4 * sizeof(T) : 0);
+    if (factor_route)
+      mp2_grad_factors_dev<T>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<T>(), deps);
+    else
+      mp2_grad_dev<T>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<T>(), dE_dg ? dg.as<T>() : nullptr, deps);
+    // results back in FP64 (an FP32 stage converts on the device first)
+    auto fetch = [&](double* host, const DevBuf& dev, size_t count) {
+      if constexpr (sizeof(T) == sizeof(double)) {
+        d2h(ctx, host, dev.p, count * sizeof(double));
+      } else {
+        DevBuf d64(count * sizeof(double));
+        convert<T, double>(ctx, dev.as<T>(), d64.as<double>(), (long)count);
+        d2h(ctx, host, d64.p, count * sizeof(double));
+      }
+    };
+    if (dE_dC) fetch(dE_dC, dC, nn);
+    if (dE_dg) fetch(dE_dg, dg, n4);
+  };
+  if (prec == JUES_FP64) run(0.0);
+  else run(0.0f);
+  if (dE_deps) d2h(ctx, dE_deps, deps, (size_t)n * sizeof(double));
+  if (!std::isfinite(*E)) throw Error(JUES_NONFINITE, "jues_rmp2_grad: non-finite energy (a zero denominator?)");
   JUES_API_END(ctx)
 }

@@ -47,17 +47,22 @@ const float* ao_factors<float>(jues_ctx* ctx) {
 template <typename T>
 void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* L) {
   const long n = ctx->n, naux = ctx->naux;
+  T* Y = static_cast<T*>(scratch(ctx, 4, sizeof(T) * (size_t)(n * std::min(d1, d2) * naux)));
+  mo_factors_dev<T>(ctx, C1, d1, C2, d2, L, Y);
+}
+
+template <typename T>
+void mo_factors_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, T* L, T* Y) {
+  const long n = ctx->n, naux = ctx->naux;
   const T* B = ao_factors<T>(ctx);
   // The smaller of d1, d2 is contracted first: 2 n^2 min(d1,d2) naux + 2 n d1 d2 naux flops.
   if (d1 <= d2) {
-    T* Y = static_cast<T*>(scratch(ctx, 4, sizeof(T) * (size_t)(n * d1 * naux)));
     // Y[nu, p, P] = B_P^T C1 ; L[p, q, P] = Y_P^T C2
     gemm<T>(ctx, true, false, n, d1, n, T(1), B, n, n * n, C1, n, 0, T(0), Y, n, n * d1, (int)naux,
             "xform_factor");
     gemm<T>(ctx, true, false, d1, d2, n, T(1), Y, n, n * d1, C2, n, 0, T(0), L, d1, (long)d1 * d2, (int)naux,
             "xform_factor");
   } else {
-    T* Y = static_cast<T*>(scratch(ctx, 4, sizeof(T) * (size_t)(n * d2 * naux)));
     // Y[mu, q, P] = B_P C2 ; L[p, q, P] = C1^T Y_P
     gemm<T>(ctx, false, false, n, d2, n, T(1), B, n, n * n, C2, n, 0, T(0), Y, n, n * d2, (int)naux,
             "xform_factor");
@@ -223,6 +228,8 @@ double mp2_energy_dev(jues_ctx* ctx, const T* K, const double* eps, int o, int v
 
 template void mo_factors_dev<double>(jues_ctx*, const double*, int, const double*, int, double*);
 template void mo_factors_dev<float>(jues_ctx*, const float*, int, const float*, int, float*);
+template void mo_factors_dev<double>(jues_ctx*, const double*, int, const double*, int, double*, double*);
+template void mo_factors_dev<float>(jues_ctx*, const float*, int, const float*, int, float*, float*);
 template void tei_transform_dev<double>(jues_ctx*, const double*, int, const double*, int, const double*, int,
                                         const double*, int, double*);
 template void tei_transform_dev<float>(jues_ctx*, const float*, int, const float*, int, const float*, int,

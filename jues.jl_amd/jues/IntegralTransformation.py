@@ -12,22 +12,25 @@ def get_eri(wfn: Wfn, eri_string: str, notation: str = "phys", fcn: int = 0, *, 
     """MO integral block named by a 4-letter string over {o,O,v,V}.
 
     ``notation="phys"`` returns <pq|rs> (the default, as the reference);
-    ``"chem"`` returns (pq|rs).  ``fcn`` drops the first fcn occupied orbitals.
-    Raises ``JuesError`` (BAD_ARG) for an invalid string, like the reference's ``error``.
+    ``"chem"`` returns (pq|rs).  Upper-case letters are the alpha orbitals
+    (``O`` -> ``Cao``, nalpha; ``V`` -> ``Cav``, nvira), lower-case the beta ones
+    (``o`` -> ``Cbo``, nbeta; ``v`` -> ``Cbv``, nvirb), as IntegralTransformation.jl:52-68;
+    they coincide for a restricted ``Wfn``.  ``fcn`` drops the first fcn occupied
+    orbitals of either spin.  Raises ``JuesError`` (BAD_ARG) for an invalid string, like
+    the reference's ``error``.
     """
     ctx = _lib.context(device)
     n = ensure_ao(ctx, wfn.ao_eri)
-    o, v = wfn.nalpha, wfn.nvira
     s = eri_string if isinstance(eri_string, str) else str(eri_string)
     if len(s.encode()) != 4 or any(c not in "oOvV" for c in s):
         # same message the library returns; checked here to size the output
         raise _lib.JuesError(_lib.JUES_BAD_ARG, f"Invalid string given to get_eri: {s}")
-    shape_src = s
-    dims = [(o - fcn) if c in "oO" else v for c in shape_src]
-    out = np.empty(dims, dtype=np.float64, order="F")
-    Co, Cv = _lib.f64(wfn.Cao), _lib.f64(wfn.Cav)
-    ctx.check(ctx.lib.jues_get_eri(ctx.h, prec_code(prec), s.encode(), notation.encode(), _lib.ptr(Co), o,
-                                   _lib.ptr(Cv), v, n, fcn, _lib.ptr(out)))
+    size = {"O": wfn.nalpha - fcn, "o": wfn.nbeta - fcn, "V": wfn.nvira, "v": wfn.nvirb}
+    out = np.empty([size[c] for c in s], dtype=np.float64, order="F")
+    Cao, Cav, Cbo, Cbv = (_lib.f64(x) for x in (wfn.Cao, wfn.Cav, wfn.Cbo, wfn.Cbv))
+    ctx.check(ctx.lib.jues_get_eri(ctx.h, prec_code(prec), s.encode(), notation.encode(),
+                                   _lib.ptr(Cao), wfn.nalpha, _lib.ptr(Cav), wfn.nvira,
+                                   _lib.ptr(Cbo), wfn.nbeta, _lib.ptr(Cbv), wfn.nvirb, n, fcn, _lib.ptr(out)))
     return out
 
 

@@ -52,9 +52,11 @@ def do_ump2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0) -> float:
 
 
 def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: int = 0):
-    """(E, dE/dC (n x n), dE/dg (n^4 or None)) with every C and g entry independent.
+    """(E, dE/dC (n x n), dE/dg (n^4 or None), dE/deps (n)) with every entry of C, g and
+    eps independent (eps occupied first, as ``refWfn.epsa``).
 
-    ``dE/dg`` needs dense AO integrals; with a ``FactoredERI`` it is None.
+    ``dE/dg`` needs dense AO integrals; with a ``FactoredERI`` it is None and the
+    gradient takes the factor route (no n^2 o v intermediate).
     """
     ctx = _lib.context(device)
     n = ensure_ao(ctx, refWfn.ao_eri)
@@ -62,12 +64,13 @@ def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: 
     eps = _lib.f64(refWfn.epsa)
     E = np.zeros(1)
     dC = np.zeros((n, n), order="F")
+    deps = np.zeros(n)
     dg = None
     if want_dg and not is_factored(refWfn.ao_eri):
         dg = np.zeros((n, n, n, n), order="F")
     ctx.check(ctx.lib.jues_rmp2_grad(ctx.h, prec_code(prec), _lib.ptr(C), refWfn.nalpha, n, _lib.ptr(eps),
-                                     _lib.ptr(E), _lib.ptr(dC), _lib.ptr(dg)))
-    return float(E[0]), dC, dg
+                                     _lib.ptr(E), _lib.ptr(dC), _lib.ptr(dg), _lib.ptr(deps)))
+    return float(E[0]), dC, dg, deps
 
 
 __all__ = ["do_rmp2", "do_df_rmp2", "do_ump2", "rmp2_grad"]

@@ -98,9 +98,9 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "jues_ump2": (i, [_ctxp, i, _dp, i, _dp, i, _dp, i, _dp, i, i, _dp, _dp, _dp]),
             "jues_get_fock": (i, [_ctxp, _dp, _dp, i, _dp, i, i, _dp]),
             "jues_tei_transform": (i, [_ctxp, i, _dp, i, _dp, i, _dp, i, _dp, i, i, _dp]),
-            "jues_get_eri": (i, [_ctxp, i, C.c_char_p, C.c_char_p, _dp, i, _dp, i, i, i, _dp]),
+            "jues_get_eri": (i, [_ctxp, i, C.c_char_p, C.c_char_p, _dp, i, _dp, i, _dp, i, _dp, i, i, i, _dp]),
             "jues_rmp2": (i, [_ctxp, i, _dp, i, _dp, i, i, _dp, _dp]),
-            "jues_rmp2_grad": (i, [_ctxp, i, _dp, i, i, _dp, _dp, _dp, _dp]),
+            "jues_rmp2_grad": (i, [_ctxp, i, _dp, i, i, _dp, _dp, _dp, _dp, _dp]),
             "jues_rccd": (i, [_ctxp, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp, ITER_CB, C.c_void_p]),
             "jues_rccsd": (i, [_ctxp, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp, _dp, ITER_CB,
                                C.c_void_p]),
@@ -171,6 +171,12 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def release(self):
+        """Free every device buffer the context holds (integrals, CC state, scratch)."""
+        self.check(self.lib.jues_release(self.h))
+        self.ao_key = None
+        self.ao_ref = None
 
     def check(self, st: int, allow=(JUES_OK,)) -> int:
         if st not in allow:

@@ -176,13 +176,17 @@ function get_eri(wfn, eri_string::String; notation::String="phys", fcn::Int=0, p
     sizeof(eri_string) == 4 || error("Invalid string given to JuES.IntegralTransformation.get_eri: $eri_string")
     ctx = context()
     n = ensure_ao!(ctx, wfn.ao_eri)
-    o, v = wfn.nalpha, wfn.nvira
-    dims = Tuple(c in ('o', 'O') ? o - fcn : v for c in eri_string)
-    out = Array{Float64,4}(undef, dims...)
-    Co, Cv = wfn.Cao, wfn.Cav
-    GC.@preserve Co Cv out check(ctx, ccall((:jues_get_eri, libjues), Cint,
-        (Ptr{Cvoid}, Cint, Cstring, Cstring, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Cint, Ptr{Float64}),
-        ctx.handle, precision_code(precision), eri_string, notation, Co, o, Cv, v, n, fcn, out))
+    # IntegralTransformation.jl:52-68: upper case alpha (Cao, Cav), lower case beta (Cbo, Cbv)
+    size_of = Dict('O' => wfn.nalpha - fcn, 'o' => wfn.nbeta - fcn, 'V' => wfn.nvira, 'v' => wfn.nvirb)
+    all(c -> haskey(size_of, c), eri_string) ||
+        error("Invalid string given to JuES.IntegralTransformation.get_eri: $eri_string")
+    out = Array{Float64,4}(undef, Tuple(size_of[c] for c in eri_string)...)
+    Cao, Cav, Cbo, Cbv = wfn.Cao, wfn.Cav, wfn.Cbo, wfn.Cbv
+    GC.@preserve Cao Cav Cbo Cbv out check(ctx, ccall((:jues_get_eri, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Cstring, Cstring, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint,
+         Ptr{Float64}, Cint, Cint, Cint, Ptr{Float64}),
+        ctx.handle, precision_code(precision), eri_string, notation, Cao, wfn.nalpha, Cav, wfn.nvira,
+        Cbo, wfn.nbeta, Cbv, wfn.nvirb, n, fcn, out))
     return out
 end
 
@@ -252,36 +256,42 @@ end
     rmp2_energy(C, gao, eps, nocc)
 
 RMP2 correlation energy as a differentiable function of the full MO matrix `C`
-(n x n, occupied columns first) and the dense AO integrals `gao` (n^4); `eps`
-is held fixed.
+(n x n, occupied columns first), the dense AO integrals `gao` (n^4) and the
+orbital energies `eps` (n, occupied first).
 """
 function rmp2_energy(C::Matrix{Float64}, gao::Array{Float64,4}, eps::Vector{Float64}, nocc::Int;
                      precision=:fp64)
-    E, _, _ = rmp2_grad(C, gao, eps, nocc; want_dC=false, want_dg=false, precision=precision)
+    E, _, _, _ = rmp2_grad(C, gao, eps, nocc; want_dC=false, want_dg=false, want_deps=false,
+                           precision=precision)
     return E
 end
 
-function rmp2_grad(C, gao, eps, nocc; want_dC=true, want_dg=true, precision=:fp64)
+"(E, dE/dC, dE/dg, dE/deps) from the library's hand-written adjoint; unwanted parts are `nothing`."
+function rmp2_grad(C, gao, eps, nocc; want_dC=true, want_dg=true, want_deps=true, precision=:fp64)
     ctx = context()
     n = ensure_ao!(ctx, gao)
     E = Ref{Float64}(0.0)
     dC = want_dC ? zeros(n, n) : nothing
     dg = want_dg ? zeros(n, n, n, n) : nothing
+    de = want_deps ? zeros(n) : nothing
     pC = dC === nothing ? Ptr{Float64}(C_NULL) : pointer(dC)
     pg = dg === nothing ? Ptr{Float64}(C_NULL) : pointer(dg)
-    GC.@preserve C eps dC dg check(ctx, ccall((:jues_rmp2_grad, libjues), Cint,
-        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}, Ptr{Float64}, Ptr{Float64}),
-        ctx.handle, precision_code(precision), C, nocc, n, eps, E, pC, pg))
-    return E[], dC, dg
+    pe = de === nothing ? Ptr{Float64}(C_NULL) : pointer(de)
+    GC.@preserve C eps dC dg de check(ctx, ccall((:jues_rmp2_grad, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}),
+        ctx.handle, precision_code(precision), C, nocc, n, eps, E, pC, pg, pe))
+    return E[], dC, dg, de
 end
 
-# reverse rule: the library's hand-written adjoint (recomputes the half-transformed integrals)
+# reverse rule: the library's hand-written adjoint (recomputes the half-transformed integrals);
+# eps enters every denominator, so its tangent is dE/deps, not zero
 function ChainRulesCore.rrule(::typeof(rmp2_energy), C::Matrix{Float64}, gao::Array{Float64,4},
                               eps::Vector{Float64}, nocc::Int; precision=:fp64)
-    E, dC, dg = rmp2_grad(C, gao, eps, nocc; precision=precision)
+    E, dC, dg, de = rmp2_grad(C, gao, eps, nocc; precision=precision)
     function rmp2_energy_pullback(Ebar)
         eb = unthunk(Ebar)
-        return NoTangent(), eb .* dC, eb .* dg, ZeroTangent(), NoTangent()
+        return NoTangent(), eb .* dC, eb .* dg, eb .* de, NoTangent()
     end
     return E, rmp2_energy_pullback
 end

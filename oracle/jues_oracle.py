@@ -99,21 +99,24 @@ def mo_from_factors(B, C1, C2, C3, C4):
     return _es("pqP,rsP->pqrs", L, R)
 
 
-def get_eri(gao, Cao, Cav, eri_string, notation="phys", fcn=0):
-    """IntegralTransformation.jl:38-101 (RHF: lower- and upper-case letters coincide)."""
+def get_eri(gao, Cao, Cav, eri_string, notation="phys", fcn=0, Cbo=None, Cbv=None):
+    """IntegralTransformation.jl:38-101.  Upper-case letters take the alpha orbitals
+    (``O`` -> Cao, ``V`` -> Cav), lower-case the beta ones (``o`` -> Cbo, ``v`` -> Cbv),
+    :52-68; ``fcn`` drops the first fcn columns of either occupied block (:53,57).
+    With ``Cbo``/``Cbv`` omitted the reference is restricted and beta = alpha."""
     if len(eri_string) != 4:
         raise ValueError(f"Invalid string given to get_eri: {eri_string}")
+    Cbo = Cao if Cbo is None else Cbo
+    Cbv = Cav if Cbv is None else Cbv
     s = eri_string
     if notation == "phys":
         s = s[0] + s[2] + s[1] + s[3]                       # :46-48
+    blocks = {"O": Cao[:, fcn:], "o": Cbo[:, fcn:], "V": Cav, "v": Cbv}
     Cs = []
     for ch in s:                                            # :52-68
-        if ch in "oO":
-            Cs.append(Cao[:, fcn:])
-        elif ch in "vV":
-            Cs.append(Cav)
-        else:
+        if ch not in blocks:
             raise ValueError(f"Invalid string given to get_eri: {eri_string}")
+        Cs.append(blocks[ch])
     V = tei_transform(gao, *Cs)                             # :94
     if notation == "phys":
         V = np.transpose(V, (0, 2, 1, 3))                   # :96-98
@@ -204,6 +207,24 @@ def rmp2_factors(B, C, eps, o):
     return float(np.sum(K * (2 * K - K.transpose(0, 3, 2, 1)) / D))
 
 
+def rmp2_factors_slabs(B, C, eps, o):
+    """RMP2.jl:33-37 from 3-index factors, one occupied index j at a time: for fixed j,
+    K[i,a,j,b] = sum_P L[i,a,P] L[j,b,P] is an (ov) x v slab and the exchange K[i,b,j,a]
+    lies in the same slab, so memory stays at o v^2 (the o=80, v=420 K is 9 GB)."""
+    Co, Cv = C[:, :o], C[:, o:]
+    v = Cv.shape[1]
+    L = mo_ov_factors(B, Co, Cv)
+    naux = L.shape[2]
+    Lm = L.reshape(o * v, naux, order="F")
+    eo, ev = eps[:o], eps[o:o + v]
+    E = 0.0
+    for j in range(o):
+        Kj = (Lm @ L[j].T).reshape(o, v, v, order="F")          # [i,a,b] = (ia|jb)
+        D = eo[:, None, None] + eo[j] - ev[None, :, None] - ev[None, None, :]
+        E += float(np.sum(Kj * (2.0 * Kj - Kj.transpose(0, 2, 1)) / D))
+    return E
+
+
 def rmp2_grad_factors(B, C, eps, o):
     """dE/dC of the RMP2 energy through the factor route: with K = L L^T and
     G = dE/dK (as in ``rmp2_grad``), dE/dL = (G + G^T) L and L = Co^T B_P Cv gives
@@ -225,6 +246,20 @@ def rmp2_grad_factors(B, C, eps, o):
     dCo = np.einsum("iaP,maP->mi", dL, Z, optimize=True)
     dCv = np.einsum("iaP,inP->na", dL, W, optimize=True)
     return E, np.concatenate([dCo, dCv], axis=1)
+
+
+def rmp2_grad_eps(K, eps, o):
+    """dE/deps of the RMP2 energy (RMP2.jl:33-37) from K = (ia|jb) stored as [i,a,j,b].
+
+    With W = K (2K - K_ibja) / D^2 and D = e_i + e_j - e_{o+a} - e_{o+b}, each orbital
+    energy enters D through two slots: dE/de_k = -(sum_{a,jb} W[k,a,jb] + sum_{ia,b} W[ia,k,b])
+    for occupied k and dE/de_{o+c} = sum_{i,jb} W[i,c,jb] + sum_{ia,j} W[ia,j,c] for virtual c.
+    No symmetry of K is assumed.  Returns an (o + v) vector, occupied first."""
+    v = K.shape[1]
+    D = form_Dijab(eps, o, v).transpose(0, 2, 1, 3)
+    W = K * (2.0 * K - K.transpose(0, 3, 2, 1)) / D ** 2
+    r = W.sum(axis=(2, 3)) + W.sum(axis=(0, 1))          # [i,a]: both slots of (ia)
+    return np.concatenate([-r.sum(axis=1), r.sum(axis=0)])
 
 
 def rmp2_energy_from_C(gao, C, eps, o):

@@ -5,7 +5,10 @@
 cfg1 (n=24, o=5, v=19, naux=72): RMP2, RCCD (mRCCD.jl equations), RCCSD with
 both reference equation sets, the first three plain-Jacobi energies from t=0,
 converged amplitudes.  cfg3 (n=256, o=40, v=216, naux=768): RMP2 energy and dE/dC at 64 seeded (mu, p)
-entries, analytic (factor route) and by central differences with h = 1e-4.
+entries, analytic (factor route), by central differences with h = 1e-5 and by the
+five-point stencil at h = 1e-1 (exact for the quartic E along one entry of C).
+cfg4 (n=400, o=60, v=340, naux=1200) and cfg5 (n=500, o=80, v=420, naux=1500): E_MP2
+through the factor route.
 cfg2 (n=120, o=20, v=100, naux=360): the full-strength
 iteration diverges for these integrals, so RCCD and RCCSD are converged for the
 integrals scaled by a coupling of 0.02 (factors scaled by sqrt(0.02)); norms and
@@ -67,7 +70,13 @@ def cfg2(lam=0.02):
                         T2_mrccsd_sample=T2_m.ravel(order="F")[idx], sample_idx=idx)
 
 
-def cfg3(h=1e-4, count=64):
+def cfg3(h=1e-5, h5=1e-1, count=64):
+    """E_MP2 and dE/dC at 64 seeded entries: analytic (factor route), central differences at
+    the configured h = 1e-5, and the five-point stencil at h5 = 1e-1.  Along one entry of C,
+    E is a polynomial of degree 4 in the step (K is quadratic in C, E quadratic in K), which
+    the five-point stencil differentiates exactly at any step; its error is rounding alone,
+    about 1e-9 / h5 ~ 1e-8 for energy sums carrying ~1e-9 Eh of rounding, where the central
+    difference at 1e-5 carries ~1e-4."""
     s = synthetic.workload("cfg3_mp2grad_n256")
     t0 = time.time()
     E, dC = O.rmp2_grad_factors(s.B, s.C, s.eps, s.o)
@@ -75,21 +84,44 @@ def cfg3(h=1e-4, count=64):
     idx = rng.integers(0, s.n, (count, 2))
     an = np.array([dC[mu, p] for mu, p in idx])
     fd = np.empty(count)
+    fd5 = np.empty(count)
+
+    def E_at(mu, p, step):
+        Cs = s.C.copy()
+        Cs[mu, p] += step
+        return O.rmp2_factors(s.B, Cs, s.eps, s.o)
+
     for k, (mu, p) in enumerate(idx):
-        Cp, Cm = s.C.copy(), s.C.copy()
-        Cp[mu, p] += h
-        Cm[mu, p] -= h
-        fd[k] = (O.rmp2_factors(s.B, Cp, s.eps, s.o) - O.rmp2_factors(s.B, Cm, s.eps, s.o)) / (2 * h)
-        print(f"cfg3 fd {k} ({mu},{p}) an {an[k]:.10f} fd {fd[k]:.10f} ({time.time() - t0:.0f} s)", flush=True)
-    np.savez_compressed(HERE / "cfg3.npz", E_mp2=E, idx=idx, grad=an, fd=fd, fd_h=h)
-    print("cfg3", E, np.max(np.abs(an - fd)))
+        fd[k] = (E_at(mu, p, h) - E_at(mu, p, -h)) / (2 * h)
+        fd5[k] = (-E_at(mu, p, 2 * h5) + 8 * E_at(mu, p, h5) - 8 * E_at(mu, p, -h5) + E_at(mu, p, -2 * h5)) / (12 * h5)
+        print(f"cfg3 fd {k} ({mu},{p}) an {an[k]:.10f} fd {fd[k]:.10f} fd5 {fd5[k]:.10f} "
+              f"({time.time() - t0:.0f} s)", flush=True)
+    np.savez_compressed(HERE / "cfg3.npz", E_mp2=E, idx=idx, grad=an, fd=fd, fd_h=h, fd5=fd5, fd5_h=h5)
+    print("cfg3", E, np.max(np.abs(an - fd)), np.max(np.abs(an - fd5)))
+
+
+def cfg4():
+    """E_MP2 of the o=60, v=340 factors through the factor route, summed in slabs of one
+    occupied index: the first Jacobi step from t = 0 at coupling lambda has energy
+    lambda^2 E_MP2."""
+    s = synthetic.workload("cfg4_rccd_n400")
+    t0 = time.time()
+    E = O.rmp2_factors_slabs(s.B, s.C, s.eps, s.o)
+    np.savez_compressed(HERE / "cfg4.npz", E_mp2=E)
+    print("cfg4", E, time.time() - t0)
+
+
+def cfg5():
+    """E_MP2 of the o=80, v=420 factors (K would be 9 GB: summed in slabs of one j), as for cfg4."""
+    s = synthetic.workload("cfg5_rccsd_n500")
+    t0 = time.time()
+    E = O.rmp2_factors_slabs(s.B, s.C, s.eps, s.o)
+    np.savez_compressed(HERE / "cfg5.npz", E_mp2=E)
+    print("cfg5", E, time.time() - t0)
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["cfg1", "cfg2", "cfg3"]
-    if "cfg1" in which:
-        cfg1()
-    if "cfg2" in which:
-        cfg2()
-    if "cfg3" in which:
-        cfg3()
+    which = sys.argv[1:] or ["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"]
+    for name in ("cfg1", "cfg2", "cfg3", "cfg4", "cfg5"):
+        if name in which:
+            globals()[name]()

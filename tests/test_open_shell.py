@@ -53,6 +53,40 @@ def test_oracle_ump2_cation_known_answer(cation):
     assert abs(E - E2) < 1e-14
 
 
+def _ump2_from_get_eri_blocks(blk, epsa, epsb, na, nb):
+    """UMP2 from get_eri's physicist blocks: 1/4 sum <IJ||AB>^2/D over "OOVV" and "oovv",
+    plus sum <Ij|Ab>^2/D over "OoVv" (upper case alpha, lower case beta)."""
+    def den(e1o, e2o, e1v, e2v):
+        return e1o[:, None, None, None] + e2o[None, :, None, None] - e1v[None, None, :, None] - e2v[None, None, None, :]
+    ea_o, ea_v, eb_o, eb_v = epsa[:na], epsa[na:], epsb[:nb], epsb[nb:]
+    E = 0.0
+    for s, eo, ev in (("OOVV", ea_o, ea_v), ("oovv", eb_o, eb_v)):
+        V = blk(s)
+        A = V - V.transpose(0, 1, 3, 2)
+        E += 0.25 * float(np.sum(A * A / den(eo, eo, ev, ev)))
+    V = blk("OoVv")
+    return E + float(np.sum(V * V / den(ea_o, eb_o, ea_v, eb_v)))
+
+
+def test_oracle_get_eri_spin_blocks_cation(cation):
+    """get_eri's lower-case letters are the beta orbitals (IntegralTransformation.jl:52-68):
+    the alpha, beta and mixed-spin blocks of the UHF cation give the UMP2 known answer, and
+    taking alpha orbitals for the lower-case letters does not."""
+    w = cation
+    Cao, Cav, Cbo, Cbv = _split(w)
+    blk = lambda s: O.get_eri(w["g"], Cao, Cav, s, Cbo=Cbo, Cbv=Cbv)  # noqa: E731
+    assert blk("OoVv").shape == (w["na"], w["nb"], Cav.shape[1], Cbv.shape[1])
+    E = _ump2_from_get_eri_blocks(blk, w["epsa"], w["epsb"], w["na"], w["nb"])
+    assert abs(E - E_UMP2_CATION) < 1e-12
+    # alpha orbitals standing in for the beta ones give a different number
+    wrong = lambda s: O.get_eri(w["g"], Cao, Cav, s.upper())  # noqa: E731
+    Ew = _ump2_from_get_eri_blocks(wrong, w["epsa"], w["epsa"], w["na"], w["na"])
+    assert abs(Ew - E_UMP2_CATION) > 1e-4
+    # fcn drops the first occupied orbitals of either spin (:53,57)
+    f = O.get_eri(w["g"], Cao, Cav, "OoVv", fcn=1, Cbo=Cbo, Cbv=Cbv)
+    assert np.allclose(f, blk("OoVv")[1:, 1:])
+
+
 def test_oracle_ump2_closed_shell_is_rmp2(water):
     w = water
     o = w["nocc"]
@@ -86,6 +120,44 @@ def test_gpu_ump2_cation_known_answer(cation):
     wf = Wfn.from_uhf(w["Ca"], w["Cb"], w["epsa"], w["epsb"], w["na"], w["nb"], np.asfortranarray(w["g"]))
     assert abs(MollerPlesset.do_ump2(wf) - E_UMP2_CATION) < 1e-12
     assert abs(MollerPlesset.do_ump2(wf, prec="fp32_mixed") - E_UMP2_CATION) < 1e-6 * abs(E_UMP2_CATION)
+
+
+@pytest.mark.gpu
+def test_gpu_get_eri_spin_blocks_cation(cation):
+    """Mixed-spin get_eri blocks of the UHF cation against the oracle; the device blocks
+    reproduce the UMP2 known answer."""
+    from jues import IntegralTransformation as IT
+    from jues.Wavefunction import Wfn
+    w = cation
+    Cao, Cav, Cbo, Cbv = _split(w)
+    wf = Wfn.from_uhf(w["Ca"], w["Cb"], w["epsa"], w["epsb"], w["na"], w["nb"], np.asfortranarray(w["g"]))
+    for s in ("OoVv", "oOvV", "oovv", "OOVV", "OvOv", "ovOV", "vVoO"):
+        for note in ("phys", "chem"):
+            ref = O.get_eri(w["g"], Cao, Cav, s, notation=note, Cbo=Cbo, Cbv=Cbv)
+            got = IT.get_eri(wf, s, notation=note)
+            assert got.shape == ref.shape, (s, note)
+            assert np.abs(got - ref).max() < 1e-13, (s, note)
+    ref = O.get_eri(w["g"], Cao, Cav, "OoVv", fcn=1, Cbo=Cbo, Cbv=Cbv)
+    assert np.abs(IT.get_eri(wf, "OoVv", fcn=1) - ref).max() < 1e-13
+    E = _ump2_from_get_eri_blocks(lambda s: IT.get_eri(wf, s), w["epsa"], w["epsb"], w["na"], w["nb"])
+    assert abs(E - E_UMP2_CATION) < 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_get_eri_spin_blocks_random_factors():
+    """Beta blocks of a different size than the alpha ones, through the factor route."""
+    from jues import IntegralTransformation as IT
+    from jues.Wavefunction import Wfn, FactoredERI
+    s = _random_uhf()
+    na, nb = s["na"], s["nb"]
+    Cao, Cav, Cbo, Cbv = s["Ca"][:, :na], s["Ca"][:, na:], s["Cb"][:, :nb], s["Cb"][:, nb:]
+    wf = Wfn.from_uhf(s["Ca"], s["Cb"], s["epsa"], s["epsb"], na, nb, FactoredERI(s["B"]))
+    for blk in ("OoVv", "ooVV", "vOoV", "oooo"):
+        ref = O.get_eri(s["g"], Cao, Cav, blk, Cbo=Cbo, Cbv=Cbv)
+        for prec, tol in (("fp64", 1e-12), ("fp32_mixed", 1e-5)):
+            got = IT.get_eri(wf, blk, prec=prec)
+            assert got.shape == ref.shape
+            assert np.abs(got - ref).max() < tol * np.abs(ref).max(), (blk, prec)
 
 
 @pytest.mark.gpu

@@ -126,15 +126,59 @@ def test_rmp2_factor_route_gradient():
         assert abs(fd - dC2[mu, p]) < 1e-8 * max(1, abs(fd))
 
 
+def test_rmp2_factors_slabs():
+    """The slab-by-slab factor-route energy (used for the o=60/v=340 and o=80/v=420 fixtures)
+    equals the dense restatement on a system where direct and exchange terms differ."""
+    s = tiny(n=11, o=4, naux=15, scale=0.5)
+    E0 = O.rmp2(O.get_eri(s.ao_eri(), s.Co, s.Cv, "OOVV"), s.eps, s.o)
+    assert abs(O.rmp2_factors_slabs(s.B, s.C, s.eps, s.o) - E0) < 1e-13 * abs(E0)
+    assert abs(O.rmp2_factors(s.B, s.C, s.eps, s.o) - E0) < 1e-13 * abs(E0)
+    K = O.get_eri(s.ao_eri(), s.Co, s.Cv, "OOVV")
+    assert np.abs(K - K.transpose(0, 1, 3, 2)).max() > 1e-3 * np.abs(K).max()
+
+
+def test_rmp2_grad_eps():
+    """dE/deps: central differences on every orbital energy (with a K that has no pair
+    symmetry, so both slots of each orbital are exercised), plus two exact invariants
+    of D = e_i + e_j - e_a - e_b: a uniform shift leaves E unchanged (sum_p dE/de_p = 0)
+    and E is homogeneous of degree -1 in eps (sum_p e_p dE/de_p = -E)."""
+    s = tiny(n=8, o=3, naux=12, scale=0.5)
+    o, v = s.o, s.v
+    rng = np.random.default_rng(11)
+    K = O.tei_transform(s.ao_eri(), s.Co, s.Cv, s.Co, s.Cv) + 0.02 * rng.normal(size=(o, v, o, v))
+    E_of = lambda e: O.rmp2(K.transpose(0, 2, 1, 3), e, o)  # noqa: E731
+    de = O.rmp2_grad_eps(K, s.eps, o)
+    assert de.shape == (s.n,)
+    h = 1e-6
+    for p in range(s.n):
+        ep, em = s.eps.copy(), s.eps.copy()
+        ep[p] += h
+        em[p] -= h
+        fd = (E_of(ep) - E_of(em)) / (2 * h)
+        assert abs(fd - de[p]) < 1e-8 * max(1.0, abs(fd)), p
+    E = E_of(s.eps)
+    assert abs(de.sum()) < 1e-13 * np.abs(de).max()
+    assert abs(np.dot(s.eps, de) + E) < 1e-13 * abs(E)
+    # a dropped slot (only the (ia) position of each orbital) fails the FD check
+    D = O.form_Dijab(s.eps, o, v).transpose(0, 2, 1, 3)
+    W = K * (2 * K - K.transpose(0, 3, 2, 1)) / D ** 2
+    one_slot = -W.sum(axis=(1, 2, 3))
+    assert np.abs(one_slot - de[:o]).max() > 1e-3 * np.abs(de[:o]).max()
+
+
 def test_golden_cfg3_consistent():
-    """cfg3 fixture: the stored analytic entries agree with their stored central
-    differences to the finite-difference noise floor (E ~ 6e5 Eh, h = 1e-4)."""
+    """cfg3 fixture: the stored analytic entries agree with the five-point differences (exact
+    for the quartic E along one entry of C, rounding ~1e-8) to 1e-7, and with the central
+    differences at h = 1e-5 to their rounding floor (E ~ 6.4e5 Eh)."""
     path = GOLD / "cfg3.npz"
     if not path.exists():
         pytest.skip("cfg3 fixture not generated")
     z = np.load(path)
     assert z["grad"].shape == (64,) and z["idx"].shape == (64, 2)
-    assert np.all(np.abs(z["grad"] - z["fd"]) < 1e-7 * np.abs(z["fd"]) + 5e-5)
+    assert float(z["fd_h"]) == 1e-5
+    assert np.max(np.abs(z["grad"] - z["fd5"])) < 1e-7
+    floor = 200 * np.finfo(float).eps * abs(float(z["E_mp2"])) / float(z["fd_h"])
+    assert np.all(np.abs(z["grad"] - z["fd"]) < 1e-7 * np.abs(z["fd"]) + floor)
 
 
 # ---------------------------------------------------------------- RCCD

@@ -212,6 +212,94 @@ def test_device_rccsd_sequence_matches_oracle(n, o, naux, seed):
     assert np.max(np.abs(d2 - r2)) < 1e-12 * max(1.0, np.max(np.abs(r2)))
 
 
+def _gemm_F(buf_a, lda, off_a, ta, buf_b, ldb, off_b, tb, M, N, K, ldc):
+    """C (M x N, leading dimension ldc) = op(A) op(B) on flat column-major buffers, as the
+    device's gemm reads them."""
+    def mat(buf, ld, off, rows, cols):
+        return np.lib.stride_tricks.as_strided(buf[off:], shape=(rows, cols), strides=(8, 8 * ld))
+    A = mat(buf_a, lda, off_a, K, M).T if ta else mat(buf_a, lda, off_a, M, K)
+    B = mat(buf_b, ldb, off_b, N, K).T if tb else mat(buf_b, ldb, off_b, K, N)
+    return A @ B
+
+
+def _factor_ovvv_terms(t1, t2, B, Co, Cv, mrccsd):
+    """The device's (ia|bc) terms through the factors (ccsd.hip with AO factors), each GEMM
+    restated on the flat column-major buffers with the device's leading dimensions, batch
+    strides and interleaved outputs: returns the Fae, n1, Lz, WJ, WE and H contributions."""
+    o, v = t1.shape
+    Lov = O.mo_ov_factors(B, Co, Cv)                          # [m,e,P]
+    naux = Lov.shape[2]
+    Lvv = np.einsum("ma,mnP,nb->abP", Cv, B, Cv)              # [a,e,P]
+    f = lambda x: np.ascontiguousarray(np.asarray(x).ravel(order="F"))  # noqa: E731
+    lov, lvv, T1 = f(Lov), f(Lvv), f(t1)
+    lovT, lovp = f(Lov.transpose(1, 0, 2)), f(Lov.transpose(0, 2, 1))
+    ov, vv, oo = o * v, v * v, o * o
+    # B1[m,P,e] (ldc = o naux, batch stride o) and Y2[m,e,P] (ldc = o, batch stride ov)
+    B1, Y2, Y3 = np.zeros(ov * naux), np.zeros(ov * naux), np.zeros(oo * naux)
+    for P in range(naux):
+        c = _gemm_F(T1, o, 0, False, lvv, v, vv * P, True, o, v, v, 0)
+        for e in range(v):
+            B1[o * P + o * naux * e: o * P + o * naux * e + o] = c[:, e]
+        Y2[ov * P: ov * (P + 1)] = f(c)
+        Y3[oo * P: oo * (P + 1)] = f(_gemm_F(lov, o, ov * P, False, T1, o, 0, True, o, o, v, 0))
+    g = _gemm_F(lov, ov, 0, True, T1, ov, 0, False, naux, 1, ov, 0)
+    Fae = 2 * _gemm_F(lvv, vv, 0, False, f(g), naux, 0, False, vv, 1, naux, 0).reshape(v, v, order="F")
+    A_, B_ = (lovp, B1) if mrccsd else (B1, lovp)
+    Fae -= _gemm_F(A_, o * naux, 0, True, B_, o * naux, 0, False, v, v, o * naux, 0)
+    Tov = f(t2.transpose(0, 2, 1, 3))
+    W1 = f(_gemm_F(Tov, ov, 0, False, lov, ov, 0, False, ov, naux, ov, 0))
+    n1 = 2 * _gemm_F(W1, o, 0, False, lvv, v, 0, True, o, v, v * naux, 0)
+    W2 = np.zeros(ov * naux)
+    for fi in range(v):                                       # batch over f, ldc = ov, stride o
+        c = _gemm_F(Tov, o, oo * v * fi, False, lovT, ov, 0, False, o, naux, ov, 0)
+        for P in range(naux):
+            W2[o * fi + ov * P: o * fi + ov * P + o] = c[:, P]
+    n1 -= _gemm_F(W2, o, 0, False, lvv, v, 0, True, o, v, v * naux, 0)
+    # Z[i,j,a,m] over slabs of m, then Lz[(ija),b] = -Z[(ija),m] t1[m,b]
+    tau = f(t2 + np.einsum("ia,jb->ijab", t1, t1))
+    Z = np.zeros(oo * v * o)
+    ms = 2
+    for m0 in range(0, o, ms):
+        cnt = min(ms, o - m0)
+        Zs = _gemm_F(lovT, ov, v * m0, False, lvv, vv, 0, True, v * cnt, vv, naux, 0)
+        Zsp = f(Zs.reshape(v, cnt, v, v, order="F").transpose(3, 0, 2, 1))
+        Z[oo * v * m0: oo * v * (m0 + cnt)] = f(_gemm_F(tau, oo, 0, False, Zsp, vv, 0, False, oo, v * cnt, vv, 0))
+    Lz = -_gemm_F(Z, oo * v, 0, False, T1, o, 0, False, oo * v, v, o, 0).reshape(o, o, v, v, order="F")
+    WJ = _gemm_F(lov, ov, 0, False, Y2, ov, 0, True, ov, ov, naux, 0).reshape(o, v, o, v, order="F")
+    Qm = _gemm_F(Y3, oo, 0, False, lvv, vv, 0, True, oo, vv, naux, 0).reshape(o, o, v, v, order="F")
+    WE = -Qm.transpose(0, 3, 1, 2)
+    H = _gemm_F(Y2, ov, 0, False, lov, ov, 0, True, ov, ov, naux, 0).reshape(o, v, o, v, order="F")
+    return Fae, n1, Lz, WJ, WE, H
+
+
+@pytest.mark.parametrize("mrccsd", [True, False])
+def test_factored_ovvv_terms_equal_stored_block(mrccsd):
+    """The (ia|bc) terms of the factor route equal the stored-block contractions of the device
+    sequence above (and so, through it, mRCCSD.jl / RCCSD.jl) to 1e-12, on factors without
+    (mu,nu) symmetry; swapping the two Fae variants breaks the Fae check."""
+    s = synthetic.make_system(9, 3, 6, 13, 31, name="fac")
+    rng = np.random.default_rng(5)
+    B = np.asfortranarray(s.B + 0.05 * rng.normal(size=s.B.shape))
+    t1 = rng.normal(size=(s.o, s.v)) * 0.1
+    t2 = _amplitudes(s.o, s.v, 77)
+    ovvv = O.mo_from_factors(B, s.Co, s.Cv, s.Cv, s.Cv)
+    es = np.einsum
+    tau = t2 + es("ia,jb->ijab", t1, t1)
+    Tov = t2.transpose(0, 2, 1, 3)
+    ref_Fae = 2 * es("mf,mfae->ae", t1, ovvv) - es("mf,maef->ae" if mrccsd else "mf,meaf->ae", t1, ovvv)
+    other = 2 * es("mf,mfae->ae", t1, ovvv) - es("mf,meaf->ae" if mrccsd else "mf,maef->ae", t1, ovvv)
+    ref_n1 = 2 * es("iemf,mfae->ia", Tov, ovvv) - es("iemf,meaf->ia", Tov, ovvv)
+    ref_Lz = -es("mb,ijma->ijab", t1, es("ijef,mfae->ijma", tau, ovvv))
+    ref_WJ = es("jf,mebf->mejb", t1, ovvv)
+    ref_WE = -es("jf,mfbe->mejb", t1, ovvv)
+    ref_H = es("ie,jbae->iajb", t1, ovvv)
+    got = _factor_ovvv_terms(t1, t2, B, s.Co, s.Cv, mrccsd)
+    for name, g, r in zip(("Fae", "n1", "Lz", "WJ", "WE", "H"), got,
+                          (ref_Fae, ref_n1, ref_Lz, ref_WJ, ref_WE, ref_H)):
+        assert np.max(np.abs(g - r)) < 1e-12 * max(1.0, np.max(np.abs(r))), name
+    assert np.max(np.abs(got[0] - other)) > 1e-4 * np.max(np.abs(other))
+
+
 # ---------------------------------------------------------------- recomputed ladder
 
 @pytest.mark.parametrize("nf", [1, 2, 4, 6])
