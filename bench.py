@@ -112,6 +112,9 @@ def main():
     ap.add_argument("--pt-prec", default="fp64,fp32_mixed", help="--triples: forms to run")
     ap.add_argument("--n4", action="store_true",
                     help="instead of RCCD: UMP2, get_fock and Fock-driven CCSD iterations at o=60 v=340")
+    ap.add_argument("--rccsd5", action="store_true",
+                    help="instead of RCCD: RCCSD at o=80 v=420 (n=500, naux=1500), recomputed ladder, factored "
+                         "(ia|bc); s/iter of the three forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: measures one MI355X only; the iteration is not sharded across GPUs",
@@ -128,6 +131,9 @@ def main():
         return
     if args.n4:
         print(json.dumps(n4_bench()), flush=True)
+        return
+    if args.rccsd5:
+        print(json.dumps(rccsd5_bench(args.coupling, args.steps, args.warmup, args.diis_max)), flush=True)
         return
 
     from jues import synthetic, _lib
@@ -182,14 +188,17 @@ def main():
                    "ladder": args.ladder},
         "fp64_s_per_iter": round(d["s_per_iter"], 4),
         "speedup_mixed_vs_fp64": round(d["s_per_iter"] / m["s_per_iter"], 3),
-        "tflops_canonical": {"fp32_mixed": round(F / m["s_per_iter"] / 1e12, 2),
-                             "fp64": round(F / d["s_per_iter"] / 1e12, 2), "flops_per_iter": F},
+        # canonical flops over time: the packed ladders execute about a quarter of their canonical
+        # flops, so this effective rate can exceed the MFMA peak (tflops_executed is the MFMA work)
+        "tflops_effective_canonical": {"fp32_mixed": round(F / m["s_per_iter"] / 1e12, 2),
+                                       "fp64": round(F / d["s_per_iter"] / 1e12, 2), "flops_per_iter": F,
+                                       "note": "canonical F_RCCD / time; not an MFMA rate"},
         "tflops_executed": {"fp32_mixed": round(executed(m["stats"]) / args.steps / m["s_per_iter"] / 1e12, 2),
                             "fp64": round(executed(d["stats"]) / args.steps / d["s_per_iter"] / 1e12, 2),
                             "flops_per_iter": executed(m["stats"]) / args.steps},
         "fp16x3": {"s_per_iter": round(h["s_per_iter"], 4),
                    "speedup_vs_fp64": round(d["s_per_iter"] / h["s_per_iter"], 3),
-                   "tflops_canonical": round(F / h["s_per_iter"] / 1e12, 2),
+                   "tflops_effective_canonical": round(F / h["s_per_iter"] / 1e12, 2),
                    "roofline": roofline_split(h["stats"], args.workload)},
         "roofline": roofline(m["stats"], "f32", args.workload),
         "roofline_fp64": roofline(d["stats"], "f64", args.workload),
@@ -204,7 +213,7 @@ def main():
         cb = cpu_baseline.rccd_iteration_sample(o, v, k=args.cpu_k)
         out["cpu_baseline"] = {"value": round(cb["seconds_per_iter"], 2), "unit": "s/iter",
                                "cores": cb["threads"], "kind": "port", "sample": cb["sample"],
-                               "host_cpus": os.cpu_count()}
+                               "host_cpus": os.cpu_count(), **cpu_baseline.host_info()}
     if args.converge:
         out["convergence"] = converge(w, args.coupling, args.fast_r_conv, args.keep_diis, args.forms.split(","))
     print(json.dumps(out), flush=True)
@@ -264,7 +273,65 @@ def mp2_bench() -> dict:
     out["E_mp2_fp64"] = E64
     out["E_mp2_fp32_mixed_abs_diff"] = abs(E32 - E64)
     out["grad_E"] = res[0]
+    del g, G2, w
+    ctx.release()
+    out["factor_route"] = {name: mp2_factor_adjoint(name, fd_entries=4 if name.startswith("cfg5") else 0)
+                           for name in ("cfg3_mp2grad_n256", "cfg5_rccsd_n500")}
     return out
+
+
+def mp2_factor_adjoint(name: str, fd_entries: int = 0) -> dict:
+    """The MP2 adjoint through the factor route on a seeded workload (no n^4 tensor, no n^2 o v
+    half-transform), FP64 and FP32-mixed; with fd_entries, dE/dC at that many seeded entries
+    against the five-point stencil of the device energy at h = 0.1 (exact for E, a quartic
+    along one entry of C, up to rounding)."""
+    import numpy as np
+    from jues import synthetic, MollerPlesset, _lib
+    from jues.Wavefunction import Wfn, FactoredERI
+    s = synthetic.workload(name)
+    n, o, v, naux = s.n, s.o, s.v, s.naux
+    ctx = _lib.context(0)
+    wf = lambda C: Wfn.from_orbitals(C, s.eps, o, FactoredERI(s.B))  # noqa: E731
+    ov = o * v
+    # mo factors, K = L L^T, dL = 2 G L, B_P Cv and B_P^T Co, and the two dC products
+    F = (2 * n * n * o * naux + 2 * n * o * v * naux + 4 * ov * ov * naux + 2 * n * n * v * naux
+         + 2 * n * n * o * naux + 4 * n * o * v * naux)
+    rec = {"workload": f"RMP2 adjoint, factor route, n={n} o={o} v={v} naux={naux}", "flops": F}
+    for prec, peak in (("fp64", 78.6), ("fp32_mixed", 157.3)):
+        MollerPlesset.rmp2_grad(wf(s.C), want_dg=False, prec=prec)
+        ctx.stats_reset(True)
+        t0 = time.perf_counter()
+        E, dC, _, de = MollerPlesset.rmp2_grad(wf(s.C), want_dg=False, prec=prec)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.stats_reset(False)
+        dev = sum(x["ms"] for x in st.values())
+        rec[prec] = {"E": E, "device_ms": round(dev, 2), "wall_s_incl_host_copies": round(wall, 3),
+                     "TFLOP/s": round(F / dev / 1e9, 1), "frac_of_peak": round(F / dev / 1e9 / peak, 3),
+                     "kernels_ms": {k: round(x["ms"], 2) for k, x in st.items()}}
+        if prec == "fp64":
+            dC64, de64 = dC, de
+        else:
+            rec[prec]["rel_err_dC_vs_fp64"] = float(np.abs(dC - dC64).max() / np.abs(dC64).max())
+            rec[prec]["rel_err_E_vs_fp64"] = abs(E - rec["fp64"]["E"]) / abs(rec["fp64"]["E"])
+        print(f"[bench] {name} adjoint {prec}: device {dev:.1f} ms, {F / dev / 1e9:.1f} TFLOP/s", file=sys.stderr,
+              flush=True)
+    if fd_entries:
+        rng = np.random.default_rng(5)
+        h = 0.1
+        checks = []
+        for mu, p in rng.integers(0, n, (fd_entries, 2)):
+            def E_at(step):
+                C = s.C.copy()
+                C[mu, p] += step
+                return MollerPlesset.do_rmp2(wf(C))
+            fd5 = (-E_at(2 * h) + 8 * E_at(h) - 8 * E_at(-h) + E_at(-2 * h)) / (12 * h)
+            checks.append({"entry": [int(mu), int(p)], "analytic": float(dC64[mu, p]), "fd5": fd5,
+                           "abs_diff": abs(fd5 - float(dC64[mu, p]))})
+        rec["fd5_check_h0.1"] = checks
+        print(f"[bench] {name} fd5 max |diff| {max(c['abs_diff'] for c in checks):.2e}", file=sys.stderr, flush=True)
+    ctx.release()
+    return rec
 
 
 def df_bench(coupling: float) -> dict:
@@ -417,6 +484,46 @@ def n4_bench() -> dict:
             ms = [round(x["ms"], 1) for x in hist if x["stage"] == (0 if prec == "fp64" else 1)]
             out["stages"][f"{label}_{prec}"] = {"ms_per_iteration": ms}
             print(f"[bench] {label} {prec}: {ms} ms/iter", file=sys.stderr, flush=True)
+    return out
+
+
+def rccsd5_bench(coupling: float, steps: int, warmup: int, diis_max: int) -> dict:
+    """RCCSD at o=80, v=420 (n=500, naux=1500) on seeded factors: the pp-ladder rebuilt from the
+    factors every iteration and (ia|bc) never stored.  One resident state, timed per form as the
+    RCCD benchmark does (FP32-mixed, split-FP16, FP64); device memory held after each stage."""
+    from jues import synthetic, _lib
+    from jues.CoupledCluster import Solver
+    from jues.Wavefunction import Wfn
+    s = synthetic.workload("cfg5_rccsd_n500")
+    o, v, naux = s.o, s.v, s.naux
+    w = Wfn.from_synthetic(s, dense=False)
+    t0 = time.perf_counter()
+    solver = Solver(w, method="rccsd", prec="fp16x3", guess="zero", diis=True, diis_max=min(diis_max, 3),
+                    coupling=coupling, ladder="recompute")
+    ctx = solver.ctx
+    out = {"workload": f"RCCSD o={o} v={v} n={s.n} naux={naux}, B-factor integrals, pp-ladder recomputed "
+                       "per iteration, (ia|bc) factored", "coupling": coupling, "setup_s": round(time.perf_counter() - t0, 1),
+           "device_gib_after_setup": round(ctx.lib.jues_device_bytes(ctx.h) / 2**30, 1), "forms": {}}
+    print(f"[bench] rccsd5 setup {out['setup_s']} s, {out['device_gib_after_setup']} GiB", file=sys.stderr, flush=True)
+    for stage, prec in (("fp32", "fp32_mixed"), ("fp16x3", "fp16x3"), ("fp64", "fp64")):
+        solver.reset_diis()
+        if warmup:
+            solver.iterate(warmup, prec=prec)
+        ctx.stats_reset(True)
+        t0 = time.perf_counter()
+        info, _ = solver.iterate(steps, prec=prec)
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+        ctx.stats_reset(False)
+        ex = sum(x["flops"] for x in st.values()) / steps
+        out["forms"][stage] = {"s_per_iter": round(dt / steps, 3), "E": info["energy"], "rnorm": info["rnorm"],
+                               "executed_tflops": round(ex / (dt / steps) / 1e12, 1),
+                               "device_gib": round(ctx.lib.jues_device_bytes(ctx.h) / 2**30, 1),
+                               "kernel_ms_per_iter": {k: round(x["ms"] / steps, 1) for k, x in st.items()}}
+        print(f"[bench] rccsd5 {stage}: {dt / steps:.3f} s/iter, E={info['energy']:.10f}, "
+              f"{out['forms'][stage]['device_gib']} GiB", file=sys.stderr, flush=True)
+    f = out["forms"]
+    out["speedup_vs_fp64"] = {k: round(f["fp64"]["s_per_iter"] / f[k]["s_per_iter"], 2) for k in ("fp32", "fp16x3")}
     return out
 
 

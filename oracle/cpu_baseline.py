@@ -34,6 +34,26 @@ def _blas_threads() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
+def host_info() -> dict:
+    """CPU model, sockets and RAM of the host the baseline runs on (from /proc)."""
+    info = {}
+    try:
+        txt = open("/proc/cpuinfo").read()
+        models = [ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ln.startswith("model name")]
+        if models:
+            info["cpu_model"] = models[0]
+        info["sockets"] = len({ln for ln in txt.splitlines() if ln.startswith("physical id")}) or 1
+    except OSError:
+        pass
+    try:
+        for ln in open("/proc/meminfo"):
+            if ln.startswith("MemTotal"):
+                info["ram_gib"] = round(int(ln.split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    return info
+
+
 def rccd_iteration_sample(o: int, v: int, k: int = 32, seed: int = 0) -> dict:
     rng = np.random.default_rng(seed)
     ov, oo, vv = o * v, o * o, v * v
