@@ -172,6 +172,22 @@ jues_status jues_get_eri(jues_ctx* ctx, int prec, const char* eri_string, const 
 jues_status jues_rmp2(jues_ctx* ctx, int prec, const double* Cocc, int o, const double* Cvir,
                       int v, int n, const double* eps, double* E);
 
+/* ---- Allocation-free RMP2 (no reference counterpart) -----------------------------
+ * For callers that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The FP64 RMP2 energy of jues_rmp2 (same
+ * inputs, prec must be JUES_FP64) from the context's AO factors, which must already be set
+ * (jues_set_ao_factors / jues_set_df_integrals allocate; this call does not).  Every device
+ * intermediate lives in one static array of JUES_NOALLOC_SCRATCH_BYTES per GPU, which each
+ * GPU the library runs on holds until the process ends; the call makes no hipMalloc/hipFree
+ * and touches no context scratch.  It needs, in 8-byte elements with each region rounded up
+ * to 256 bytes, n*o + n*v + (o+v) + (o*v+1) + max(n*min(o,v), o*v)*naux + o*v*naux + o*v*v
+ * (one occupied index's slab of (ia|jb)); larger problems return JUES_BAD_ARG with the byte
+ * count (o=60, v=340, n=400, naux=1200 needs 483 MB and fits).  Calls on one GPU share the
+ * array: calls that overlap on one GPU (two threads, two contexts) race. */
+#define JUES_NOALLOC_SCRATCH_BYTES (512ull << 20)
+jues_status jues_rmp2_noalloc(jues_ctx* ctx, int prec, const double* Cocc, int o, const double* Cvir,
+                              int v, int n, const double* eps, double* E);
+
 /* MP2 adjoint (no reference counterpart; the reverse rule of jues_rmp2 for a
  * ChainRulesCore rrule).  C is the full n x n matrix [Cocc Cvir] (o occupied
  * columns first), eps the n orbital energies (occupied first).  Outputs, each may be

@@ -15,6 +15,12 @@ jues_status jues_create(jues_ctx** out, int device) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return JUES_HIP_ERROR;
   if (hipSetDevice(device) != hipSuccess) return JUES_HIP_ERROR;
+  try {
+    noalloc_attach(device);  // the allocation-free entry point's scratch address on this device
+  } catch (...) {
+    (void)hipGetLastError();
+    return JUES_HIP_ERROR;
+  }
   jues_ctx* c = new jues_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -1015,7 +1015,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   JUES_REQUIRE(tiles_m * tiles_n < (1L << 31), "gemm: too many tiles");
   // Few output tiles and a long K (the Fock-like terms): split K over workgroups,
   // write partial products to a workspace and reduce them in a fixed order.
-  if (allow_split && batch == 1 && tiles_m * tiles_n < 128 && K >= 4096) {
+  if (allow_split && batch == 1 && tiles_m * tiles_n < 128 && K >= 4096 && !no_alloc_active()) {
     const long want = std::max(2L, std::min(K / 1024, 512L / (tiles_m * tiles_n)));
     const int BKs = sizeof(T) == 8 ? 16 : 32;
     long kc = (K + want - 1) / want;
@@ -1047,7 +1047,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (beta != T(0) ? 2.0 : 1.0) * M * N) * batch;
   // Short, wide NN products (M <= 64): gemm32_sk_kernel / gemm64_sk_kernel, 64x256 tiles; a
   // transposed A (at most 64 x K, the transform's orbital block) is transposed first
-  if (shape == 1 && ta && !tb && lda == K && batch == 1 && K % 16 == 0 &&
+  if (shape == 1 && ta && !tb && lda == K && batch == 1 && K % 16 == 0 && !no_alloc_active() &&
       (long)sizeof(T) * (256 + 64) * (std::max(M, ldb) + 1) < (1L << 31)) {
     T* At = static_cast<T*>(scratch(ctx, 18, sizeof(T) * (size_t)(M * K)));
     permute<T, T>(ctx, A, {K, M}, {1, 0}, At, 1.0, 0.0, name);

@@ -54,6 +54,19 @@ struct DevBuf {
 
 size_t device_bytes_in_use();
 
+// Allocation-free scope: while one is active on the calling thread, DevBuf::alloc and
+// scratch() raise instead of allocating, and gemm() avoids its workspace paths (split-K,
+// the transposed copy of a short A).  Used by the entry points that must not allocate.
+struct NoAllocScope {
+  bool prev;
+  NoAllocScope();
+  ~NoAllocScope();
+};
+bool no_alloc_active();
+// looks up the device address of the allocation-free entry point's static scratch array on
+// `device` (current) once; called by jues_create (noalloc.hip)
+void noalloc_attach(int device);
+
 struct KStat {
   double ms = 0, calls = 0, flops = 0, bytes = 0;
 };

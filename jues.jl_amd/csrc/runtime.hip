@@ -8,9 +8,15 @@ namespace jues {
 static std::atomic<size_t> g_bytes{0};
 size_t device_bytes_in_use() { return g_bytes.load(); }
 
+static thread_local bool tl_no_alloc = false;
+NoAllocScope::NoAllocScope() : prev(tl_no_alloc) { tl_no_alloc = true; }
+NoAllocScope::~NoAllocScope() { tl_no_alloc = prev; }
+bool no_alloc_active() { return tl_no_alloc; }
+
 void DevBuf::alloc(size_t nbytes) {
   release();
   if (nbytes == 0) return;
+  if (tl_no_alloc) throw Error(JUES_BAD_ARG, "device allocation inside an allocation-free entry point");
   void* q = nullptr;
   const hipError_t e = hipMalloc(&q, nbytes);
   if (e != hipSuccess) {
@@ -43,6 +49,7 @@ std::map<const jues_ctx*, ScratchPool>& pools() {
 }  // namespace
 
 void* scratch(jues_ctx* ctx, int slot, size_t bytes) {
+  if (tl_no_alloc) throw Error(JUES_BAD_ARG, "context scratch inside an allocation-free entry point");
   DevBuf& b = pools()[ctx].slot[slot];
   if (b.bytes < bytes) {
     if (b.p) JUES_HIP(hipStreamSynchronize(ctx->stream));

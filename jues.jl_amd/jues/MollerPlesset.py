@@ -21,6 +21,23 @@ def do_rmp2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0, **kwargs) -> fl
     return float(E[0])
 
 
+def do_rmp2_noalloc(refWfn: Wfn, *, device: int = 0) -> float:
+    """``do_rmp2`` in FP64 through ``jues_rmp2_noalloc``: no device allocation during the
+    call (every intermediate in the library's static per-GPU scratch array).  The AO
+    integrals must be factored (``FactoredERI`` / ``DFIntegrals``); uploading them, on
+    first use, is the only allocating step and happens before the call."""
+    if not is_factored(refWfn.ao_eri):
+        raise TypeError("do_rmp2_noalloc needs FactoredERI or DFIntegrals integrals")
+    ctx = _lib.context(device)
+    n = ensure_ao(ctx, refWfn.ao_eri)
+    Co, Cv = _lib.f64(refWfn.Cao), _lib.f64(refWfn.Cav)
+    eps = _lib.f64(refWfn.epsa)
+    E = np.zeros(1)
+    ctx.check(ctx.lib.jues_rmp2_noalloc(ctx.h, _lib.FP64, _lib.ptr(Co), refWfn.nalpha, _lib.ptr(Cv),
+                                        refWfn.nvira, n, _lib.ptr(eps), _lib.ptr(E)))
+    return float(E[0])
+
+
 def do_df_rmp2(refWfn: Wfn, *, prec: str = "fp64", device: int = 0) -> float:
     """Density-fitted closed-shell MP2 (DF-RMP2.jl:1-46).
 
@@ -73,4 +90,4 @@ def rmp2_grad(refWfn: Wfn, *, want_dg: bool = True, prec: str = "fp64", device: 
     return float(E[0]), dC, dg, deps
 
 
-__all__ = ["do_rmp2", "do_df_rmp2", "do_ump2", "rmp2_grad"]
+__all__ = ["do_rmp2", "do_rmp2_noalloc", "do_df_rmp2", "do_ump2", "rmp2_grad"]

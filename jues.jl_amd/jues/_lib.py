@@ -16,6 +16,7 @@ _HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = pathlib.Path(os.environ.get("JUES_HIP_LIB", _HERE.parent / "lib" / "libjues_hip.so"))
 
 JUES_OK, JUES_NOT_CONVERGED, JUES_NONFINITE, JUES_OOM, JUES_BAD_ARG, JUES_HIP_ERROR = range(6)
+NOALLOC_SCRATCH_BYTES = 512 << 20  # JUES_NOALLOC_SCRATCH_BYTES
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "NONFINITE", 3: "OOM", 4: "BAD_ARG", 5: "HIP_ERROR"}
 FP64, FP32_MIXED, FP16X3_MIXED = 0, 1, 2
 LADDER = {"auto": 0, "stored": 1, "recompute": 2}
@@ -27,7 +28,7 @@ EXPORTS = (
     "jues_create", "jues_destroy", "jues_last_error", "jues_cc_default_opts", "jues_device_bytes",
     "jues_release", "jues_set_ao_eri", "jues_set_ao_factors", "jues_set_df_integrals", "jues_get_ao_factors",
     "jues_tei_transform", "jues_get_eri",
-    "jues_rmp2", "jues_rmp2_grad", "jues_rccd", "jues_rccsd", "jues_cc_setup", "jues_cc_iterate",
+    "jues_rmp2", "jues_rmp2_noalloc", "jues_rmp2_grad", "jues_rccd", "jues_rccsd", "jues_cc_setup", "jues_cc_iterate",
     "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test", "jues_gemm_pair_test",
     "jues_compute_pT", "jues_rccsd_pT", "jues_ump2", "jues_get_fock", "jues_rccsd_fock",
 )
@@ -100,6 +101,7 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "jues_tei_transform": (i, [_ctxp, i, _dp, i, _dp, i, _dp, i, _dp, i, i, _dp]),
             "jues_get_eri": (i, [_ctxp, i, C.c_char_p, C.c_char_p, _dp, i, _dp, i, _dp, i, _dp, i, i, i, _dp]),
             "jues_rmp2": (i, [_ctxp, i, _dp, i, _dp, i, i, _dp, _dp]),
+            "jues_rmp2_noalloc": (i, [_ctxp, i, _dp, i, _dp, i, i, _dp, _dp]),
             "jues_rmp2_grad": (i, [_ctxp, i, _dp, i, i, _dp, _dp, _dp, _dp, _dp]),
             "jues_rccd": (i, [_ctxp, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp, ITER_CB, C.c_void_p]),
             "jues_rccsd": (i, [_ctxp, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp, _dp, ITER_CB,

@@ -218,7 +218,7 @@ end
 module MollerPlesset
 import ..JuESHip: context, ensure_ao!, check, libjues, precision_code, FactoredERI, DFIntegrals
 using ChainRulesCore
-export do_rmp2, do_df_rmp2, do_ump2, rmp2_energy
+export do_rmp2, do_rmp2_noalloc, do_df_rmp2, do_ump2, rmp2_energy
 
 function do_rmp2(refWfn; precision=:fp64, kwargs...)
     ctx = context()
@@ -228,6 +228,26 @@ function do_rmp2(refWfn; precision=:fp64, kwargs...)
     GC.@preserve Co Cv eps check(ctx, ccall((:jues_rmp2, libjues), Cint,
         (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}),
         ctx.handle, precision_code(precision), Co, refWfn.nalpha, Cv, refWfn.nvira, n, eps, E))
+    return E[]
+end
+
+"""
+    do_rmp2_noalloc(refWfn)
+
+FP64 `do_rmp2` through `jues_rmp2_noalloc`, which allocates no device memory during the
+call (for callers under HIP graph capture or harnesses that forbid allocation): every
+intermediate lives in the library's static per-GPU scratch array (JUES_NOALLOC_SCRATCH_BYTES).
+Needs factored AO integrals; their upload, on first use, happens before the call.
+"""
+function do_rmp2_noalloc(refWfn)
+    refWfn.ao_eri isa Union{DFIntegrals,FactoredERI} || error("do_rmp2_noalloc needs DFIntegrals or FactoredERI")
+    ctx = context()
+    n = ensure_ao!(ctx, refWfn.ao_eri)
+    Co, Cv, eps = refWfn.Cao, refWfn.Cav, refWfn.epsa
+    E = Ref{Float64}(0.0)
+    GC.@preserve Co Cv eps check(ctx, ccall((:jues_rmp2_noalloc, libjues), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}, Cint, Ptr{Float64}, Cint, Cint, Ptr{Float64}, Ref{Float64}),
+        ctx.handle, precision_code(:fp64), Co, refWfn.nalpha, Cv, refWfn.nvira, n, eps, E))
     return E[]
 end
 
