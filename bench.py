@@ -112,6 +112,9 @@ def main():
     ap.add_argument("--pt-prec", default="fp64,fp32_mixed", help="--triples: forms to run")
     ap.add_argument("--n4", action="store_true",
                     help="instead of RCCD: UMP2, get_fock and Fock-driven CCSD iterations at o=60 v=340")
+    ap.add_argument("--cfg2", action="store_true",
+                    help="instead of RCCD: RCCSD at n=120 o=20 v=100 naux=360 (coupling 0.02) in the three forms, "
+                         "per-iteration time and |dE| against the oracle's fixed point after each stage")
     ap.add_argument("--rccsd5", action="store_true",
                     help="instead of RCCD: RCCSD at o=80 v=420 (n=500, naux=1500), recomputed ladder, factored "
                          "(ia|bc); s/iter of the three forms")
@@ -131,6 +134,9 @@ def main():
         return
     if args.n4:
         print(json.dumps(n4_bench()), flush=True)
+        return
+    if args.cfg2:
+        print(json.dumps(cfg2_bench()), flush=True)
         return
     if args.rccsd5:
         print(json.dumps(rccsd5_bench(args.coupling, args.steps, args.warmup, args.diis_max)), flush=True)
@@ -484,6 +490,44 @@ def n4_bench() -> dict:
             ms = [round(x["ms"], 1) for x in hist if x["stage"] == (0 if prec == "fp64" else 1)]
             out["stages"][f"{label}_{prec}"] = {"ms_per_iteration": ms}
             print(f"[bench] {label} {prec}: {ms} ms/iter", file=sys.stderr, flush=True)
+    return out
+
+
+def cfg2_bench() -> dict:
+    """RCCSD (mRCCSD equations) at n=120, o=20, v=100, naux=360 with the factors scaled by
+    sqrt(0.02), the integrals the committed fixture tests/golden/cfg2_coupling0.02.npz was
+    converged for by the CPU oracle: each form from t = 0 to the FP64 fixed point, with the
+    device time per iteration of each stage and |E - E_oracle| after each stage."""
+    import numpy as np
+    from jues import synthetic
+    from jues import CoupledCluster as CC
+    from jues.Wavefunction import Wfn, FactoredERI
+    gold = np.load(ROOT / "tests" / "golden" / "cfg2_coupling0.02.npz")
+    lam = float(gold["coupling"])
+    s = synthetic.workload("cfg2_rccsd_n120")
+    w = Wfn.from_orbitals(s.C, s.eps, s.o, FactoredERI(np.asfortranarray(s.B * np.sqrt(lam))))
+    E_ref = float(gold["E_mrccsd"])
+    out = {"workload": f"RCCSD n={s.n} o={s.o} v={s.v} naux={s.naux}, factors scaled by sqrt({lam})",
+           "E_oracle": E_ref, "forms": {}}
+    names = {0: "fp64", 1: "fp32", 2: "fp16x3"}
+    for prec in ("fp64", "fp32_mixed", "fp16x3"):
+        hist = []
+        CC.mRCCSD.do_rccsd(w, prec=prec, guess="zero", diis_max=8, cc_e_conv=1e-12, cc_max_rms=1e-11,
+                           fp32_r_conv=1e-6, maxit=300, callback=hist.append)
+        stages = {}
+        for h in hist:
+            st = stages.setdefault(names[h["stage"]], {"iterations": 0, "ms": []})
+            st["iterations"] += 1
+            st["ms"].append(h["ms"])
+            st["E_end"] = h["energy"]
+        rec = {}
+        for name, st in stages.items():
+            rec[name] = {"iterations": st["iterations"], "median_ms_per_iter": round(float(np.median(st["ms"])), 2),
+                         "abs_dE_vs_oracle_at_stage_end": abs(st["E_end"] - E_ref)}
+        out["forms"][prec] = rec
+        print(f"[bench] cfg2 {prec}: " + ", ".join(f"{k} {v['iterations']} it {v['median_ms_per_iter']} ms "
+                                                     f"|dE| {v['abs_dE_vs_oracle_at_stage_end']:.1e}"
+                                                     for k, v in rec.items()), file=sys.stderr, flush=True)
     return out
 
 

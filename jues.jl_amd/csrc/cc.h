@@ -40,6 +40,7 @@ struct CCWork {
   // RCCSD o v^3 terms through the factors: B1[m,P,e] / Y2[m,e,P] = sum_f t1[m,f] Lvv[e,f,P],
   // Y3[m,j,P] = sum_f Lov[m,f,P] t1[j,f], W1[i,e,P], W2[i,f,P], g[P]; Zs / Zsp: slabs of (mf|ae)
   DevBuf B1, Y2, Y3, W1, W2, gP, Zs, Zsp;
+  DevBuf UL;  // U Lov (ov x naux): the factored K U ring product
   // where the last residual evaluation left H, Y (ov layout) and, when the update kernel does
   // not symmetrise them itself, S = H + Y^x + (H + Y^x)^T (the buffers may be shared)
   T* out_H = nullptr;
@@ -81,6 +82,7 @@ struct CCState {
   std::vector<int> active;   // DIIS slots, oldest first
   DevBuf R1;       // FP64 T1 residual numerator (RCCSD)
   bool ovvv_factored = false;  // RCCSD: the (ia|bc) terms through the factors (no o v^3 block)
+  bool k_factored = false;     // K = (ia|jb) = Lov Lov^T with Lov kept: K U as two naux-wide products
   DevBuf Co, Cv;               // device orbitals (frozen core removed), to rebuild FP64 blocks
   bool i64_thin = false;       // FP64 blocks other than <ij|ab> released during a fast stage
   int last_stage = -1;         // precision of the last cc_run stage
@@ -118,6 +120,11 @@ T* bufp_t(DevBuf& b, size_t n) {
 // operand is symmetric; slotA/slotB name the operands for the split-FP16 stage.
 template <typename T>
 void ring_mm(jues_ctx* ctx, CCState& S, T alpha, const T* A, int slotA, const T* B, int slotB, T beta, T* C);
+// C += alpha K B with K = (ia|jb) at [i,a,j,b] and B a symmetric (ov) x (ov) matrix: the full
+// ring product, or with factored integrals (K = Lov Lov^T) C += alpha Lov (B Lov)^T, two
+// products of 2 (ov)^2 naux flops instead of one of 2 (ov)^3
+template <typename T>
+void ring_k_mm(jues_ctx* ctx, CCState& S, T alpha, const T* B, int slotB, T* C);
 // Hole-hole ladder sum_kl W'[kl,ij] t[kl,ab], W' = Wp + <kl|cd> t[ij,cd], on packed pairs,
 // added into the pp-ladder's Lp / Lm (run after ladder_mm, whose Tp / Tm it reads).
 // Wp: [m,n,i,j], oooo plus any T1 terms.

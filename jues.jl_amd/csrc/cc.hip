@@ -148,6 +148,7 @@ void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv
                  (long)v * v, 0, 0.0, Jc, oo, 0, 1, "setup_ints");
     gemm<double>(ctx, false, true, oo, oo, naux, 1.0, Loo.as<double>(), oo, 0, Loo.as<double>(), oo, 0, 0.0, Oc, oo,
                  0, 1, "setup_ints");
+    S.k_factored = true;
     if (S.method == 1) {
       // (ij|ka) -> ooov chem [i,j,k,a]; (ia|bc) stays factored: Lov, its two permutations and Lvv
       double* ooov = bufp<double>(I.ooov_c, oo * ov);
@@ -159,9 +160,10 @@ void cc_build_ints(jues_ctx* ctx, CCState& S, const double* Co, const double* Cv
       permute<double, double>(ctx, Lov.as<double>(), {o, v, naux}, {0, 2, 1}, bufp<double>(I.Lovp, ov * naux), 1.0,
                               0.0, "setup_ints");
       sync(ctx);
-      I.Lov = std::move(Lov);
       if (!S.ladder_recompute) I.Lvv = std::move(Lvv);
     }
+    sync(ctx);
+    I.Lov = std::move(Lov);
   }
   if (ctx->gao.p && S.method == 1) {
     tei_transform_dev<double>(ctx, Co, o, Co, o, Co, o, Cv, v, bufp<double>(I.ooov_c, oo * ov));
@@ -304,6 +306,22 @@ void ring_mm<float>(jues_ctx* ctx, CCState& S, float alpha, const float* A, int 
   const Split16& b = split_operand<float>(ctx, S, slotB, B, ov, ov);
   gemm16_tn(ctx, ov, ov, alpha, a, b, beta, C, ov, "cc_ring");
 }
+
+template <typename T>
+void ring_k_mm(jues_ctx* ctx, CCState& S, T alpha, const T* B, int slotB, T* C) {
+  if (!S.k_factored) {
+    ring_mm<T>(ctx, S, alpha, ints<T>(S).Kov.template as<T>(), SP_K, B, slotB, T(1), C);
+    return;
+  }
+  const long ov = (long)S.o * S.v, naux = ctx->naux;
+  const T* L = ints<T>(S).Lov.template as<T>();
+  T* UL = bufp<T>(work<T>(S).UL, (size_t)(ov * naux));
+  gemm<T>(ctx, false, false, ov, naux, ov, T(1), B, ov, 0, L, ov, 0, T(0), UL, ov, 0, 1, "cc_ring_lr");
+  gemm<T>(ctx, false, true, ov, ov, naux, alpha, L, ov, 0, UL, ov, 0, T(1), C, ov, 0, 1, "cc_ring_lr");
+}
+
+template void ring_k_mm<double>(jues_ctx*, CCState&, double, const double*, int, double*);
+template void ring_k_mm<float>(jues_ctx*, CCState&, float, const float*, int, float*);
 
 // The ladder with <ab|ef> rebuilt from the factors every iteration (DF-RCCD.jl:198-215
 // contracts bvv[a,e,Q] bvv[b,f,Q] the same way): for a slab of f, X[e,a,b,f] =
@@ -520,7 +538,7 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   const T* K = I.Kov.template as<T>();
   const T* Kx = I.Ktov.template as<T>();
   JUES_HIP(hipMemcpyAsync(WJ, K, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  ring_mm<T>(ctx, S, half, K, SP_K, Uov, SP_UOV, one, WJ);
+  ring_k_mm<T>(ctx, S, half, Uov, SP_UOV, WJ);
   ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, one, WJ);
   permute<T, T>(ctx, I.Jov.template as<T>(), {ov, ov}, {0, 1}, WE, -1.0, 0.0, "cc_prep");
   ring_mm<T>(ctx, S, half, Kx, SP_KX, Ttov, SP_TTOV, one, WE);

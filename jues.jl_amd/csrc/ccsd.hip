@@ -291,7 +291,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   permute<T, T>(ctx, Uov, {ov, ov}, {0, 1}, U2, 1.0, -2.0, "cc_prep");
   JUES_HIP(hipMemcpyAsync(WJ, Kov, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
   // every left operand of the ring products is a symmetric (ov) x (ov) matrix
-  ring_mm<T>(ctx, S, half, Kov, SP_K, U2, SP_U2, one, WJ);
+  ring_k_mm<T>(ctx, S, half, U2, SP_U2, WJ);
   ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, one, WJ);
   // WJ[me,jb] += sum_f t1[jf] (me|bf)
   if (fac) gemm<T>(ctx, false, true, ov, ov, naux, one, Lov, ov, 0, Y2, ov, 0, one, WJ, ov, 0, 1, "cc_ovvv_fac");

@@ -78,9 +78,11 @@ def test_cfg3_taylor_remainder(cfg3, grad64):
     r = [abs(MollerPlesset.do_rmp2(_wfn(cfg3, C=cfg3.C + h * X)) - E0 - h * slope) for h in hs]
     print("taylor remainders", dict(zip(hs, r)))
     for a, b in zip(r, r[1:]):
-        assert 70 < a / b < 140, r
-    # with a gradient off by 1e-6 relative the remainder at h = 1e-5 would stall at h * 1e-6 |slope|
-    assert r[-1] < 1e-3 * 1e-5 * abs(slope)
+        assert 95 < a / b < 105, r
+    # an error d in the slope adds d |slope| h to the remainder, so at the last decade (where
+    # the h^2 term is smallest) a ratio within 1% of 100 bounds d by about 1e-5
+    assert 99 < r[-2] / r[-1] < 101, r
+    assert abs(r[-2] / r[-1] - 100) * r[-1] / 100 < 1e-5 * abs(slope) * 1e-5
 
 
 def test_cfg3_eps_gradient_and_invariants(cfg3, grad64):

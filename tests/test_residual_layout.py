@@ -300,6 +300,27 @@ def test_factored_ovvv_terms_equal_stored_block(mrccsd):
     assert np.max(np.abs(got[0] - other)) > 1e-4 * np.max(np.abs(other))
 
 
+def test_factored_k_ring_product():
+    """With K[ia,jb] = (ia|jb) = sum_P Lov[ia,P] Lov[jb,P], the ring product K U equals
+    Lov (U Lov)^T for the symmetric U = 2T - Tx of pair-symmetric amplitudes (the device's
+    two naux-wide products); it needs U symmetric, which a non-symmetric U breaks."""
+    s = synthetic.make_system(10, 3, 7, 15, 41, name="klr")
+    rng = np.random.default_rng(2)
+    B = np.asfortranarray(s.B + 0.05 * rng.normal(size=s.B.shape))
+    o, v = s.o, s.v
+    Lov = O.mo_ov_factors(B, s.Co, s.Cv).reshape(o * v, -1, order="F")
+    K = O.mo_from_factors(B, s.Co, s.Cv, s.Co, s.Cv).reshape(o * v, o * v, order="F")
+    t = _amplitudes(o, v, 9)
+    T = t.transpose(0, 2, 1, 3).reshape(o * v, o * v, order="F")
+    Tx = np.einsum("ijba->iajb", t).reshape(o * v, o * v, order="F")
+    U = 2 * T - Tx
+    assert np.max(np.abs(U - U.T)) < 1e-14
+    lr = Lov @ (U @ Lov).T
+    assert np.max(np.abs(lr - K @ U)) < 1e-12 * np.max(np.abs(K @ U))
+    Un = U + 0.1 * rng.normal(size=U.shape)
+    assert np.max(np.abs(Lov @ (Un @ Lov).T - K @ Un)) > 1e-4
+
+
 # ---------------------------------------------------------------- recomputed ladder
 
 @pytest.mark.parametrize("nf", [1, 2, 4, 6])
