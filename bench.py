@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--keep-diis", action="store_true",
                     help="--converge: keep the fast stage's DIIS history for the FP64 refinement")
     ap.add_argument("--forms", default="fp64,fp32_mixed,fp16x3", help="--converge: forms to run")
+    ap.add_argument("--refine", default="restart,defect",
+                    help="--converge: FP64 hand-overs to time after each fast stage (restart: FP64 iterations "
+                         "with a fresh DIIS history; defect: defect correction)")
     ap.add_argument("--diis-max", type=int, default=8)
     ap.add_argument("--ladder", default="auto", choices=["auto", "stored", "recompute"],
                     help="pp-ladder integrals stored (packed V+/V-) or rebuilt from the factors every iteration")
@@ -221,7 +224,8 @@ def main():
                                "cores": cb["threads"], "kind": "port", "sample": cb["sample"],
                                "host_cpus": os.cpu_count(), **cpu_baseline.host_info()}
     if args.converge:
-        out["convergence"] = converge(w, args.coupling, args.fast_r_conv, args.keep_diis, args.forms.split(","))
+        out["convergence"] = converge(w, args.coupling, args.fast_r_conv, args.keep_diis, args.forms.split(","),
+                                      args.refine.split(","))
     print(json.dumps(out), flush=True)
 
 
@@ -572,9 +576,10 @@ def rccsd5_bench(coupling: float, steps: int, warmup: int, diis_max: int) -> dic
 
 
 def converge(w, coupling: float, fast_r_conv: float = 1e-6, keep_diis: bool = False,
-             forms=("fp64", "fp32_mixed", "fp16x3")) -> dict:
+             forms=("fp64", "fp32_mixed", "fp16x3"), refines=("restart", "defect")) -> dict:
     """Each form to convergence: plain FP64, and each fast form (FP32-mixed, split-FP16) followed
-    by FP64 refinement (DIIS history reset unless keep_diis)."""
+    by each FP64 hand-over: FP64 iterations (DIIS history reset unless keep_diis), or defect
+    correction (FP64 iterations anchoring inner solves in the fast form)."""
     from jues.CoupledCluster import Solver
     res = {"coupling": coupling, "fast_r_conv": fast_r_conv, "keep_diis": keep_diis}
     kw = dict(guess="zero", diis=True, diis_max=8, coupling=coupling, cc_e_conv=1e-10, cc_max_rms=1e-9,
@@ -592,27 +597,36 @@ def converge(w, coupling: float, fast_r_conv: float = 1e-6, keep_diis: bool = Fa
     for form in ("fp32_mixed", "fp16x3"):
         if form not in forms:
             continue
-        sm = Solver(w, method="rccd", prec=form, **kw)
-        t0 = time.perf_counter()
-        i1, _ = sm.iterate(200, prec=form, stop_on_conv=True)
-        t1 = time.perf_counter() - t0
-        E1 = sm.result()[0]
-        if not keep_diis:
-            sm.reset_diis()
-        t0 = time.perf_counter()
-        i2, c2 = sm.iterate(100, prec="fp64", stop_on_conv=True)
-        t2 = time.perf_counter() - t0
-        E2 = sm.result()[0]
-        print(f"[bench] {form}: fast {i1['iter']} it {t1:.1f} s E={E1:.12f}; refine {i2['iter'] - i1['iter']} it "
-              f"{t2:.1f} s E={E2:.12f} conv={c2}", file=sys.stderr, flush=True)
-        rec = dict(fast_iters=i1["iter"], fast_s=round(t1, 2), E_after_fast=E1, refine_iters=i2["iter"] - i1["iter"],
-                   refine_s=round(t2, 2), E=E2, converged=c2, time_to_solution_s=round(t1 + t2, 2),
-                   final_rnorm=i2["rnorm"], final_dE=i2["delta_e"])
-        if "fp64" in res:
-            rec["abs_dE_vs_fp64"] = abs(E2 - res["fp64"]["E"])
-            rec["abs_dE_fast_stage_vs_fp64"] = abs(E1 - res["fp64"]["E"])
-        res[form] = rec
-        del sm
+        for mode in refines:
+            sm = Solver(w, method="rccd", prec=form, **kw)
+            t0 = time.perf_counter()
+            i1, _ = sm.iterate(200, prec=form, stop_on_conv=True)
+            t1 = time.perf_counter() - t0
+            E1 = sm.result()[0]
+            t0 = time.perf_counter()
+            if mode == "defect":
+                # stage 0 records are the FP64 iterations; the rest are inner correction steps
+                i2, c2 = sm.refine(form, max_anchors=20, inner_maxit=100)
+            else:
+                if not keep_diis:
+                    sm.reset_diis()
+                i2, c2 = sm.iterate(100, prec="fp64", stop_on_conv=True)
+            t2 = time.perf_counter() - t0
+            E2 = sm.result()[0]
+            n64 = i2["n_fp64"] if mode == "defect" else i2["iter"] - i1["iter"]
+            print(f"[bench] {form}+{mode}: fast {i1['iter']} it {t1:.1f} s E={E1:.12f}; refine "
+                  f"{i2['iter'] - i1['iter']} it ({n64} FP64) {t2:.1f} s E={E2:.12f} conv={c2}",
+                  file=sys.stderr, flush=True)
+            rec = dict(fast_iters=i1["iter"], fast_s=round(t1, 2), E_after_fast=E1,
+                       refine_iters=i2["iter"] - i1["iter"], refine_fp64_iters=n64, refine_s=round(t2, 2), E=E2,
+                       converged=c2, time_to_solution_s=round(t1 + t2, 2), final_rnorm=i2["rnorm"],
+                       final_dE=i2["delta_e"])
+            if "fp64" in res:
+                rec["abs_dE_vs_fp64"] = abs(E2 - res["fp64"]["E"])
+                rec["abs_dE_fast_stage_vs_fp64"] = abs(E1 - res["fp64"]["E"])
+                rec["speedup_time_to_solution_vs_fp64"] = round(res["fp64"]["time_to_solution_s"] / (t1 + t2), 2)
+            res[f"{form}+{mode}"] = rec
+            del sm
     return res
 
 

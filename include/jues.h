@@ -72,6 +72,18 @@ typedef enum {
                                    DF-RCCD.jl:198-215 contracts bvv . bvv; never stored */
 } jues_ladder;
 
+/* How a fast stage (FP32_MIXED / FP16X3_MIXED) hands over to FP64. */
+typedef enum {
+    JUES_REFINE_RESTART = 0,  /* FP64 iterations from the fast stage's amplitudes with a fresh DIIS
+                                 history, until e_conv / r_conv (at most refine_maxit) */
+    JUES_REFINE_DEFECT = 1    /* defect correction (RCCD; RCCSD falls back to RESTART): an FP64
+                                 iteration at t0 keeps its numerator R0 = <ij|ab> + S + L, and the fast
+                                 stage then solves D t = lambda (R0 + [R(t) - R(t0)]) with the bracket
+                                 formed from d = t - t0, so that its rounding scales with |d|, not
+                                 |t|; repeated until an FP64 iteration meets e_conv / r_conv, at most
+                                 refine_maxit FP64 iterations */
+} jues_refine;
+
 /* Options; jues_cc_default_opts() fills the defaults noted here.
  * Names mirror CoupledCluster.jl:36-43 (cc_max_iter, cc_e_conv, cc_max_rms, diis). */
 typedef struct {
@@ -97,6 +109,7 @@ typedef struct {
     double coupling;          /* c, default 1.0 */
     int32_t continuation;     /* N, default 1 (no continuation) */
     int32_t ladder;           /* jues_ladder, default JUES_LADDER_AUTO */
+    int32_t refine;           /* jues_refine, default JUES_REFINE_RESTART */
 } jues_cc_opts;
 
 /* Per-iteration record passed to the optional callback. */
@@ -272,6 +285,15 @@ jues_status jues_cc_iterate(jues_ctx* ctx, int stage_prec, int nsteps, int stop_
                             jues_iter_info* info, int* converged);
 /* Reset the DIIS history (e.g. at the FP32 -> FP64 hand-over). */
 jues_status jues_cc_reset_diis(jues_ctx* ctx);
+/* Defect-correction refinement (JUES_REFINE_DEFECT) of the resident RCCD state: at most
+ * max_anchors FP64 iterations, each (unless it meets e_conv / r_conv, or is the last)
+ * followed by at most inner_maxit iterations of the correction equation in inner_prec
+ * (JUES_FP32_MIXED, JUES_FP16X3_MIXED, or JUES_FP64, where the bracket is exact), which stop
+ * once ||dt||_2 < r_conv / 10 or stall.  info receives the last FP64 iteration's record;
+ * converged is set when it met the convergence test; n_fp64 (may be null) receives the
+ * number of FP64 iterations run. */
+jues_status jues_cc_refine(jues_ctx* ctx, int inner_prec, int max_anchors, int inner_maxit,
+                           jues_iter_info* info, int* converged, int* n_fp64);
 jues_status jues_cc_get(jues_ctx* ctx, double* E, double* T1_out, double* T2_out);
 
 /* Accumulated device time per kernel class since the last reset (HIP events on

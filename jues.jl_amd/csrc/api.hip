@@ -338,6 +338,7 @@ void jues_cc_default_opts(jues_cc_opts* o) {
   o->coupling = 1.0;
   o->continuation = 1;
   o->ladder = JUES_LADDER_AUTO;
+  o->refine = JUES_REFINE_RESTART;
 }
 
 }  // extern "C"

@@ -51,9 +51,13 @@ struct CCWork {
 // Split-FP16 operand planes of the fast stage.  Slots below SPLIT_CONST hold
 // per-iteration operands (re-split on their first use in an iteration); the
 // others hold integral matrices, split once.
+// The _T slots hold the full amplitudes' layouts beside those of the correction d in a
+// defect-correction iteration; SP_WE0 / SP_WJ0 hold the anchor's intermediates.
 enum SplitSlot {
   SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT, SP_TQ, SP_TM, SP_WPT, SP_WMT,
-  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPT, SP_KMT, SP_LVVT, SP_LVVTT, SPLIT_SLOTS
+  SP_TOV_T, SP_UOV_T, SP_TTOV_T,
+  SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPT, SP_KMT, SP_LVVT, SP_LVVTT, SP_WE0, SP_WJ0,
+  SPLIT_SLOTS
 };
 struct SplitState {
   struct Entry {
@@ -63,6 +67,24 @@ struct SplitState {
   };
   Entry e[SPLIT_SLOTS];
   Split16Buf vps, vms, tpc, tmc;  // per-slab planes of the recomputed ladder (never cached)
+};
+
+// Defect-correction refinement of RCCD.  An FP64 iteration at the anchor t0 keeps its numerator
+// R0 = <ij|ab> + S(t0) + L(t0) and its intermediates WJ0, WE0 and the packed hole-hole W'0; the
+// iterations in between solve D t = lambda (R0 + [R(t) - R(t0)]) with the bracket formed from
+// d = t - t0 (every product has d, or a difference linear in d, as one factor):
+//   dWJ = K U(d)/2 - Kx T(d)/2,  dWE = Kx Tx(d)/2 + D(Fae(d), Fmi(d))/2
+//   dH = T(d) WE0 + T(t) dWE + U(d) WJ0 + U(t) dWJ,   dY = Tx(d) WE0 + Tx(t) dWE
+//   dL = pp-ladder(d) + W'0 d + dW' t,  dW'[kl,ij] = sum_cd <kl|cd> d[ij,cd]
+// so that rounding in the inner stage's type scales with |d| rather than |t|.
+struct DCState {
+  DevBuf t0, R0, d;             // FP64: anchor amplitudes, its numerator, t - t0
+  DevBuf WJ0, WE0, Wpt0, Wmt0;  // anchor intermediates in the inner stage's type
+  DevBuf Tt, Ttt, Ut;           // ov layouts of the full amplitudes t
+  DevBuf Tpt, Tmt, Tqt;         // packed pairs of t (Tqt: Tpt with the (a,a) columns doubled)
+  int inner_prec = -1;          // the type the anchor was captured in
+  bool capture = false;         // the next FP64 iteration records an anchor
+  bool on = false;              // iterations solve the correction equation
 };
 
 struct CCState {
@@ -91,6 +113,7 @@ struct CCState {
   std::unique_ptr<CCWork<double>> w64;
   std::unique_ptr<CCWork<float>> w32;
   std::unique_ptr<SplitState> sp;
+  std::unique_ptr<DCState> dc;
   bool split_on = false;  // the FP32 stage runs its ring / pp-ladder products on split-FP16 MFMA
   bool ladder_recompute = false;  // <ab|ef> rebuilt from factors every iteration instead of stored
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -139,6 +162,19 @@ template <typename T>
 void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2);
 template <typename T>
 void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const double* t1);
+template <typename T>
+void rccd_delta_contractions(jues_ctx* ctx, CCState& S, const double* t2);
+// the anchor's intermediates, in the inner stage's type (called by the FP64 contractions)
+void dc_capture(jues_ctx* ctx, CCState& S, const double* WJ, const double* WE, const double* Wpt,
+                const double* Wmt);
+template <typename T>
+void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info);
+void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_conv, jues_iter_info* last,
+            bool* converged, jues_iter_cb cb, void* user, double fast_r_conv = -1.0);
+void cc_reset_diis(CCState& S);
+// defect-correction refinement (RCCD): at most max_anchors FP64 iterations with inner solves between
+int cc_refine_dc(jues_ctx* ctx, CCState& S, int inner_prec, int max_anchors, int inner_maxit,
+                 jues_iter_info* last, bool* converged, jues_iter_cb cb, void* user);
 void cc_build_ccsd_ints(jues_ctx* ctx, CCState& S);
 double cc_energy(jues_ctx* ctx, CCState& S);
 double cc_energy_t1(jues_ctx* ctx, CCState& S);

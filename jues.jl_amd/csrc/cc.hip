@@ -550,6 +550,9 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
   ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ, SP_WJ, one, H);
   // exchange ring term t[m,i,b,e] WmBEj[m,a,e,j] = (Tx WE)[ib,ja]
   ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE, SP_WE, zero, Y);
+  // an FP64 iteration that anchors a defect correction keeps WJ, WE and W' (before WJ is reused)
+  if constexpr (sizeof(T) == 8)
+    if (S.dc && S.dc->capture) dc_capture(ctx, S, WJ, WE, W.Wpt.template as<T>(), W.Wmt.template as<T>());
   // S = H + Y^x + (H + Y^x)^T: summed by the update kernel, or here (into WJ) when its LDS slab
   // would not fit
   if (!fused_sym(o, sizeof(T))) {
@@ -564,6 +567,145 @@ void rccd_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
 
 template void rccd_contractions<double>(jues_ctx*, CCState&, const double*);
 template void rccd_contractions<float>(jues_ctx*, CCState&, const double*);
+
+// ---------------------------------------------------------------------------
+// defect correction (RCCD): the residual's change from the anchor t0, formed from d = t - t0
+// ---------------------------------------------------------------------------
+void dc_capture(jues_ctx* ctx, CCState& S, const double* WJ, const double* WE, const double* Wpt,
+                const double* Wmt) {
+  DCState& D = *S.dc;
+  const long ov = (long)S.o * S.v, N2 = ov * ov;
+  const PairTables& P = S.P;
+  auto put = [&](DevBuf& dst, const double* src, long n) {
+    if (D.inner_prec == JUES_FP64) {
+      double* p = bufp<double>(dst, (size_t)n);
+      JUES_HIP(hipMemcpyAsync(p, src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      convert<double, float>(ctx, src, bufp<float>(dst, (size_t)n), n);
+    }
+  };
+  put(D.WJ0, WJ, N2);
+  put(D.WE0, WE, N2);
+  put(D.Wpt0, Wpt, P.npo * P.npo);
+  if (P.nmo > 0 && P.nmv > 0) put(D.Wmt0, Wmt, P.nmo * P.nmo);
+}
+
+namespace {
+// Hole-hole part of the bracket: t W'(t) - t0 W'0 = W'0 d + dW' t with dW' = <kl|cd> d[ij,cd], on
+// the packed pairs of hh_mm (W'0 is the anchor's W+^T / W-^T).  Tpd / Tmd: packed d (from ladder_mm).
+template <typename T>
+void hh_delta_mm(jues_ctx* ctx, CCState& S, const T* Tpd, const T* Tmd, const double* t2, T* Lp, T* Lm) {
+  const PairTables& P = S.P;
+  const bool minus = P.nmo > 0 && P.nmv > 0;
+  CCWork<T>& W = work<T>(S);
+  DCState& D = *S.dc;
+  T* dWpt = bufp<T>(W.Wpt, P.npo * P.npo);
+  T* dWmt = bufp<T>(W.Wmt, std::max(1L, P.nmo * P.nmo));
+  T* Tqd = bufp<T>(W.Tq, P.npo * P.npv);
+  T* Tpt = bufp<T>(D.Tpt, P.npo * P.npv);
+  T* Tmt = bufp<T>(D.Tmt, std::max(1L, P.nmo * P.nmv));
+  T* Tqt = bufp<T>(D.Tqt, P.npo * P.npv);
+  pack_t<T>(ctx, P, t2, S.o, S.v, Tpt, Tmt);
+  pair_diag2<T>(ctx, P, Tpd, Tqd);
+  pair_diag2<T>(ctx, P, Tpt, Tqt);
+  if constexpr (sizeof(T) == 4) {
+    CCInts<float>& b = *S.i32;
+    if (!b.KP.p) {
+      convert_buf<float>(ctx, S.i64->KP, b.KP);
+      convert_buf<float>(ctx, S.i64->KM, b.KM);
+    }
+  }
+  // (plain products in the stage's type also in the split stage: a few percent of the iteration)
+  CCInts<T>& I = ints<T>(S);
+  gemm<T>(ctx, false, true, P.npo, P.npo, P.npv, T(1), Tpd, P.npo, 0, I.KP.template as<T>(), P.npo, 0, T(0), dWpt,
+          P.npo, 0, 1, "cc_hh");
+  gemm<T>(ctx, false, false, P.npo, P.npv, P.npo, T(0.5), D.Wpt0.as<T>(), P.npo, 0, Tqd, P.npo, 0, T(1), Lp, P.npo,
+          0, 1, "cc_hh");
+  gemm<T>(ctx, false, false, P.npo, P.npv, P.npo, T(0.5), dWpt, P.npo, 0, Tqt, P.npo, 0, T(1), Lp, P.npo, 0, 1,
+          "cc_hh");
+  if (minus) {
+    gemm<T>(ctx, false, true, P.nmo, P.nmo, P.nmv, T(1), Tmd, P.nmo, 0, I.KM.template as<T>(), P.nmo, 0, T(0), dWmt,
+            P.nmo, 0, 1, "cc_hh");
+    gemm<T>(ctx, false, false, P.nmo, P.nmv, P.nmo, T(0.5), D.Wmt0.as<T>(), P.nmo, 0, Tmd, P.nmo, 0, T(1), Lm,
+            P.nmo, 0, 1, "cc_hh");
+    gemm<T>(ctx, false, false, P.nmo, P.nmv, P.nmo, T(0.5), dWmt, P.nmo, 0, Tmt, P.nmo, 0, T(1), Lm, P.nmo, 0, 1,
+            "cc_hh");
+  }
+}
+}  // namespace
+
+// The products of rccd_contractions with d in place of t and the anchor's intermediates (the
+// split into products is the one documented with DCState); out_H / out_Y / out_S and Lp / Lm
+// then hold the bracket's parts, which the update kernel adds to R0.
+template <typename T>
+void rccd_delta_contractions(jues_ctx* ctx, CCState& S, const double* t2) {
+  const int o = S.o, v = S.v;
+  const long ov = (long)o * v, N2 = ov * ov, oo = (long)o * o, vv = (long)v * v;
+  CCInts<T>& I = ints<T>(S);
+  CCWork<T>& W = work<T>(S);
+  DCState& D = *S.dc;
+  const std::vector<long> d4 = {o, o, v, v};
+  double* d = bufp<double>(D.d, N2);
+  JUES_HIP(hipMemcpyAsync(d, t2, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  axpby<double>(ctx, N2, -1.0, D.t0.as<double>(), 1.0, d);
+  T* Tpp = bufp<T>(W.Tpp, N2);
+  T* Upp = bufp<T>(W.Upp, N2);
+  T* Tov = bufp<T>(W.Tov, N2);
+  T* Ttov = bufp<T>(W.Ttov, N2);
+  T* Uov = bufp<T>(W.Uov, N2);
+  T* Ua = bufp<T>(W.Ua, N2);
+  T* WJ = bufp<T>(W.WJ, N2);
+  T* WE = bufp<T>(W.WE, N2);
+  T* Y = bufp<T>(W.Y, N2);
+  T* H = bufp<T>(W.H, N2);
+  T* Fae = bufp<T>(W.Fae, vv);
+  T* Fmi = bufp<T>(W.Fmi, oo);
+  T* Tp = bufp<T>(W.Tp, S.P.npo * S.P.npv);
+  T* Tm = bufp<T>(W.Tm, std::max(1L, S.P.nmo * S.P.nmv));
+  T* Lp = bufp<T>(W.Lp, S.P.npo * S.P.npv);
+  T* Lm = bufp<T>(W.Lm, std::max(1L, S.P.nmo * S.P.nmv));
+  T* Tt = bufp<T>(D.Tt, N2);
+  T* Ttt = bufp<T>(D.Ttt, N2);
+  T* Ut = bufp<T>(D.Ut, N2);
+  const T* WJ0 = D.WJ0.as<T>();
+  const T* WE0 = D.WE0.as<T>();
+  const T one = T(1), zero = T(0), half = T(0.5);
+
+  // layouts of d (as rccd_contractions makes those of t) and the ring layouts of t
+  rccd_layouts<T>(ctx, d, o, v, Tpp, Upp, Tov, Ttov, Uov);
+  rccd_layouts<T>(ctx, t2, o, v, nullptr, nullptr, Tt, Ttt, Ut);
+  permute<T, T>(ctx, Upp, d4, {2, 0, 1, 3}, Ua, 1.0, 0.0, "cc_prep");
+  // Fae, Fmi are linear in t: their values at d
+  gemm<T>(ctx, false, true, v, v, oo * v, T(-1), Ua, v, 0, I.Ka.template as<T>(), v, 0, zero, Fae, v, 0, 1, "cc_fock");
+  gemm<T>(ctx, false, true, o, o, ov * v, one, Upp, o, 0, I.Kpp.template as<T>(), o, 0, zero, Fmi, o, 0, 1, "cc_fock");
+  // ladders: pp-ladder of d (linear), hole-hole W'0 d + dW' t
+  ladder_mm<T>(ctx, S, d, Tp, Tm, Lp, Lm);
+  hh_delta_mm<T>(ctx, S, Tp, Tm, t2, Lp, Lm);
+  // dWJ = K U(d)/2 - Kx T(d)/2,  dWE = Kx Tx(d)/2 + D(Fae(d), Fmi(d))/2
+  const T* Kx = I.Ktov.template as<T>();
+  ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, zero, WJ);
+  ring_k_mm<T>(ctx, S, half, Uov, SP_UOV, WJ);
+  ring_mm<T>(ctx, S, half, Kx, SP_KX, Ttov, SP_TTOV, zero, WE);
+  fock_dress<T>(ctx, WE, Fae, Fmi, o, v, half);
+  // dH = T(d) WE0 + U(d) WJ0 + T(t) dWE + U(t) dWJ,  dY = Tx(d) WE0 + Tx(t) dWE
+  ring_mm<T>(ctx, S, one, Tov, SP_TOV, WE0, SP_WE0, zero, H);
+  ring_mm<T>(ctx, S, one, Uov, SP_UOV, WJ0, SP_WJ0, one, H);
+  ring_mm<T>(ctx, S, one, Tt, SP_TOV_T, WE, SP_WE, one, H);
+  ring_mm<T>(ctx, S, one, Ut, SP_UOV_T, WJ, SP_WJ, one, H);
+  ring_mm<T>(ctx, S, one, Ttov, SP_TTOV, WE0, SP_WE0, zero, Y);
+  ring_mm<T>(ctx, S, one, Ttt, SP_TTOV_T, WE, SP_WE, one, Y);
+  if (!fused_sym(o, sizeof(T))) {
+    permute<T, T>(ctx, Y, {o, v, o, v}, {0, 3, 2, 1}, H, 1.0, 1.0, "cc_sym");
+    JUES_HIP(hipMemcpyAsync(WJ, H, N2 * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    permute<T, T>(ctx, H, {ov, ov}, {1, 0}, WJ, 1.0, 1.0, "cc_sym");
+  }
+  W.out_H = H;
+  W.out_Y = Y;
+  W.out_S = WJ;
+}
+
+template void rccd_delta_contractions<double>(jues_ctx*, CCState&, const double*);
+template void rccd_delta_contractions<float>(jues_ctx*, CCState&, const double*);
 
 // ---------------------------------------------------------------------------
 // iteration
@@ -604,8 +746,14 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
   double* part = static_cast<double*>(scratch(ctx, 13, sizeof(double) * (size_t)(PB * (2 + MAX_DIIS))));
 
   const T* R1extra = nullptr;
+  DCState* dc = S.dc.get();
+  const bool dc_on = dc && dc->on && S.method == 0;
+  const bool dc_cap = dc && dc->capture && S.method == 0 && sizeof(T) == 8;
+  if (dc_cap)
+    JUES_HIP(hipMemcpyAsync(bufp<double>(dc->t0, N2), t, N2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   if (S.method == 0) {
-    rccd_contractions<T>(ctx, S, t);
+    if (dc_on) rccd_delta_contractions<T>(ctx, S, t);
+    else rccd_contractions<T>(ctx, S, t);
   } else {
     rccsd_contractions<T>(ctx, S, t, t + N2);
     R1extra = nullptr;
@@ -615,7 +763,9 @@ void cc_iteration(jues_ctx* ctx, CCState& S, jues_iter_info* info) {
   A.o = o;
   A.v = v;
   A.eps = S.eps.as<double>();
-  A.Kpp = S.i64->Kpp.as<double>();
+  // a correction iteration adds its bracket to the anchor's numerator R0 in place of <ij|ab>
+  A.Kpp = dc_on ? dc->R0.as<double>() : S.i64->Kpp.as<double>();
+  A.R_out = dc_cap ? bufp<double>(dc->R0, N2) : nullptr;
   A.Lener = S.Lener.as<double>();
   if (fused_sym(o, sizeof(T))) {
     A.H = W.out_H;
@@ -721,6 +871,7 @@ void cc_setup(jues_ctx* ctx, int method, int prec, const jues_cc_opts* opts_in, 
   JUES_REQUIRE(opts.guess >= 0 && opts.guess <= 3, "cc_setup: bad guess");
   JUES_REQUIRE(opts.guess != JUES_GUESS_USER || T2_init, "cc_setup: guess=USER needs T2");
   JUES_REQUIRE(opts.continuation >= 1 && std::isfinite(opts.coupling), "cc_setup: bad continuation/coupling");
+  JUES_REQUIRE(opts.refine == JUES_REFINE_RESTART || opts.refine == JUES_REFINE_DEFECT, "cc_setup: bad refine");
   JUES_REQUIRE(!fock || (method == 1 && opts.coupling == 1.0 && opts.continuation == 1),
                "cc_setup: a Fock matrix needs CCSD at coupling 1 without continuation");
   sync(ctx);
@@ -846,8 +997,12 @@ static void ensure_fp64_ints(jues_ctx* ctx, CCState& S) {
 }
 
 void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_conv, jues_iter_info* last,
-            bool* converged, jues_iter_cb cb, void* user) {
+            bool* converged, jues_iter_cb cb, void* user, double fast_r_conv) {
   *converged = false;
+  // the fast stage's test (step below a target, or stalled); with fast_r_conv >= 0 (the inner
+  // solves of a defect correction) in every precision, with that target
+  const bool fast_test = stage_prec != JUES_FP64 || fast_r_conv >= 0.0;
+  const double fast_target = fast_r_conv >= 0.0 ? fast_r_conv : S.opts.fp32_r_conv;
   // Only one stage's buffers are held at a time (at o=80, v=420 the FP64 and FP32 work sets
   // together do not fit in one GPU); a later switch rebuilds what it needs.
   if (stage_prec == JUES_FP64) {
@@ -882,16 +1037,16 @@ void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_
       std::fprintf(stderr, "  cc iter %3d %s E=% .15f dE=% .3e |dt|=%.3e diis=%d %.1f ms\n", info.iter,
                    stage_name[info.stage], info.energy, info.delta_e, info.rnorm, info.diis_size, info.ms);
     *last = info;
-    if (stage_prec == JUES_FP64) {
+    if (!fast_test) {
       if (std::fabs(info.delta_e) < S.opts.e_conv && info.rnorm < S.opts.r_conv) {
         *converged = true;
         if (stop_on_conv) break;
       }
     } else {
-      // FP32 stage: done when the step is below fp32_r_conv or stops shrinking
+      // fast stage: done when the step is below the target or stops shrinking
       if (info.rnorm < best * 0.9) { best = info.rnorm; stall = 0; }
       else ++stall;
-      if (info.rnorm < S.opts.fp32_r_conv || stall >= 4) {
+      if (info.rnorm < fast_target || stall >= 4) {
         *converged = true;
         if (stop_on_conv) break;
       }
@@ -900,6 +1055,52 @@ void cc_run(jues_ctx* ctx, CCState& S, int stage_prec, int nsteps, bool stop_on_
 }
 
 void cc_reset_diis(CCState& S) { S.active.clear(); }
+
+// Defect correction: each FP64 iteration (DIIS reset, so a Jacobi step) anchors R0, t0 and the
+// intermediates; unless it meets e_conv / r_conv, the correction equation is then iterated in
+// inner_prec (with DIIS) until its step is below r_conv / 10 or stalls, and the next FP64
+// iteration starts from that solution.  Each anchor's step measures the FP64 residual itself.
+// Returns the number of FP64 iterations run.
+int cc_refine_dc(jues_ctx* ctx, CCState& S, int inner_prec, int max_anchors, int inner_maxit,
+                 jues_iter_info* last, bool* converged, jues_iter_cb cb, void* user) {
+  JUES_REQUIRE(S.method == 0, "defect-correction refinement is implemented for RCCD");
+  JUES_REQUIRE(inner_prec == JUES_FP64 || inner_prec == JUES_FP32_MIXED || inner_prec == JUES_FP16X3_MIXED,
+               "defect correction: bad inner precision");
+  JUES_REQUIRE(max_anchors >= 1 && inner_maxit >= 0, "defect correction: bad iteration limits");
+  *converged = false;
+  if (!S.dc) S.dc.reset(new DCState());
+  DCState& D = *S.dc;
+  struct Clear {
+    DCState& D;
+    ~Clear() { D.capture = D.on = false; }
+  } clear{D};
+  if (D.inner_prec != inner_prec) {
+    // the anchor copies change type
+    for (DevBuf* b : {&D.WJ0, &D.WE0, &D.Wpt0, &D.Wmt0, &D.Tt, &D.Ttt, &D.Ut, &D.Tpt, &D.Tmt, &D.Tqt}) b->release();
+    D.inner_prec = inner_prec;
+  }
+  int k = 0;
+  while (k < max_anchors) {
+    cc_reset_diis(S);
+    D.capture = true;
+    bool c = false;
+    cc_run(ctx, S, JUES_FP64, 1, false, last, &c, cb, user);
+    D.capture = false;
+    ++k;
+    if (c) {
+      *converged = true;
+      break;
+    }
+    if (k == max_anchors || inner_maxit == 0) continue;
+    cc_reset_diis(S);
+    D.on = true;
+    jues_iter_info inner{};
+    bool ci = false;
+    cc_run(ctx, S, inner_prec, inner_maxit, true, &inner, &ci, cb, user, 0.1 * S.opts.r_conv);
+    D.on = false;
+  }
+  return k;
+}
 
 }  // namespace jues
 
@@ -930,7 +1131,10 @@ jues_status cc_solve(jues_ctx* ctx, int method, int prec, const jues_cc_opts* op
       cc_run(ctx, S, prec, S.opts.maxit, true, &last, &conv, cb, user);
       if (k == nl) {
         cc_reset_diis(S);
-        cc_run(ctx, S, JUES_FP64, S.opts.refine_maxit, true, &last, &conv, cb, user);
+        if (S.opts.refine == JUES_REFINE_DEFECT && method == 0)
+          cc_refine_dc(ctx, S, prec, S.opts.refine_maxit, S.opts.maxit, &last, &conv, cb, user);
+        else
+          cc_run(ctx, S, JUES_FP64, S.opts.refine_maxit, true, &last, &conv, cb, user);
       }
     }
     if (!conv && k < nl) {
@@ -1000,6 +1204,19 @@ jues_status jues_cc_iterate(jues_ctx* ctx, int stage_prec, int nsteps, int stop_
   cc_run(ctx, *ctx->cc, stage_prec, nsteps, stop_on_conv != 0, &last, &conv, nullptr, nullptr);
   if (info) *info = last;
   if (converged) *converged = conv ? 1 : 0;
+  JUES_API_END(ctx)
+}
+
+jues_status jues_cc_refine(jues_ctx* ctx, int inner_prec, int max_anchors, int inner_maxit, jues_iter_info* info,
+                           int* converged, int* n_fp64) {
+  JUES_API_BEGIN(ctx)
+  JUES_REQUIRE(ctx->cc, "jues_cc_refine: call jues_cc_setup first");
+  jues_iter_info last{};
+  bool conv = false;
+  const int n = cc_refine_dc(ctx, *ctx->cc, inner_prec, max_anchors, inner_maxit, &last, &conv, nullptr, nullptr);
+  if (info) *info = last;
+  if (converged) *converged = conv ? 1 : 0;
+  if (n_fp64) *n_fp64 = n;
   JUES_API_END(ctx)
 }
 

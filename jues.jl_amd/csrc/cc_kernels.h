@@ -32,6 +32,7 @@ struct UpdateArgs {
   const TR* Lm;         // packed ladders (pp + hh), antisymmetric part
   const TR* extra;      // any further ijab-ordered residual terms (RCCSD), may be null
   const double* t1n;    // new T1 (o x v) for the CCSD energy term, null for RCCD
+  double* R_out;        // if set, receives the numerator R (FP64, ijab; before lambda and D)
   double lambda;        // coupling: t_new = lambda R / D, E = lambda sum Lener tau
   long npo, nmo;
   const double* t_old;
@@ -79,7 +80,7 @@ void pack_khh(jues_ctx* ctx, const PairTables& P, const double* K, int o, int v,
 template <typename T>
 void pack_whh(jues_ctx* ctx, const PairTables& P, const T* W, int o, T* WPT, T* WMT);
 // RCCD amplitude layouts from FP64 t2 [i,j,a,b]: Tpp = t, Upp = 2t - t^(ij), Tov[i,a,j,b] = t,
-// Ttov[i,a,j,b] = t[i,j,b,a], Uov[i,a,j,b] = U[i,j,a,b]
+// Ttov[i,a,j,b] = t[i,j,b,a], Uov[i,a,j,b] = U[i,j,a,b]; Tpp and Upp may be null
 template <typename T>
 void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, T* Tov, T* Ttov, T* Uov);
 // WE[(k,c),(j,b)] += s (delta_kj Fae[b,c] - delta_cb Fmi[k,j]) (ov layout; FmiT[j,k] = Fmi[k,j])

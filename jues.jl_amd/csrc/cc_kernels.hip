@@ -156,8 +156,8 @@ __global__ __launch_bounds__(256) void rccd_layouts_kernel(const double* __restr
     const int a = (int)(ab % v), b = (int)(ab / v);
     const double tv = t[x];
     const double u = 2.0 * tv - t[j + (long)o * i + oo * ab];
-    Tpp[x] = (T)tv;
-    Upp[x] = (T)u;
+    if (Tpp) Tpp[x] = (T)tv;
+    if (Upp) Upp[x] = (T)u;
     const long y = i + (long)o * a + ov * (j + (long)o * b);
     Tov[y] = (T)tv;
     Uov[y] = (T)u;
@@ -186,8 +186,8 @@ __global__ __launch_bounds__(256) void rccd_layouts_slab_kernel(const double* __
       const int i = q % o, j = q / o;
       const double tv = slab[i + ld * j];
       const double u = 2.0 * tv - slab[j + ld * i];
-      Tpp[ab * oo + q] = (T)tv;
-      Upp[ab * oo + q] = (T)u;
+      if (Tpp) Tpp[ab * oo + q] = (T)tv;
+      if (Upp) Upp[ab * oo + q] = (T)u;
       const long y = i + (long)o * a + ov * (j + (long)o * b);
       Tov[y] = (T)tv;
       Uov[y] = (T)u;
@@ -253,6 +253,7 @@ __global__ __launch_bounds__(256) void t2_update_kernel(UpdateArgs<TR> A) {
       }
     }
     if (A.extra) R += (double)A.extra[x];
+    if (A.R_out) A.R_out[x] = R;
     const double tn = A.lambda * R / (A.eps[i] + A.eps[j] + dab);
     const double e = tn - A.t_old[x];
     A.t_new[x] = tn;
@@ -498,7 +499,8 @@ void rccd_layouts(jues_ctx* ctx, const double* t, int o, int v, T* Tpp, T* Upp, 
   const long n = (long)o * o * v * v;
   const size_t slab = (size_t)o * (o + 1) * sizeof(double);
   // t is read once through the LDS slab, twice (direct and exchanged) otherwise
-  KTimer k(ctx, "cc_prep", 0.0, (double)n * ((slab <= 65536 ? 8.0 : 16.0) + 5.0 * sizeof(T)));
+  const double outs = 3.0 + (Tpp ? 1.0 : 0.0) + (Upp ? 1.0 : 0.0);
+  KTimer k(ctx, "cc_prep", 0.0, (double)n * ((slab <= 65536 ? 8.0 : 16.0) + outs * sizeof(T)));
   if (slab <= 65536) {
     const long vv = (long)v * v;
     hipLaunchKernelGGL((rccd_layouts_slab_kernel<T>), dim3((unsigned)std::min(vv, 1L << 20)), dim3(256), slab,
@@ -528,7 +530,7 @@ void t2_update(jues_ctx* ctx, const UpdateArgs<TR>& A) {
   const long v2 = (long)A.v * A.v;
   const long n2 = (long)A.o * A.o * v2;
   // S (or H and Y, each read twice), Lp, Lm and extra in the residual's compute type
-  KTimer k(ctx, "cc_update", 0.0, (double)n2 * (8.0 * (5 + A.nold) + (A.H ? 7 : 4) * sizeof(TR)));
+  KTimer k(ctx, "cc_update", 0.0, (double)n2 * (8.0 * (5 + A.nold + (A.R_out ? 1 : 0)) + (A.H ? 7 : 4) * sizeof(TR)));
   size_t lds = 0;
   if (A.H) {
     JUES_REQUIRE(A.Y && !A.S && fused_sym(A.o, sizeof(TR)), "t2_update: fused symmetrisation misconfigured");

@@ -42,6 +42,8 @@ def _opts(kw: dict, method_default_guess: str) -> _lib.CCOpts:
     o.continuation = int(kw.pop("continuation", 1))
     lad = kw.pop("ladder", "auto")
     o.ladder = _lib.LADDER[lad] if isinstance(lad, str) else int(lad)
+    ref = kw.pop("refine", "restart")
+    o.refine = _lib.REFINE[ref] if isinstance(ref, str) else int(ref)
     return o
 
 
@@ -269,6 +271,18 @@ class Solver:
 
     def reset_diis(self):
         self.ctx.check(self.ctx.lib.jues_cc_reset_diis(self.ctx.h))
+
+    def refine(self, inner_prec: str = "fp16x3", max_anchors: int = 60, inner_maxit: int = 100):
+        """Defect-correction refinement (RCCD): FP64 iterations anchoring inner solves of the
+        correction equation in inner_prec; returns the last FP64 iteration's record (with
+        n_fp64, the number of FP64 iterations) and whether it converged."""
+        info = _lib.IterInfo()
+        conv = C.c_int(0)
+        n64 = C.c_int(0)
+        self.ctx.check(self.ctx.lib.jues_cc_refine(self.ctx.h, prec_code(inner_prec), max_anchors, inner_maxit,
+                                                   C.byref(info), C.byref(conv), C.byref(n64)))
+        return dict(iter=info.iter, stage=info.stage, energy=info.energy, delta_e=info.delta_e, rnorm=info.rnorm,
+                    diis_size=info.diis_size, ms=info.ms, coupling=info.coupling, n_fp64=n64.value), bool(conv.value)
 
     def result(self):
         E = np.zeros(1)

@@ -20,6 +20,7 @@ NOALLOC_SCRATCH_BYTES = 512 << 20  # JUES_NOALLOC_SCRATCH_BYTES
 STATUS_NAMES = {0: "OK", 1: "NOT_CONVERGED", 2: "NONFINITE", 3: "OOM", 4: "BAD_ARG", 5: "HIP_ERROR"}
 FP64, FP32_MIXED, FP16X3_MIXED = 0, 1, 2
 LADDER = {"auto": 0, "stored": 1, "recompute": 2}
+REFINE = {"restart": 0, "defect": 1}
 GUESS = {"zero": 0, "mp2": 1, "ovov": 2, "user": 3}
 CCSD_EQS = {"mRCCSD": 0, "RCCSD": 1}
 
@@ -29,7 +30,7 @@ EXPORTS = (
     "jues_release", "jues_set_ao_eri", "jues_set_ao_factors", "jues_set_df_integrals", "jues_get_ao_factors",
     "jues_tei_transform", "jues_get_eri",
     "jues_rmp2", "jues_rmp2_noalloc", "jues_rmp2_grad", "jues_rccd", "jues_rccsd", "jues_cc_setup", "jues_cc_iterate",
-    "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test", "jues_gemm_pair_test",
+    "jues_cc_refine", "jues_cc_reset_diis", "jues_cc_get", "jues_kernel_stats", "jues_kernel_stats_reset", "jues_gemm_test", "jues_gemm_pair_test",
     "jues_compute_pT", "jues_rccsd_pT", "jues_ump2", "jues_get_fock", "jues_rccsd_fock",
 )
 
@@ -49,7 +50,7 @@ class CCOpts(C.Structure):
         ("maxit", C.c_int32), ("e_conv", C.c_double), ("r_conv", C.c_double), ("diis", C.c_int32),
         ("diis_max", C.c_int32), ("guess", C.c_int32), ("ccsd_eqs", C.c_int32), ("fcn", C.c_int32),
         ("fp32_r_conv", C.c_double), ("refine_maxit", C.c_int32), ("verbose", C.c_int32),
-        ("coupling", C.c_double), ("continuation", C.c_int32), ("ladder", C.c_int32),
+        ("coupling", C.c_double), ("continuation", C.c_int32), ("ladder", C.c_int32), ("refine", C.c_int32),
     ]
 
 
@@ -111,6 +112,7 @@ def load(path: os.PathLike | str | None = None) -> C.CDLL:
             "jues_cc_setup": (i, [_ctxp, i, i, C.POINTER(CCOpts), _dp, i, _dp, i, i, _dp, _dp, _dp]),
             "jues_cc_iterate": (i, [_ctxp, i, i, i, C.POINTER(IterInfo), C.POINTER(C.c_int)]),
             "jues_cc_reset_diis": (i, [_ctxp]),
+            "jues_cc_refine": (i, [_ctxp, i, i, i, C.POINTER(IterInfo), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "jues_cc_get": (i, [_ctxp, _dp, _dp, _dp]),
             "jues_kernel_stats": (i, [_ctxp, C.POINTER(C.c_char_p), _dp, _dp, _dp, _dp, i]),
             "jues_kernel_stats_reset": (None, [_ctxp, i]),

@@ -42,16 +42,19 @@ struct CCOpts
     coupling::Float64
     continuation::Int32
     ladder::Int32
+    refine::Int32
 end
 
 function CCOpts(; maxit=100, cc_max_iter=maxit, cc_e_conv=1e-10, cc_max_rms=1e-9, diis=true,
                 diis_max=5, guess=:mp2, ccsd_eqs=:mRCCSD, fcn=0, fp32_r_conv=1e-5,
-                refine_maxit=60, doprint=false, coupling=1.0, continuation=1, ladder=:auto)
+                refine_maxit=60, doprint=false, coupling=1.0, continuation=1, ladder=:auto,
+                refine=:restart)
     g = Dict(:zero => 0, :mp2 => 1, :ovov => 2, :user => 3)[guess]
     e = Dict(:mRCCSD => 0, :RCCSD => 1)[ccsd_eqs]
     l = Dict(:auto => 0, :stored => 1, :recompute => 2)[ladder]
+    r = Dict(:restart => 0, :defect => 1)[refine]
     CCOpts(cc_max_iter, cc_e_conv, cc_max_rms, diis, diis_max, g, e, fcn, fp32_r_conv,
-           refine_maxit, doprint, coupling, continuation, l)
+           refine_maxit, doprint, coupling, continuation, l, r)
 end
 
 precision_code(p::Symbol) = p === :fp64 ? JUES_FP64 : p === :fp32_mixed ? JUES_FP32_MIXED :

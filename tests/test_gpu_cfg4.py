@@ -10,7 +10,9 @@ term carries one integral: t = lambda R(t) / D), as the benchmark does.  Checks:
   every form (RCCD.jl:99-117 energy of t = lambda <ij|ab> / D);
 * FP32-mixed and split-FP16, each followed by the FP64 refinement, end within 1e-8 Eh of
   the plain FP64 fixed point, and their amplitudes within 1e-7 of its amplitudes;
-* the converged amplitudes keep the pair symmetry t[i,j,a,b] = t[j,i,b,a].
+* the converged amplitudes keep the pair symmetry t[i,j,a,b] = t[j,i,b,a];
+* with the defect-correction hand-over (refine = DEFECT) the fast forms reach the same
+  fixed point with at most five FP64 iterations.
 """
 import pathlib
 
@@ -39,11 +41,11 @@ def gold():
     return np.load(GOLD / "cfg4.npz")
 
 
-def _solve(w, prec):
+def _solve(w, prec, **kw):
     from jues import CoupledCluster as CC
     hist = []
-    E, T2 = CC.mRCCD.do_rccd(w, prec=prec, callback=hist.append, **SOLVE)
-    print(f"cfg4 {prec}: E = {E:.12f} in {len(hist)} iterations "
+    E, T2 = CC.mRCCD.do_rccd(w, prec=prec, callback=hist.append, **SOLVE, **kw)
+    print(f"cfg4 {prec} {kw}: E = {E:.12f} in {len(hist)} iterations "
           f"({sum(h['stage'] != 0 for h in hist)} fast), {sum(h['ms'] for h in hist) / 1e3:.1f} s device", flush=True)
     return E, T2, hist
 
@@ -80,3 +82,16 @@ def test_cfg4_fast_forms_reach_fp64_fixed_point(cfg4, fp64, gold, prec):
     assert abs(E - E64) < 1e-8
     assert np.max(np.abs(T2 - T64)) < 1e-7
     assert np.max(np.abs(T2 - T2.transpose(1, 0, 3, 2))) < 1e-12
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("prec", ["fp32_mixed", "fp16x3"])
+def test_cfg4_defect_correction(cfg4, fp64, prec):
+    E64, T64, _ = fp64
+    E, T2, hist = _solve(cfg4[1], prec, refine="defect")
+    n64 = sum(h["stage"] == 0 for h in hist)
+    print(f"cfg4 {prec} + defect correction: {n64} FP64 iterations, |dE| vs FP64 = {abs(E - E64):.3e}, "
+          f"max|dT2| = {np.max(np.abs(T2 - T64)):.3e}")
+    assert hist[-1]["stage"] == 0 and n64 <= 5
+    assert abs(E - E64) < 1e-8
+    assert np.max(np.abs(T2 - T64)) < 1e-7

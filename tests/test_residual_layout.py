@@ -120,6 +120,63 @@ def test_device_residual_layout_matches_oracle(n, o, naux, seed, packed):
     assert np.max(np.abs(got - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
+def _device_residual_delta(t, t0, I):
+    """The difference R(t) - R(t0) as the device's defect-correction stage forms it from
+    d = t - t0, the layouts of t and the anchor's intermediates WJ0, WE0, W'0:
+      dWJ = K U(d)/2 - Kx T(d)/2,  dWE = Kx Tx(d)/2 + D(dFae, dFmi)/2  (Fae, Fmi are linear in t),
+      dH0 = T(d) WE0 + T(t) dWE + U(d) WJ0 + U(t) dWJ,   dY = Tx(d) WE0 + Tx(t) dWE,
+      dS summed as S, the pp-ladder of d, and the hole-hole difference
+      t.W'(t) - t0.W'0 = t.dW' + d.W'0 with dW' = <mn|ef> d[ij,ef]."""
+    oovv, ovov, oooo, vvvv = I["oovv"], I["ovov"], I["oooo"], I["vvvv"]
+    o, v = t.shape[0], t.shape[2]
+    mm = lambda A, B: np.einsum("iakc,kcjb->iajb", A, B)  # noqa: E731
+    K = np.einsum("ijab->iajb", oovv)
+    Kx = np.einsum("ijba->iajb", oovv)
+    J = ovov
+    lay = lambda x: (np.einsum("ijab->iajb", x), np.einsum("ijba->iajb", x))  # noqa: E731
+    d = t - t0
+    (T0, Tx0), (Td, Txd) = lay(t0), lay(d)
+    U0, Ud = 2 * T0 - Tx0, 2 * Td - Txd
+    Ut0 = 2 * t0 - t0.transpose(1, 0, 2, 3)
+    Fae0 = -np.einsum("mnef,mnaf->ae", oovv, Ut0)
+    Fmi0 = np.einsum("mnef,inef->mi", oovv, Ut0)
+    Dm = lambda Fa, Fm: (np.einsum("kj,cb->kcjb", np.eye(o), Fa.T)  # noqa: E731
+                         - np.einsum("cb,kj->kcjb", np.eye(v), Fm))
+    WJ0 = K + mm(K, U0) / 2 - mm(Kx, T0) / 2
+    WE0 = -J + mm(Kx, Tx0) / 2 + Dm(Fae0, Fmi0) / 2
+    Utd = 2 * d - d.transpose(1, 0, 2, 3)
+    dFae = -np.einsum("mnef,mnaf->ae", oovv, Utd)
+    dFmi = np.einsum("mnef,inef->mi", oovv, Utd)
+    dWJ = mm(K, Ud) / 2 - mm(Kx, Td) / 2
+    dWE = mm(Kx, Txd) / 2 + Dm(dFae, dFmi) / 2
+    Tt, Txt, Ut = T0 + Td, Tx0 + Txd, U0 + Ud
+    dH = mm(Td, WE0) + mm(Tt, dWE) + mm(Ud, WJ0) + mm(Ut, dWJ)
+    dY = mm(Txd, WE0) + mm(Txt, dWE)
+    dS = dH + dY.transpose(0, 3, 2, 1) + dH.transpose(2, 3, 0, 1) + dY.transpose(2, 1, 0, 3)
+    W0 = oooo + np.einsum("mnef,ijef->mnij", oovv, t0)
+    dW = np.einsum("mnef,ijef->mnij", oovv, d)
+    hh = np.einsum("mnab,mnij->ijab", t, dW) + np.einsum("mnab,mnij->ijab", d, W0)
+    return dS.transpose(0, 2, 1, 3) + np.einsum("ijef,abef->ijab", d, vvvv) + hh
+
+
+@pytest.mark.parametrize("n,o,naux,seed", [(9, 3, 14, 5), (8, 1, 10, 6)])
+def test_defect_correction_delta_residual(n, o, naux, seed):
+    """R(t0) + [R(t) - R(t0)] as the defect-correction stage forms the bracket equals R(t)
+    (RCCD.jl:242-256 restated) to 1e-12, for a t far from t0; dropping either half of a
+    product's difference fails it."""
+    s, I = _system(n, o, naux, seed)
+    t0 = _amplitudes(s.o, s.v, seed + 400)
+    t = t0 + _amplitudes(s.o, s.v, seed + 500, scale=0.03)
+    R0 = O.rccd_residual_tensor(t0, I)
+    ref = O.rccd_residual_tensor(t, I)
+    dR = _device_residual_delta(t, t0, I)
+    assert np.max(np.abs(R0 + dR - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    # the linearised difference (t0 in place of t in the second halves) misses the d^2 terms
+    lin = _device_residual_delta(t0 + 0.0 * t, t0, I)
+    assert np.max(np.abs(lin)) < 1e-15
+    assert np.max(np.abs(R0 + dR - O.rccd_residual_tensor(t0, I))) > 1e-6
+
+
 def test_fock_terms_are_carried_by_we():
     """Without the D/2 dressing of WE the Fock-like terms are missing: the comparison has
     the power to see them."""
