@@ -289,7 +289,8 @@ jues_status jues_cc_reset_diis(jues_ctx* ctx);
  * max_anchors FP64 iterations, each (unless it meets e_conv / r_conv, or is the last)
  * followed by at most inner_maxit iterations of the correction equation in inner_prec
  * (JUES_FP32_MIXED, JUES_FP16X3_MIXED, or JUES_FP64, where the bracket is exact), which stop
- * once ||dt||_2 < r_conv / 10 or stall.  info receives the last FP64 iteration's record;
+ * once ||dt||_2 < max(r_conv / 10, kappa ||dt||_2 of the FP64 iteration) or stall, kappa being
+ * about what one FP64 iteration can contract with that inner form (1e-5 FP32, 1e-3 split-FP16, 0 FP64).  info receives the last FP64 iteration's record;
  * converged is set when it met the convergence test; n_fp64 (may be null) receives the
  * number of FP64 iterations run. */
 jues_status jues_cc_refine(jues_ctx* ctx, int inner_prec, int max_anchors, int inner_maxit,

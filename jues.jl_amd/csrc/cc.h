@@ -53,11 +53,12 @@ struct CCWork {
 // others hold integral matrices, split once.
 // The _T slots hold the full amplitudes' layouts beside those of the correction d in a
 // defect-correction iteration; SP_WE0 / SP_WJ0 hold the anchor's intermediates.
+// SP_LOV / SP_LOVT hold the ov x naux factor Lov and its transpose (the factored K U product).
 enum SplitSlot {
   SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT, SP_TQ, SP_TM, SP_WPT, SP_WMT,
   SP_TOV_T, SP_UOV_T, SP_TTOV_T,
   SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPT, SP_KMT, SP_LVVT, SP_LVVTT, SP_WE0, SP_WJ0,
-  SPLIT_SLOTS
+  SP_LOV, SP_LOVT, SPLIT_SLOTS
 };
 struct SplitState {
   struct Entry {
@@ -67,6 +68,7 @@ struct SplitState {
   };
   Entry e[SPLIT_SLOTS];
   Split16Buf vps, vms, tpc, tmc;  // per-slab planes of the recomputed ladder (never cached)
+  Split16Buf ult;                 // (U Lov)^T of the factored K U product (never cached)
 };
 
 // Defect-correction refinement of RCCD.  An FP64 iteration at the anchor t0 keeps its numerator

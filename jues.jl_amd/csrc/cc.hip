@@ -316,6 +316,20 @@ void ring_k_mm(jues_ctx* ctx, CCState& S, T alpha, const T* B, int slotB, T* C) 
   const long ov = (long)S.o * S.v, naux = ctx->naux;
   const T* L = ints<T>(S).Lov.template as<T>();
   T* UL = bufp<T>(work<T>(S).UL, (size_t)(ov * naux));
+  if constexpr (sizeof(T) == 4) {
+    if (S.split_on) {
+      // UL = B^T Lov (B symmetric: its planes are the TN kernel's left operand, shared with the
+      // later B WJ product); then C += alpha (Lov^T)^T (UL^T), both k = naux operands transposed
+      SplitState& Z = split_state(S);
+      const Split16& b = split_operand<float>(ctx, S, slotB, B, ov, ov);
+      const Split16& l = split_operand<double>(ctx, S, SP_LOV, S.i64->Lov.as<double>(), ov, naux);
+      gemm16_tn(ctx, ov, naux, 1.0f, b, l, 0.0f, UL, ov, "cc_ring_lr");
+      const Split16& lt = split_operand<double>(ctx, S, SP_LOVT, S.i64->Lov.as<double>(), naux, ov, true);
+      split16t<float>(ctx, UL, naux, ov, ov, Z.ult, "cc_split");
+      gemm16_tn(ctx, ov, ov, alpha, lt, Z.ult.s, 1.0f, C, ov, "cc_ring_lr");
+      return;
+    }
+  }
   gemm<T>(ctx, false, false, ov, naux, ov, T(1), B, ov, 0, L, ov, 0, T(0), UL, ov, 0, 1, "cc_ring_lr");
   gemm<T>(ctx, false, true, ov, ov, naux, alpha, L, ov, 0, UL, ov, 0, T(1), C, ov, 0, 1, "cc_ring_lr");
 }
@@ -1058,8 +1072,11 @@ void cc_reset_diis(CCState& S) { S.active.clear(); }
 
 // Defect correction: each FP64 iteration (DIIS reset, so a Jacobi step) anchors R0, t0 and the
 // intermediates; unless it meets e_conv / r_conv, the correction equation is then iterated in
-// inner_prec (with DIIS) until its step is below r_conv / 10 or stalls, and the next FP64
-// iteration starts from that solution.  Each anchor's step measures the FP64 residual itself.
+// inner_prec (with DIIS) until its step is below max(r_conv / 10, kappa ||dt||_anchor) or stalls,
+// and the next FP64 iteration starts from that solution.  Each anchor's step measures the FP64
+// residual itself.  kappa is about the factor by which one anchor can shrink the error, set by the
+// inner products' rounding (relative to |d|): 1e-5 for FP32 MFMA, 1e-3 for split-FP16, whose
+// products err by up to ~30x more; iterating the inner equation below that buys nothing.
 // Returns the number of FP64 iterations run.
 int cc_refine_dc(jues_ctx* ctx, CCState& S, int inner_prec, int max_anchors, int inner_maxit,
                  jues_iter_info* last, bool* converged, jues_iter_cb cb, void* user) {
@@ -1079,6 +1096,7 @@ int cc_refine_dc(jues_ctx* ctx, CCState& S, int inner_prec, int max_anchors, int
     for (DevBuf* b : {&D.WJ0, &D.WE0, &D.Wpt0, &D.Wmt0, &D.Tt, &D.Ttt, &D.Ut, &D.Tpt, &D.Tmt, &D.Tqt}) b->release();
     D.inner_prec = inner_prec;
   }
+  const double kappa = inner_prec == JUES_FP16X3_MIXED ? 1e-3 : (inner_prec == JUES_FP32_MIXED ? 1e-5 : 0.0);
   int k = 0;
   while (k < max_anchors) {
     cc_reset_diis(S);
@@ -1096,7 +1114,8 @@ int cc_refine_dc(jues_ctx* ctx, CCState& S, int inner_prec, int max_anchors, int
     D.on = true;
     jues_iter_info inner{};
     bool ci = false;
-    cc_run(ctx, S, inner_prec, inner_maxit, true, &inner, &ci, cb, user, 0.1 * S.opts.r_conv);
+    cc_run(ctx, S, inner_prec, inner_maxit, true, &inner, &ci, cb, user,
+           std::max(0.1 * S.opts.r_conv, kappa * last->rnorm));
     D.on = false;
   }
   return k;

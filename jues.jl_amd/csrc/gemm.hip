@@ -1014,9 +1014,11 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   const long tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   JUES_REQUIRE(tiles_m * tiles_n < (1L << 31), "gemm: too many tiles");
   // Few output tiles and a long K (the Fock-like terms): split K over workgroups,
-  // write partial products to a workspace and reduce them in a fixed order.
+  // write partial products to a workspace and reduce them in a fixed order.  The chunks fill
+  // the chip's resident workgroup slots (four FP32 or two FP64 high-occupancy workgroups per CU).
   if (allow_split && batch == 1 && tiles_m * tiles_n < 128 && K >= 4096 && !no_alloc_active()) {
-    const long want = std::max(2L, std::min(K / 1024, 512L / (tiles_m * tiles_n)));
+    const long slots = sizeof(T) == 4 ? 1024 : 512;
+    const long want = std::max(2L, std::min(K / 1024, slots / (tiles_m * tiles_n)));
     const int BKs = sizeof(T) == 8 ? 16 : 32;
     long kc = (K + want - 1) / want;
     kc = (kc + BKs - 1) / BKs * BKs;

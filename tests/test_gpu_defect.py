@@ -96,7 +96,7 @@ def test_fast_correction_error_scales_with_d(cfg1, inner):
 def test_defect_solve_reaches_fp64_fixed_point(cfg1, e_ccd, prec, ladder, dense):
     from jues import CoupledCluster as CC
     hist = []
-    E = CC.mRCCD.do_rccd(_wfn(cfg1, dense), prec=prec, ladder=ladder, refine="defect", cc_e_conv=1e-12,
+    E = CC.mRCCD.do_rccd(_wfn(cfg1, dense), prec=prec, ladder=ladder, refine="defect", cc_e_conv=1e-11,
                          cc_max_rms=1e-11, maxit=300, callback=hist.append)
     n64 = sum(h["stage"] == 0 for h in hist)
     anchors = [f"{h['rnorm']:.1e}" for h in hist if h["stage"] == 0]
