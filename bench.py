@@ -141,6 +141,16 @@ def main():
     if args.cfg2:
         print(json.dumps(cfg2_bench()), flush=True)
         return
+    if args.rccsd5 and args.converge:
+        # RCCSD at o=80 v=420 to convergence (the restart hand-over; RCCSD has no defect correction)
+        from jues import synthetic
+        from jues.Wavefunction import Wfn
+        w5 = Wfn.from_synthetic(synthetic.workload("cfg5_rccsd_n500"), dense=False)
+        res = converge(w5, args.coupling, args.fast_r_conv, args.keep_diis, args.forms.split(","), ("restart",),
+                       method="rccsd", diis_max=min(args.diis_max, 3), ladder="recompute")
+        print(json.dumps({"workload": "RCCSD o=80 v=420 n=500 naux=1500, recomputed ladder, (ia|bc) factored",
+                          "convergence": res}), flush=True)
+        return
     if args.rccsd5:
         print(json.dumps(rccsd5_bench(args.coupling, args.steps, args.warmup, args.diis_max)), flush=True)
         return
@@ -280,8 +290,11 @@ def mp2_bench() -> dict:
     timed("transform_iajb_fp32", lambda: Transformation.tei_transform(g, Co, Cv, Co, Cv, prec="fp32"), F_T, 157.3)
     E64 = timed("rmp2_fp64", lambda: MollerPlesset.do_rmp2(w), F_T, 78.6)
     E32 = timed("rmp2_fp32_mixed", lambda: MollerPlesset.do_rmp2(w, prec="fp32_mixed"), F_T, 157.3)
-    # adjoint: back-transform of G (F_T) + recomputed half-transforms (2n^4 v + 2n^4 o) + the C contractions
-    F_G = F_T + 2 * n**4 * (o + v) + 4 * n**2 * o * v * n
+    # adjoint as executed: forward (ia|jb) (F_T); the half-transforms H12, H34, each an AO index contracted
+    # with Co first (2 (2n^4 o + 2n^3 o v)); the four dE/dC slots (4 n^2 o v (o + v)) and their G builds
+    # (8 n o^2 v^2); the back-transform to dE/dg, i contracted last (2n^4 o + 2n^3 o v + 2n^2 o^2 v)
+    F_G = (F_T + 6 * n**4 * o + 6 * n**3 * o * v + 4 * n**2 * o * v * (o + v) + 2 * n**2 * o**2 * v
+           + 8 * n * o**2 * v**2)
     res = timed("rmp2_grad_fp64_with_dEdg", lambda: MollerPlesset.rmp2_grad(w), F_G, 78.6)
     out["E_mp2_fp64"] = E64
     out["E_mp2_fp32_mixed_abs_diff"] = abs(E32 - E64)
@@ -579,20 +592,22 @@ def rccsd5_bench(coupling: float, steps: int, warmup: int, diis_max: int) -> dic
 
 
 def converge(w, coupling: float, fast_r_conv: float = 1e-6, keep_diis: bool = False,
-             forms=("fp64", "fp32_mixed", "fp16x3"), refines=("restart", "defect")) -> dict:
+             forms=("fp64", "fp32_mixed", "fp16x3"), refines=("restart", "defect"), method: str = "rccd",
+             diis_max: int = 8, ladder: str = "auto") -> dict:
     """Each form to convergence: plain FP64, and each fast form (FP32-mixed, split-FP16) followed
     by each FP64 hand-over: FP64 iterations (DIIS history reset unless keep_diis), or defect
     correction (FP64 iterations anchoring inner solves in the fast form)."""
     from jues.CoupledCluster import Solver
-    res = {"coupling": coupling, "fast_r_conv": fast_r_conv, "keep_diis": keep_diis}
-    kw = dict(guess="zero", diis=True, diis_max=8, coupling=coupling, cc_e_conv=1e-10, cc_max_rms=1e-9,
-              fp32_r_conv=fast_r_conv, doprint=True)
+    res = {"method": method, "coupling": coupling, "fast_r_conv": fast_r_conv, "keep_diis": keep_diis,
+           "diis_max": diis_max}
+    kw = dict(guess="zero", diis=True, diis_max=diis_max, coupling=coupling, cc_e_conv=1e-10, cc_max_rms=1e-9,
+              fp32_r_conv=fast_r_conv, doprint=True, ladder=ladder)
     if "fp64" in forms:
-        sd = Solver(w, method="rccd", prec="fp64", **kw)
+        sd = Solver(w, method=method, prec="fp64", **kw)
         t0 = time.perf_counter()
         d64, cd = sd.iterate(200, prec="fp64", stop_on_conv=True)
         td = time.perf_counter() - t0
-        Ed = sd.result()[0]
+        Ed = sd.energy()
         print(f"[bench] fp64: {d64['iter']} it {td:.1f} s E={Ed:.12f} conv={cd}", file=sys.stderr, flush=True)
         res["fp64"] = dict(iters=d64["iter"], time_to_solution_s=round(td, 2), E=Ed, converged=cd,
                            final_rnorm=d64["rnorm"], final_dE=d64["delta_e"])
@@ -601,11 +616,11 @@ def converge(w, coupling: float, fast_r_conv: float = 1e-6, keep_diis: bool = Fa
         if form not in forms:
             continue
         for mode in refines:
-            sm = Solver(w, method="rccd", prec=form, **kw)
+            sm = Solver(w, method=method, prec=form, **kw)
             t0 = time.perf_counter()
             i1, _ = sm.iterate(200, prec=form, stop_on_conv=True)
             t1 = time.perf_counter() - t0
-            E1 = sm.result()[0]
+            E1 = sm.energy()
             t0 = time.perf_counter()
             if mode == "defect":
                 # stage 0 records are the FP64 iterations; the rest are inner correction steps
@@ -615,7 +630,7 @@ def converge(w, coupling: float, fast_r_conv: float = 1e-6, keep_diis: bool = Fa
                     sm.reset_diis()
                 i2, c2 = sm.iterate(100, prec="fp64", stop_on_conv=True)
             t2 = time.perf_counter() - t0
-            E2 = sm.result()[0]
+            E2 = sm.energy()
             n64 = i2["n_fp64"] if mode == "defect" else i2["iter"] - i1["iter"]
             print(f"[bench] {form}+{mode}: fast {i1['iter']} it {t1:.1f} s E={E1:.12f}; refine "
                   f"{i2['iter'] - i1['iter']} it ({n64} FP64) {t2:.1f} s E={E2:.12f} conv={c2}",

@@ -53,12 +53,13 @@ struct CCWork {
 // others hold integral matrices, split once.
 // The _T slots hold the full amplitudes' layouts beside those of the correction d in a
 // defect-correction iteration; SP_WE0 / SP_WJ0 hold the anchor's intermediates.
-// SP_LOV / SP_LOVT hold the ov x naux factor Lov and its transpose (the factored K U product).
+// SP_LOV / SP_LOVT hold the ov x naux factor Lov and its transpose (the factored K U product);
+// SP_LOVAT the transpose of LovT[(a,i),P] (RCCSD's (mf|ae) slabs).
 enum SplitSlot {
   SP_TOV, SP_UOV, SP_TTOV, SP_WE, SP_WJ, SP_U2, SP_Q, SP_TPT, SP_TMT, SP_TQ, SP_TM, SP_WPT, SP_WMT,
   SP_TOV_T, SP_UOV_T, SP_TTOV_T,
   SPLIT_CONST = 16, SP_K = SPLIT_CONST, SP_KX, SP_VP, SP_VM, SP_KPT, SP_KMT, SP_LVVT, SP_LVVTT, SP_WE0, SP_WJ0,
-  SP_LOV, SP_LOVT, SPLIT_SLOTS
+  SP_LOV, SP_LOVT, SP_LOVAT, SPLIT_SLOTS
 };
 struct SplitState {
   struct Entry {
@@ -69,6 +70,9 @@ struct SplitState {
   Entry e[SPLIT_SLOTS];
   Split16Buf vps, vms, tpc, tmc;  // per-slab planes of the recomputed ladder (never cached)
   Split16Buf ult;                 // (U Lov)^T of the factored K U product (never cached)
+  Split16Buf tppt, zsp;           // RCCSD's tau^T (once per iteration) and one (ef) x (a m) slab
+  const void* tppt_src = nullptr;
+  long tppt_stamp = -2;
 };
 
 // Defect-correction refinement of RCCD.  An FP64 iteration at the anchor t0 keeps its numerator
@@ -166,6 +170,12 @@ template <typename T>
 void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const double* t1);
 template <typename T>
 void rccd_delta_contractions(jues_ctx* ctx, CCState& S, const double* t2);
+// One slab of occupied m (m0 .. m0 + cnt - 1) of RCCSD's Z[(ij),(a,m)] = sum_ef tau[ij,ef] (mf|ae) with
+// (ia|bc) factored: Zs[(f,m'),(a,e)] = sum_P LovT[(f,m),P] Lvv[(ae),P], permuted to Zsp[(e,f),(a,m')],
+// times Tpp = tau [(ij),(ef)] into Z's columns (a,m) of the slab.  Split-FP16 in the split stage.
+template <typename T>
+void ovvv_z_slab(jues_ctx* ctx, CCState& S, int m0, int cnt, const T* LovT, const T* Lvv, const T* Tpp, T* Zs,
+                 T* Zsp, T* Z);
 // the anchor's intermediates, in the inner stage's type (called by the FP64 contractions)
 void dc_capture(jues_ctx* ctx, CCState& S, const double* WJ, const double* WE, const double* Wpt,
                 const double* Wmt);

@@ -337,6 +337,48 @@ void ring_k_mm(jues_ctx* ctx, CCState& S, T alpha, const T* B, int slotB, T* C) 
 template void ring_k_mm<double>(jues_ctx*, CCState&, double, const double*, int, double*);
 template void ring_k_mm<float>(jues_ctx*, CCState&, float, const float*, int, float*);
 
+template <typename T>
+void ovvv_z_slab(jues_ctx* ctx, CCState& S, int m0, int cnt, const T* LovT, const T* Lvv, const T* Tpp, T* Zs,
+                 T* Zsp, T* Z) {
+  const int o = S.o, v = S.v;
+  const long ov = (long)o * v, vv = (long)v * v, oo = (long)o * o, naux = ctx->naux, nr = (long)v * cnt;
+  bool split = false;
+  if constexpr (sizeof(T) == 4) split = S.split_on;
+  if constexpr (sizeof(T) == 4) {
+    if (split) {
+      // Zs = (LovT slab) Lvv^T from the factors' planes (k = P), split once per stage
+      const Split16& lat = split_operand<double>(ctx, S, SP_LOVAT, S.i64->LovT.as<double>(), naux, ov, true);
+      const Split16& lvt = split_operand<double>(ctx, S, SP_LVVT, S.i64->Lvv.as<double>(), naux, vv, true);
+      gemm16_tn(ctx, nr, vv, 1.0f, split16_cols(lat, (long)v * m0, nr), lvt, 0.0f, Zs, nr, "cc_ovvv_gen");
+    }
+  }
+  if (!split)
+    gemm<T>(ctx, false, true, nr, vv, naux, T(1), LovT + (long)v * m0, ov, 0, Lvv, vv, 0, T(0), Zs, nr, 0, 1,
+            "cc_ovvv_gen");
+  permute<T, T>(ctx, Zs, {v, cnt, v, v}, {3, 0, 2, 1}, Zsp, 1.0, 0.0, "cc_ovvv_gen");
+  T* Zc = Z + oo * v * m0;
+  if constexpr (sizeof(T) == 4) {
+    if (split) {
+      // tau's transposed planes (k = (ef)) once per iteration, the slab's planes every slab
+      SplitState& Zt = split_state(S);
+      if (Zt.tppt_src != static_cast<const void*>(Tpp) || Zt.tppt_stamp != (long)S.iter) {
+        split16t<float>(ctx, Tpp, vv, oo, oo, Zt.tppt, "cc_split");
+        Zt.tppt_src = Tpp;
+        Zt.tppt_stamp = S.iter;
+      }
+      split16<float>(ctx, Zsp, vv, nr, vv, Zt.zsp, "cc_split");
+      gemm16_tn(ctx, oo, nr, 1.0f, Zt.tppt.s, Zt.zsp.s, 0.0f, Zc, oo, "cc_ovvv");
+      return;
+    }
+  }
+  gemm<T>(ctx, false, false, oo, nr, vv, T(1), Tpp, oo, 0, Zsp, vv, 0, T(0), Zc, oo, 0, 1, "cc_ovvv");
+}
+
+template void ovvv_z_slab<double>(jues_ctx*, CCState&, int, int, const double*, const double*, const double*,
+                                  double*, double*, double*);
+template void ovvv_z_slab<float>(jues_ctx*, CCState&, int, int, const float*, const float*, const float*, float*,
+                                 float*, float*);
+
 // The ladder with <ab|ef> rebuilt from the factors every iteration (DF-RCCD.jl:198-215
 // contracts bvv[a,e,Q] bvv[b,f,Q] the same way): for a slab of f, X[e,a,b,f] =
 // sum_P L[ea,P] L[bf,P] = (ae|bf); its rows of V+/V- (pairs e<=f / e<f with f in the slab)

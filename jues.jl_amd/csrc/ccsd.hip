@@ -270,11 +270,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
     T* Zsp = bufp_t<T>(W.Zsp, (size_t)(ms * per_m));
     for (int m0 = 0; m0 < o; m0 += (int)ms) {
       const int cnt = (int)std::min<long>(ms, o - m0);
-      gemm<T>(ctx, false, true, (long)v * cnt, vv, naux, one, LovT + (long)v * m0, ov, 0, Lvv, vv, 0, zero, Zs,
-              (long)v * cnt, 0, 1, "cc_ovvv_gen");
-      permute<T, T>(ctx, Zs, {v, cnt, v, v}, {3, 0, 2, 1}, Zsp, 1.0, 0.0, "cc_ovvv_gen");
-      gemm<T>(ctx, false, false, oo, (long)v * cnt, vv, one, Tpp, oo, 0, Zsp, vv, 0, zero, Z + oo * v * m0, oo, 0, 1,
-              "cc_ovvv");
+      ovvv_z_slab<T>(ctx, S, m0, cnt, LovT, Lvv, Tpp, Zs, Zsp, Z);
     }
     gemm<T>(ctx, false, false, oo * v, v, o, T(-1), Z, oo * v, 0, t1, o, 0, zero, Q, oo * v, 0, 1, "cc_t1");
   } else {

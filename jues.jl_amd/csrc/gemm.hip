@@ -1016,9 +1016,23 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   // Few output tiles and a long K (the Fock-like terms): split K over workgroups,
   // write partial products to a workspace and reduce them in a fixed order.  The chunks fill
   // the chip's resident workgroup slots (four FP32 or two FP64 high-occupancy workgroups per CU).
-  if (allow_split && batch == 1 && tiles_m * tiles_n < 128 && K >= 4096 && !no_alloc_active()) {
-    const long slots = sizeof(T) == 4 ? 1024 : 512;
-    const long want = std::max(2L, std::min(K / 1024, slots / (tiles_m * tiles_n)));
+  // A long-K product whose tiles fill their last round of slots poorly (under 80% overall, such as
+  // the o^2 x v.m x v^2 products of RCCSD's (ia|bc) terms) is split into the fewest chunks that
+  // fill the rounds, when the partial products fit in 2 GiB.
+  const long slots = sizeof(T) == 4 ? 1024 : 512;
+  const long tiles = tiles_m * tiles_n;
+  auto round_fill = [&](long s) {
+    const long t = tiles * s;
+    return (double)t / (double)(((t + slots - 1) / slots) * slots);
+  };
+  long s_fill = 1;
+  if (allow_split && batch == 1 && shape == 0 && tiles >= 128 && K >= 32768 && !no_alloc_active() &&
+      round_fill(1) < 0.8) {
+    for (long s = 2; s <= 16; ++s)
+      if (round_fill(s) > round_fill(s_fill) + 0.02 && (double)s * M * N * sizeof(T) <= double(2L << 30)) s_fill = s;
+  }
+  if (allow_split && batch == 1 && (tiles < 128 || s_fill > 1) && K >= 4096 && !no_alloc_active()) {
+    const long want = s_fill > 1 ? s_fill : std::max(2L, std::min(K / 1024, slots / tiles));
     const int BKs = sizeof(T) == 8 ? 16 : 32;
     long kc = (K + want - 1) / want;
     kc = (kc + BKs - 1) / BKs * BKs;

@@ -541,11 +541,22 @@ void gemm16_tn(jues_ctx* ctx, long M, long N, float alpha, const Split16& A, con
                            : std::strcmp(tag, "cc_hh") == 0        ? 3
                                                                    : 0);
   // Fewer tiles than two rounds of workgroups (one per CU) and a long K: split K into S
-  // chunks in one launch, partial products to a workspace, summed into C in a fixed order.
+  // chunks in one launch, partial products to a workspace, summed into C in a fixed order.  A
+  // long-K product whose tiles fill their last round poorly (under 80% overall) is split into
+  // the fewest chunks that fill the rounds, when the partial products fit in 2 GiB.
   const long Kp = A.rows_p, tiles = tiles_m * tiles_n;
-  long S = std::min(Kp / 1024, (512 + tiles - 1) / tiles);
+  long S = tiles < 256 ? std::min(Kp / 1024, (512 + tiles - 1) / tiles) : 1;
+  auto round_fill = [&](long s) {
+    const long t = tiles * s;
+    return (double)t / (double)(((t + 255) / 256) * 256);
+  };
+  if (tiles >= 256 && Kp >= 32768 && round_fill(1) < 0.8 && !no_alloc_active()) {
+    S = 1;
+    for (long s = 2; s <= 16; ++s)
+      if (round_fill(s) > round_fill(S) + 0.02 && 4.0 * s * M * N <= double(2L << 30)) S = s;
+  }
   long kchunk = Kp;
-  if (tiles < 256 && S >= 2) {
+  if (S >= 2) {
     kchunk = ((Kp + S - 1) / S + BK - 1) / BK * BK;
     S = (Kp + kchunk - 1) / kchunk;
   } else {

@@ -219,9 +219,13 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
   T* H34 = H34b.as<T>();  // [i,a,lam,sig]
   const T* g = ao_dense<T>(ctx);
   {
-    DevBuf Y(n3 * v * sizeof(T));  // Y[mu,nu,lam,b] = sum_sig g Cv[sig,b]
-    gemm<T>(ctx, false, false, n3, v, n, one, g, n3, 0, Cv, n, 0, zero, Y.as<T>(), n3, 0, 1, "mp2grad_half");
-    gemm<T>(ctx, false, false, n2, o, n, one, Y.as<T>(), n2, n3, Co, n, 0, zero, H12, n2, n2 * o, v,
+    // X[mu,nu,j,sig] = sum_lam g[mu,nu,lam,sig] Co[lam,j] (batched over sig), then
+    // H12[(mu,nu,j),b] = sum_sig X[(mu,nu,j),sig] Cv[sig,b]: 2n^4 o + 2n^3 o v flops and an n^3 o
+    // intermediate (contracting sig with Cv first takes 2n^4 v and n^3 v)
+    DevBuf X(n3 * o * sizeof(T));
+    gemm<T>(ctx, false, false, n2, o, n, one, g, n2, n3, Co, n, 0, zero, X.as<T>(), n2, n2 * o, (int)n,
+            "mp2grad_half");
+    gemm<T>(ctx, false, false, n2 * o, v, n, one, X.as<T>(), n2 * o, 0, Cv, n, 0, zero, H12, n2 * o, 0, 1,
             "mp2grad_half");
   }
   {
@@ -232,13 +236,11 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
   }
   T* dCo = dC;           // n x o  (columns 0..o-1 of dC, leading dimension n)
   T* dCv = dC + n * o;   // n x v
-  // slot 1: Gt[i,nu,jb] = G[i,a,jb] Cv[nu,a]^T ; dC1 = H12 (n x n ov) . Gt^T
-  {
-    DevBuf Gt((long)o * n * ov * sizeof(T));
-    gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, (long)o * n, (int)ov,
-            "mp2grad");
-    gemm<T>(ctx, false, true, n, o, n * ov, one, H12, n, 0, Gt.as<T>(), o, 0, zero, dCo, n, 0, 1, "mp2grad");
-  }
+  // slot 1: Gt[i,nu,jb] = G[i,a,jb] Cv[nu,a]^T ; dC1 = H12 (n x n ov) . Gt^T  (Gt is also the first
+  // step of the dE/dg back-transform)
+  DevBuf Gt((long)o * n * ov * sizeof(T));
+  gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, (long)o * n, (int)ov, "mp2grad");
+  gemm<T>(ctx, false, true, n, o, n * ov, one, H12, n, 0, Gt.as<T>(), o, 0, zero, dCo, n, 0, 1, "mp2grad");
   // slot 2: Gh[mu,a,jb] = Co . G ; dC2[nu,a] = sum_{mu,jb} H12[mu,nu,jb] Gh[mu,a,jb]
   {
     DevBuf Gh(n * v * ov * sizeof(T)), H12p(n2 * ov * sizeof(T)), Ghp(n * v * ov * sizeof(T));
@@ -267,26 +269,22 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
   H12b.release();
   H34b.release();
   if (dg) {
-    // back-transform G -> dE/dg: [i,a,j,b] -> Gh[mu,a,j,b] -> X[mu,a,j,sig] -> [a,mu,sig,j]
-    //   -> [nu,mu,sig,j] -> [nu,mu,sig,lam] -> [mu,nu,lam,sig]
-    DevBuf Gh(n * v * ov * sizeof(T));
-    gemm<T>(ctx, false, false, n, v * ov, o, one, Co, n, 0, G, o, 0, zero, Gh.as<T>(), n, 0, 1, "mp2grad_back");
-    DevBuf X(n * v * o * n * sizeof(T));
-    gemm<T>(ctx, false, true, n * v * o, n, v, one, Gh.as<T>(), n * v * o, 0, Cv, n, 0, zero, X.as<T>(), n * v * o,
-            0, 1, "mp2grad_back");
-    Gh.release();
-    DevBuf Xp(n * v * o * n * sizeof(T));
-    permute<T, T>(ctx, X.as<T>(), {n, v, o, n}, {1, 0, 3, 2}, Xp.as<T>(), 1.0, 0.0, "mp2grad_perm");  // [a,mu,sig,j]
-    X.release();
-    DevBuf Y(n3 * o * sizeof(T));
-    gemm<T>(ctx, false, false, n, n * n * o, v, one, Cv, n, 0, Xp.as<T>(), v, 0, zero, Y.as<T>(), n, 0, 1,
-            "mp2grad_back");  // [nu,mu,sig,j]
-    Xp.release();
-    DevBuf W(n3 * n * sizeof(T));
-    gemm<T>(ctx, false, true, n3, n, o, one, Y.as<T>(), n3, 0, Co, n, 0, zero, W.as<T>(), n3, 0, 1,
-            "mp2grad_back");  // [nu,mu,sig,lam]
-    Y.release();
-    permute<T, T>(ctx, W.as<T>(), {n, n, n, n}, {1, 0, 3, 2}, dg, 1.0, 0.0, "mp2grad_perm");
+    // back-transform G -> dE/dg with the occupied index i contracted last, so that the n^4 result
+    // comes out of one GEMM already in [mu,nu,lam,sig] order (no permutation pass):
+    //   Gt[(i,nu),j,b]   = sum_a G[i,a,j,b] Cv[nu,a]             (slot 1)
+    //   P2[(i,nu),lam,b] = sum_j Gt[(i,nu),j,b] Co[lam,j]        (batched over b)
+    //   Q[(i,nu,lam),sig] = sum_b P2[(i,nu,lam),b] Cv[sig,b]
+    //   dg[mu,(nu,lam,sig)] = sum_i Co[mu,i] Q[i,(nu,lam,sig)]
+    const long on = (long)o * n;
+    DevBuf P2(on * n * v * sizeof(T));
+    gemm<T>(ctx, false, true, on, n, o, one, Gt.as<T>(), on, on * o, Co, n, 0, zero, P2.as<T>(), on, on * n, v,
+            "mp2grad_back");
+    Gt.release();
+    DevBuf Q(on * n2 * sizeof(T));
+    gemm<T>(ctx, false, true, on * n, n, v, one, P2.as<T>(), on * n, 0, Cv, n, 0, zero, Q.as<T>(), on * n, 0, 1,
+            "mp2grad_back");
+    P2.release();
+    gemm<T>(ctx, false, false, n, n3, o, one, Co, n, 0, Q.as<T>(), o, 0, zero, dg, n, 0, 1, "mp2grad_back");
   }
 }
 

@@ -284,6 +284,12 @@ class Solver:
         return dict(iter=info.iter, stage=info.stage, energy=info.energy, delta_e=info.delta_e, rnorm=info.rnorm,
                     diis_size=info.diis_size, ms=info.ms, coupling=info.coupling, n_fp64=n64.value), bool(conv.value)
 
+    def energy(self) -> float:
+        """Correlation energy of the resident amplitudes (no amplitude copy to the host)."""
+        E = np.zeros(1)
+        self.ctx.check(self.ctx.lib.jues_cc_get(self.ctx.h, _lib.ptr(E), None, None))
+        return float(E[0])
+
     def result(self):
         E = np.zeros(1)
         T2 = np.zeros((self.o, self.o, self.v, self.v), order="F")
