@@ -73,6 +73,9 @@ struct SplitState {
   Split16Buf tppt, zsp;           // RCCSD's tau^T (once per iteration) and one (ef) x (a m) slab
   const void* tppt_src = nullptr;
   long tppt_stamp = -2;
+  Split16Buf y2t, y3t;            // RCCSD's Y2^T (once per iteration, two products) and Y3^T
+  const void* y2t_src = nullptr;
+  long y2t_stamp = -2;
 };
 
 // Defect-correction refinement of RCCD.  An FP64 iteration at the anchor t0 keeps its numerator
@@ -176,6 +179,14 @@ void rccd_delta_contractions(jues_ctx* ctx, CCState& S, const double* t2);
 template <typename T>
 void ovvv_z_slab(jues_ctx* ctx, CCState& S, int m0, int cnt, const T* LovT, const T* Lvv, const T* Tpp, T* Zs,
                  T* Zsp, T* Z);
+// RCCSD's (ia|bc) factor products (split-FP16 in the split stage): W1 = Tov Lov (ov x naux);
+// C += Lov Y2^T (y2_right) or C += Y2 Lov^T (Y2: ov x naux); Q = Y3 Lvv^T (Y3: o^2 x naux)
+template <typename T>
+void fac_tov_lov(jues_ctx* ctx, CCState& S, const T* Tov, T* W1);
+template <typename T>
+void fac_lov_y2(jues_ctx* ctx, CCState& S, const T* Y2, bool y2_right, T* C);
+template <typename T>
+void fac_y3_lvv(jues_ctx* ctx, CCState& S, const T* Y3, T* Q);
 // the anchor's intermediates, in the inner stage's type (called by the FP64 contractions)
 void dc_capture(jues_ctx* ctx, CCState& S, const double* WJ, const double* WE, const double* Wpt,
                 const double* Wmt);

@@ -376,6 +376,67 @@ void ovvv_z_slab(jues_ctx* ctx, CCState& S, int m0, int cnt, const T* LovT, cons
 
 template void ovvv_z_slab<double>(jues_ctx*, CCState&, int, int, const double*, const double*, const double*,
                                   double*, double*, double*);
+
+template <typename T>
+void fac_tov_lov(jues_ctx* ctx, CCState& S, const T* Tov, T* W1) {
+  const long ov = (long)S.o * S.v, naux = ctx->naux;
+  if constexpr (sizeof(T) == 4) {
+    if (S.split_on) {
+      // Tov is symmetric: its ring planes serve as the TN kernel's left operand
+      const Split16& t = split_operand<float>(ctx, S, SP_TOV, Tov, ov, ov);
+      const Split16& l = split_operand<double>(ctx, S, SP_LOV, S.i64->Lov.as<double>(), ov, naux);
+      gemm16_tn(ctx, ov, naux, 1.0f, t, l, 0.0f, W1, ov, "cc_ovvv_fac");
+      return;
+    }
+  }
+  gemm<T>(ctx, false, false, ov, naux, ov, T(1), Tov, ov, 0, ints<T>(S).Lov.template as<T>(), ov, 0, T(0), W1, ov,
+          0, 1, "cc_ovvv_fac");
+}
+
+template <typename T>
+void fac_lov_y2(jues_ctx* ctx, CCState& S, const T* Y2, bool y2_right, T* C) {
+  const long ov = (long)S.o * S.v, naux = ctx->naux;
+  if constexpr (sizeof(T) == 4) {
+    if (S.split_on) {
+      SplitState& Z = split_state(S);
+      if (Z.y2t_src != static_cast<const void*>(Y2) || Z.y2t_stamp != (long)S.iter) {
+        split16t<float>(ctx, Y2, naux, ov, ov, Z.y2t, "cc_split");
+        Z.y2t_src = Y2;
+        Z.y2t_stamp = S.iter;
+      }
+      const Split16& lt = split_operand<double>(ctx, S, SP_LOVT, S.i64->Lov.as<double>(), naux, ov, true);
+      if (y2_right) gemm16_tn(ctx, ov, ov, 1.0f, lt, Z.y2t.s, 1.0f, C, ov, "cc_ovvv_fac");
+      else gemm16_tn(ctx, ov, ov, 1.0f, Z.y2t.s, lt, 1.0f, C, ov, "cc_ovvv_fac");
+      return;
+    }
+  }
+  const T* L = ints<T>(S).Lov.template as<T>();
+  if (y2_right) gemm<T>(ctx, false, true, ov, ov, naux, T(1), L, ov, 0, Y2, ov, 0, T(1), C, ov, 0, 1, "cc_ovvv_fac");
+  else gemm<T>(ctx, false, true, ov, ov, naux, T(1), Y2, ov, 0, L, ov, 0, T(1), C, ov, 0, 1, "cc_ovvv_fac");
+}
+
+template <typename T>
+void fac_y3_lvv(jues_ctx* ctx, CCState& S, const T* Y3, T* Q) {
+  const long oo = (long)S.o * S.o, vv = (long)S.v * S.v, naux = ctx->naux;
+  if constexpr (sizeof(T) == 4) {
+    if (S.split_on) {
+      SplitState& Z = split_state(S);
+      split16t<float>(ctx, Y3, naux, oo, oo, Z.y3t, "cc_split");
+      const Split16& lvt = split_operand<double>(ctx, S, SP_LVVT, S.i64->Lvv.as<double>(), naux, vv, true);
+      gemm16_tn(ctx, oo, vv, 1.0f, Z.y3t.s, lvt, 0.0f, Q, oo, "cc_ovvv_fac");
+      return;
+    }
+  }
+  gemm<T>(ctx, false, true, oo, vv, naux, T(1), Y3, oo, 0, ints<T>(S).Lvv.template as<T>(), vv, 0, T(0), Q, oo, 0,
+          1, "cc_ovvv_fac");
+}
+
+template void fac_tov_lov<double>(jues_ctx*, CCState&, const double*, double*);
+template void fac_tov_lov<float>(jues_ctx*, CCState&, const float*, float*);
+template void fac_lov_y2<double>(jues_ctx*, CCState&, const double*, bool, double*);
+template void fac_lov_y2<float>(jues_ctx*, CCState&, const float*, bool, float*);
+template void fac_y3_lvv<double>(jues_ctx*, CCState&, const double*, double*);
+template void fac_y3_lvv<float>(jues_ctx*, CCState&, const float*, float*);
 template void ovvv_z_slab<float>(jues_ctx*, CCState&, int, int, const float*, const float*, const float*, float*,
                                  float*, float*);
 

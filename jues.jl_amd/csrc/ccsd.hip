@@ -237,7 +237,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
     // written straight into [i,f,P]), n1 -= W2[i,(fP)] Lvv[a,(fP)]
     T* W1 = bufp_t<T>(W.W1, ov * naux);
     T* W2 = bufp_t<T>(W.W2, ov * naux);
-    gemm<T>(ctx, false, false, ov, naux, ov, one, Tov, ov, 0, Lov, ov, 0, zero, W1, ov, 0, 1, "cc_ovvv_fac");
+    fac_tov_lov<T>(ctx, S, Tov, W1);
     gemm<T>(ctx, false, true, o, v, v * naux, T(2), W1, o, 0, Lvv, v, 0, one, n1, o, 0, 1, "cc_ovvv_fac");
     gemm<T>(ctx, false, false, o, naux, ov, one, Tov, o, oo * v, LovT, ov, 0, zero, W2, ov, o, v, "cc_ovvv_fac");
     gemm<T>(ctx, false, true, o, v, v * naux, T(-1), W2, o, 0, Lvv, v, 0, one, n1, o, 0, 1, "cc_ovvv_fac");
@@ -290,7 +290,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   ring_k_mm<T>(ctx, S, half, U2, SP_U2, WJ);
   ring_mm<T>(ctx, S, -half, Kx, SP_KX, Tov, SP_TOV, one, WJ);
   // WJ[me,jb] += sum_f t1[jf] (me|bf)
-  if (fac) gemm<T>(ctx, false, true, ov, ov, naux, one, Lov, ov, 0, Y2, ov, 0, one, WJ, ov, 0, 1, "cc_ovvv_fac");
+  if (fac) fac_lov_y2<T>(ctx, S, Y2, true, WJ);
   else contract<T>(ctx, 1.0, TN(t1, "jf"), TN(ovvv, "mebf"), 1.0, TN(WJ, "mejb"), "cc_t1");
   // (H is free until the ring products: the [b,j,m,e] product of the ooov terms goes there)
   contract<T>(ctx, -1.0, TN(t1, "nb"), TN(ooov, "njme"), 1.0, TN(WJ, "mejb"), "cc_t1", H, N2);
@@ -299,7 +299,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   // WE[me,jb] -= sum_f t1[jf] (mf|be): with factors, Q[(mj),(be)] = sum_P Y3[(mj),P] Lvv[(be),P]
   // (Q is free once the ring product above has read it) permuted into WE
   if (fac) {
-    gemm<T>(ctx, false, true, oo, vv, naux, one, Y3, oo, 0, Lvv, vv, 0, zero, Q, oo, 0, 1, "cc_ovvv_fac");
+    fac_y3_lvv<T>(ctx, S, Y3, Q);
     permute<T, T>(ctx, Q, d4, {0, 3, 1, 2}, WE, -1.0, 1.0, "cc_ovvv_fac");
   } else {
     contract<T>(ctx, -1.0, TN(t1, "jf"), TN(ovvv, "mfbe"), 1.0, TN(WE, "mejb"), "cc_t1");
@@ -321,7 +321,7 @@ void rccsd_contractions(jues_ctx* ctx, CCState& S, const double* t2, const doubl
   contract<T>(ctx, 1.0, TN(t1, "ie"), TN(Jov, "maje"), 0.0, TN(Q, "imaj"), "cc_t1");
   contract<T>(ctx, -1.0, TN(t1, "mb"), TN(Q, "imaj"), 1.0, TN(H, "iajb"), "cc_t1", U2, N2);
   // T1-linear terms: t1[ie] (jb|ae) - t1[ma] (mi|jb); with factors the first is Y2[(ia),P] Lov[(jb),P]
-  if (fac) gemm<T>(ctx, false, true, ov, ov, naux, one, Y2, ov, 0, Lov, ov, 0, one, H, ov, 0, 1, "cc_ovvv_fac");
+  if (fac) fac_lov_y2<T>(ctx, S, Y2, false, H);
   else contract<T>(ctx, 1.0, TN(t1, "ie"), TN(ovvv, "jbae"), 1.0, TN(H, "iajb"), "cc_t1");
   contract<T>(ctx, -1.0, TN(t1, "ma"), TN(ooov, "mijb"), 1.0, TN(H, "iajb"), "cc_t1", U2, N2);
   // S = H + Y^x + (H + Y^x)^T: summed by the update kernel, or here (into WJ) when its LDS slab

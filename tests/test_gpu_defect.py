@@ -119,3 +119,18 @@ def test_rccsd_falls_back_to_restart(cfg1):
     sv = Solver(w, method="rccsd", prec="fp32_mixed")
     with pytest.raises(_lib.JuesError, match="RCCD"):
         sv.refine("fp32_mixed")
+
+
+@pytest.mark.parametrize("prec", ["fp32_mixed", "fp16x3"])
+def test_defect_solve_with_released_fp64_blocks(cfg1, e_ccd, prec, monkeypatch):
+    """With the FP64 integral blocks released during every fast stage (JUES_THIN_FP64_INTS=1, what
+    low device memory triggers at o=80, v=420), each FP64 anchor rebuilds them and each inner solve
+    converts its FP32 copies again: the solve still reaches the oracle's fixed point, with the same
+    energy as when nothing is released."""
+    from jues import CoupledCluster as CC
+    kw = dict(prec=prec, refine="defect", cc_e_conv=1e-11, cc_max_rms=1e-11, maxit=300)
+    E_keep = CC.mRCCD.do_rccd(_wfn(cfg1), **kw)
+    monkeypatch.setenv("JUES_THIN_FP64_INTS", "1")
+    E_thin = CC.mRCCD.do_rccd(_wfn(cfg1), **kw)
+    assert abs(E_thin - e_ccd) < 1e-10
+    assert E_thin == E_keep
