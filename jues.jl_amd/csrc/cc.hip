@@ -1188,17 +1188,15 @@ int cc_refine_dc(jues_ctx* ctx, CCState& S, int inner_prec, int max_anchors, int
                "defect correction: bad inner precision");
   JUES_REQUIRE(max_anchors >= 1 && inner_maxit >= 0, "defect correction: bad iteration limits");
   *converged = false;
-  if (!S.dc) S.dc.reset(new DCState());
+  S.dc.reset(new DCState());
   DCState& D = *S.dc;
+  D.inner_prec = inner_prec;
   struct Clear {
-    DCState& D;
-    ~Clear() { D.capture = D.on = false; }
-  } clear{D};
-  if (D.inner_prec != inner_prec) {
-    // the anchor copies change type
-    for (DevBuf* b : {&D.WJ0, &D.WE0, &D.Wpt0, &D.Wmt0, &D.Tt, &D.Ttt, &D.Ut, &D.Tpt, &D.Tmt, &D.Tqt}) b->release();
-    D.inner_prec = inner_prec;
-  }
+    CCState& S;
+    ~Clear() {
+      if (S.dc) S.dc->capture = S.dc->on = false;
+    }
+  } clear{S};
   const double kappa = inner_prec == JUES_FP16X3_MIXED ? 1e-3 : (inner_prec == JUES_FP32_MIXED ? 1e-5 : 0.0);
   int k = 0;
   while (k < max_anchors) {
@@ -1221,6 +1219,10 @@ int cc_refine_dc(jues_ctx* ctx, CCState& S, int inner_prec, int max_anchors, int
            std::max(0.1 * S.opts.r_conv, kappa * last->rnorm));
     D.on = false;
   }
+  // the anchor's numerator, amplitudes and intermediates (about 20 GB at o=60, v=340) are not
+  // kept once the FP64 iterations end
+  sync(ctx);
+  S.dc.reset();
   return k;
 }
 

@@ -134,3 +134,22 @@ def test_defect_solve_with_released_fp64_blocks(cfg1, e_ccd, prec, monkeypatch):
     E_thin = CC.mRCCD.do_rccd(_wfn(cfg1), **kw)
     assert abs(E_thin - e_ccd) < 1e-10
     assert E_thin == E_keep
+
+
+def test_refine_bad_arguments(cfg1):
+    """A refine value other than RESTART / DEFECT, an unknown inner precision and zero anchors
+    return JUES_BAD_ARG before any device work; the state stays usable."""
+    from jues import _lib
+    from jues import CoupledCluster as CC
+    from jues.CoupledCluster import Solver
+    w = _wfn(cfg1)
+    with pytest.raises(_lib.JuesError) as e:
+        CC.mRCCD.do_rccd(w, prec="fp32_mixed", refine=7)
+    assert e.value.status == _lib.JUES_BAD_ARG
+    sv = _solver(w, "fp32_mixed")
+    for args in ((9, 2, 5), (_lib.FP32_MIXED, 0, 5), (_lib.FP32_MIXED, 2, -1)):
+        info, conv, n64 = _lib.IterInfo(), _lib.C.c_int(0), _lib.C.c_int(0)
+        st = sv.ctx.lib.jues_cc_refine(sv.ctx.h, *args, _lib.C.byref(info), _lib.C.byref(conv), _lib.C.byref(n64))
+        assert st == _lib.JUES_BAD_ARG, args
+    info, conv = sv.refine("fp32_mixed", max_anchors=1, inner_maxit=0)
+    assert info["n_fp64"] == 1 and np.isfinite(info["energy"])

@@ -25,9 +25,15 @@ def test_library_exports_every_symbol():
 
 
 def test_default_opts():
+    """Every field of the ctypes mirror reads the documented default: a field added to or moved in
+    jues_cc_opts without the mirror shifts the later ones and fails here (the last field included)."""
     from jues import _lib
     o = _lib.default_opts()
     assert o.maxit == 100 and o.diis == 1 and o.diis_max == 5 and o.e_conv == 1e-10
+    assert o.r_conv == 1e-9 and o.guess == _lib.GUESS["mp2"] and o.ccsd_eqs == 0 and o.fcn == 0
+    assert o.fp32_r_conv == 1e-5 and o.refine_maxit == 60 and o.verbose == 0
+    assert o.coupling == 1.0 and o.continuation == 1 and o.ladder == _lib.LADDER["auto"]
+    assert o.refine == _lib.REFINE["restart"]
 
 
 def test_no_oracle_in_product_path():
