@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--fast-r-conv", type=float, default=1e-6,
                     help="--converge: the fast stage stops once ||dt|| is below this (or it stalls); at o=60 "
                          "v=340, 1e-6 beats both 1e-5 and 1e-7 in time to solution")
+    ap.add_argument("--fast-r-conv-defect", type=float, default=1e-5,
+                    help="--converge: the same for the defect-correction hand-over, whose refinement costs the "
+                         "same wherever the fast stage stops (at o=60 v=340, 1e-5 beats 1e-6 and 1e-7)")
     ap.add_argument("--keep-diis", action="store_true",
                     help="--converge: keep the fast stage's DIIS history for the FP64 refinement")
     ap.add_argument("--forms", default="fp64,fp32_mixed,fp16x3", help="--converge: forms to run")
@@ -238,7 +241,7 @@ def main():
                                "host_cpus": os.cpu_count(), **cpu_baseline.host_info()}
     if args.converge:
         out["convergence"] = converge(w, args.coupling, args.fast_r_conv, args.keep_diis, args.forms.split(","),
-                                      args.refine.split(","))
+                                      args.refine.split(","), fast_r_conv_defect=args.fast_r_conv_defect)
     print(json.dumps(out), flush=True)
 
 
@@ -593,13 +596,14 @@ def rccsd5_bench(coupling: float, steps: int, warmup: int, diis_max: int) -> dic
 
 def converge(w, coupling: float, fast_r_conv: float = 1e-6, keep_diis: bool = False,
              forms=("fp64", "fp32_mixed", "fp16x3"), refines=("restart", "defect"), method: str = "rccd",
-             diis_max: int = 8, ladder: str = "auto") -> dict:
+             diis_max: int = 8, ladder: str = "auto", fast_r_conv_defect: float | None = None) -> dict:
     """Each form to convergence: plain FP64, and each fast form (FP32-mixed, split-FP16) followed
     by each FP64 hand-over: FP64 iterations (DIIS history reset unless keep_diis), or defect
     correction (FP64 iterations anchoring inner solves in the fast form)."""
     from jues.CoupledCluster import Solver
-    res = {"method": method, "coupling": coupling, "fast_r_conv": fast_r_conv, "keep_diis": keep_diis,
-           "diis_max": diis_max}
+    fr_dc = fast_r_conv if fast_r_conv_defect is None else fast_r_conv_defect
+    res = {"method": method, "coupling": coupling, "fast_r_conv": fast_r_conv, "fast_r_conv_defect": fr_dc,
+           "keep_diis": keep_diis, "diis_max": diis_max}
     kw = dict(guess="zero", diis=True, diis_max=diis_max, coupling=coupling, cc_e_conv=1e-10, cc_max_rms=1e-9,
               fp32_r_conv=fast_r_conv, doprint=True, ladder=ladder)
     if "fp64" in forms:
@@ -616,7 +620,7 @@ def converge(w, coupling: float, fast_r_conv: float = 1e-6, keep_diis: bool = Fa
         if form not in forms:
             continue
         for mode in refines:
-            sm = Solver(w, method=method, prec=form, **kw)
+            sm = Solver(w, method=method, prec=form, **dict(kw, fp32_r_conv=fr_dc if mode == "defect" else fast_r_conv))
             t0 = time.perf_counter()
             i1, _ = sm.iterate(200, prec=form, stop_on_conv=True)
             t1 = time.perf_counter() - t0
