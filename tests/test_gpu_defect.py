@@ -153,3 +153,15 @@ def test_refine_bad_arguments(cfg1):
         assert st == _lib.JUES_BAD_ARG, args
     info, conv = sv.refine("fp32_mixed", max_anchors=1, inner_maxit=0)
     assert info["n_fp64"] == 1 and np.isfinite(info["energy"])
+
+
+@pytest.mark.parametrize("extra", [dict(fcn=1), dict(coupling=0.8, continuation=2)])
+def test_defect_solve_with_frozen_core_and_continuation(cfg1, extra):
+    """Frozen core (get_eri's fcn) and coupling continuation (the defect correction runs at the last
+    coupling only): the same fixed point as the plain FP64 solve with the same options."""
+    from jues import CoupledCluster as CC
+    kw = dict(cc_e_conv=1e-12, cc_max_rms=1e-11, maxit=300, **extra)
+    E64 = CC.mRCCD.do_rccd(_wfn(cfg1), prec="fp64", **kw)
+    for prec in ("fp32_mixed", "fp16x3"):
+        E = CC.mRCCD.do_rccd(_wfn(cfg1), prec=prec, refine="defect", **kw)
+        assert abs(E - E64) < 1e-10, (prec, extra, E, E64)

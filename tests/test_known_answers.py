@@ -117,6 +117,18 @@ def test_gpu_rccd_known_answer(water_psi4, mod, prec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp32_mixed", "fp16x3"])
+def test_gpu_rccd_known_answer_defect_correction(water_psi4, prec):
+    """The fast forms with the defect-correction hand-over reach the reference's water CCD energy."""
+    from jues import CoupledCluster as CC
+    hist = []
+    E = CC.mRCCD.do_rccd(_wfn(water_psi4), prec=prec, refine="defect", cc_e_conv=1e-12, cc_max_rms=1e-11,
+                         maxit=200, callback=hist.append)
+    assert abs(E - E_CCD_PSI4) < 1e-10
+    assert hist[-1]["stage"] == 0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mod", ["RCCSD", "mRCCSD"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32_mixed"])
 def test_gpu_rccsd_known_answer(water_psi4, mod, prec):
