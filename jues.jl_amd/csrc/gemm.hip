@@ -935,6 +935,120 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
   }
 }
 
+// FP64 NN C = alpha A B + beta C for a tall, narrow product (N <= 64, M large, e.g. the MP2
+// adjoint's g Co half-transform step, N = o): a 256x(16 IN) tile over eight wavefronts of
+// 32x(16 IN) stacked in m, two workgroups per CU.  A (m-contiguous, the streamed operand) is
+// staged as in gemm64_kernel, B (k-contiguous, at most 64 rows) as gemm64_sk_kernel's B.
+// K must be a multiple of 16.
+template <int IN>
+__global__ __launch_bounds__(512, 4) void gemm64_tall_kernel(GemmSide<double> g, double alpha, double beta) {
+  constexpr int BM = 256, BN = 16 * IN, BK = 16;
+  __shared__ __attribute__((aligned(16))) double As[2][BM * BK];
+  __shared__ __attribute__((aligned(16))) double Bs[2][BN * BK];
+  const int tid = threadIdx.x, lane = tid & 63, wm = tid >> 6;
+  const int M = g.M, N = g.N, K = g.K;
+  const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+  const double* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
+  const double* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
+  double* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
+  int tm, tn;
+  if (!tile_coords_bounded((int)blockIdx.x, (int)gridDim.x, g.tiles_m, g.tiles_n, tm, tn)) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  // A share: row am, k = 8 kq + u (u = 0..7); B share (threads below 8 BN): row bn, 16-byte chunk bc
+  const int am = tid & 255, kq = __builtin_amdgcn_readfirstlane(tid >> 8);
+  const int bn = tid >> 3, bc = tid & 7;
+  const bool b_share = tid < 8 * BN;
+  double ra[8];
+  d2v rb;
+  const int a_voff = 8 * (min(m0 + am, M - 1) - m0);
+  const int b_voff = 8 * (2 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
+  auto load = [&](int k0) {
+    const auto rA = make_rsrc(A + m0 + (long)k0 * lda);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (8 * kq + u) * (int)lda);
+    if (b_share) {
+      const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
+      rb = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff, 0, 0));
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(4 * kq + q, am)]) = d2v{ra[2 * q], ra[2 * q + 1]};
+    if (b_share) *reinterpret_cast<d2v*>(&Bs[buf][bn * BK + 2 * g64_chunk(bc, bn)]) = rb;
+  };
+
+  dvec4 acc[2][IN];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < IN; ++j) acc[i][j] = dvec4{0, 0, 0, 0};
+  const int fr = lane & 15, fk = lane >> 4;
+  const int nk = K / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    // A's two blocks are held for the slice, B's blocks streamed one ahead of their MFMAs (for
+    // IN = 4 read as they are used: the one-ahead copy would not fit in 128 VGPRs)
+    constexpr bool ahead = IN < 4;
+    d2v af[2][2], b0[2], b1[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int r = wm * 32 + i * 16 + fr;
+      af[i][0] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+      af[i][1] = *reinterpret_cast<const d2v*>(&As[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+    }
+    if (ahead) {
+      b0[0] = *reinterpret_cast<const d2v*>(&Bs[cur][fr * BK + 2 * g64_chunk(2 * fk, fr)]);
+      b0[1] = *reinterpret_cast<const d2v*>(&Bs[cur][fr * BK + 2 * g64_chunk(2 * fk + 1, fr)]);
+    }
+#pragma unroll
+    for (int j = 0; j < IN; ++j) {
+      if (!ahead) {
+        const int r = j * 16 + fr;
+        b0[0] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+        b0[1] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+      } else if (j + 1 < IN) {
+        const int r = (j + 1) * 16 + fr;
+        b1[0] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk, r)]);
+        b1[1] = *reinterpret_cast<const d2v*>(&Bs[cur][r * BK + 2 * g64_chunk(2 * fk + 1, r)]);
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0[s2 >> 1][s2 & 1], af[i][s2 >> 1][s2 & 1], acc[i][j],
+                                                           0, 0, 0);
+      if (ahead && j + 1 < IN) { b0[0] = b1[0]; b0[1] = b1[1]; }
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+
+  // acc[i][j][r]: C(m, n) with m = m0 + 32 wm + 16 i + lane % 16, n = n0 + 16 j + lane / 16 + 4 r
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int m = m0 + wm * 32 + i * 16 + fr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < IN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + j * 16 + fk + 4 * r;
+        if (n < N) {
+          const long off = (long)m + (long)n * ldc;
+          double v = alpha * acc[i][j][r];
+          if (beta != 0.0) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+    }
+  }
+}
+
 static inline long tiles_m128(long x) { return (x + 127) / 128; }
 
 // Whether a product can run on the high-occupancy NN / NT kernels: A m-contiguous, 128x128
@@ -1007,7 +1121,12 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   // product under those tags runs on the KIND 0 kernel of its shape)
   // (a skinny product with K <= 128 is bound by streaming op(B) and C: the 128x128 kernel, whose
   // extra MFMA work on the clamped rows costs nothing there, serves it)
-  const int shape = (M <= 64 && N >= 8192 && (ta || K > 128)) ? 1
+  // (a batched short product, M <= 64 with at least 1024 workgroups of 256 columns, such as the
+  // 40 x 216 blocks of the MP2 adjoint's second half-transform, takes the skinny-M tile too: its
+  // 16-row blocks waste less of the MFMA work than a 128-row tile)
+  const bool short_batched = M <= 64 && !ta && !tb && K > 128 && K % 16 == 0 && N >= 128 &&
+                             (long)batch * ((N + 255) / 256) >= 1024;
+  const int shape = (M <= 64 && N >= 8192 && (ta || K > 128)) || short_batched ? 1
                     : (N <= 64 && M >= 8192 && (ta || K > 128)) ? 2
                                                                  : 0;
   const long BM = shape == 1 ? 64 : shape == 2 ? 256 : 128, BN = shape == 1 ? 256 : shape == 2 ? 64 : 128;
@@ -1107,6 +1226,30 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
           hipLaunchKernelGGL(gemm64_sk_kernel<3>, grid, dim3(512), 0, ctx->stream, p, (double)alpha, (double)beta);
         else
           hipLaunchKernelGGL(gemm64_sk_kernel<4>, grid, dim3(512), 0, ctx->stream, p, (double)alpha, (double)beta);
+        JUES_HIP(hipGetLastError());
+      }
+      return;
+    }
+  }
+  // Tall, narrow FP64 NN products (N <= 64): gemm64_tall_kernel, 256x48 or 256x64 tiles (a k-slice
+  // of A spans 16 lda, B's 64 rows 64 ldb, within 32-bit byte offsets)
+  if constexpr (sizeof(T) == 8) {
+    if (shape == 2 && !ta && !tb && K % 16 == 0 && 8L * 16 * (lda + 1) < (1L << 31) &&
+        8L * 64 * (ldb + 1) < (1L << 31)) {
+      KTimer timer(ctx, name, flops, bytes);
+      for (int b0 = 0; b0 < batch; b0 += 65535) {
+        const int nb = std::min(batch - b0, 65535);
+        GemmSide<double> p = make_side<double>(M, N, K, reinterpret_cast<const double*>(A + (long)b0 * sA), lda, sA,
+                                               reinterpret_cast<const double*>(B + (long)b0 * sB), ldb, sB,
+                                               reinterpret_cast<double*>(C + (long)b0 * sC), ldc, sC);
+        const int bn = N <= 48 ? 48 : 64;  // 16-column blocks: 3 or 4
+        p.tiles_m = (int)((M + 255) / 256);
+        p.tiles_n = (int)((N + bn - 1) / bn);
+        const dim3 grid((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb);
+        if (bn == 48)
+          hipLaunchKernelGGL(gemm64_tall_kernel<3>, grid, dim3(512), 0, ctx->stream, p, (double)alpha, (double)beta);
+        else
+          hipLaunchKernelGGL(gemm64_tall_kernel<4>, grid, dim3(512), 0, ctx->stream, p, (double)alpha, (double)beta);
         JUES_HIP(hipGetLastError());
       }
       return;

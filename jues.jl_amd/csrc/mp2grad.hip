@@ -224,47 +224,51 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
     // intermediate (contracting sig with Cv first takes 2n^4 v and n^3 v)
     DevBuf X(n3 * o * sizeof(T));
     gemm<T>(ctx, false, false, n2, o, n, one, g, n2, n3, Co, n, 0, zero, X.as<T>(), n2, n2 * o, (int)n,
-            "mp2grad_half");
+            "mp2grad_half_x");
     gemm<T>(ctx, false, false, n2 * o, v, n, one, X.as<T>(), n2 * o, 0, Cv, n, 0, zero, H12, n2 * o, 0, 1,
-            "mp2grad_half");
+            "mp2grad_half_h12");
   }
   {
     DevBuf Z(o * n3 * sizeof(T));  // Z[i,nu,lam,sig] = sum_mu Co[mu,i] g
-    gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z.as<T>(), o, 0, 1, "mp2grad_half");
+    gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z.as<T>(), o, 0, 1, "mp2grad_half_z");
     gemm<T>(ctx, false, false, o, v, n, one, Z.as<T>(), o, (long)o * n, Cv, n, 0, zero, H34, o, ov, (int)n2,
-            "mp2grad_half");
+            "mp2grad_half_h34");
   }
   T* dCo = dC;           // n x o  (columns 0..o-1 of dC, leading dimension n)
   T* dCv = dC + n * o;   // n x v
   // slot 1: Gt[i,nu,jb] = G[i,a,jb] Cv[nu,a]^T ; dC1 = H12 (n x n ov) . Gt^T  (Gt is also the first
   // step of the dE/dg back-transform)
   DevBuf Gt((long)o * n * ov * sizeof(T));
-  gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, (long)o * n, (int)ov, "mp2grad");
-  gemm<T>(ctx, false, true, n, o, n * ov, one, H12, n, 0, Gt.as<T>(), o, 0, zero, dCo, n, 0, 1, "mp2grad");
+  gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, (long)o * n, (int)ov, "mp2grad_s1");
+  gemm<T>(ctx, false, true, n, o, n * ov, one, H12, n, 0, Gt.as<T>(), o, 0, zero, dCo, n, 0, 1, "mp2grad_s1");
   // slot 2: Gh[mu,a,jb] = Co . G ; dC2[nu,a] = sum_{mu,jb} H12[mu,nu,jb] Gh[mu,a,jb]
   {
     DevBuf Gh(n * v * ov * sizeof(T)), H12p(n2 * ov * sizeof(T)), Ghp(n * v * ov * sizeof(T));
-    gemm<T>(ctx, false, false, n, v * ov, o, one, Co, n, 0, G, o, 0, zero, Gh.as<T>(), n, 0, 1, "mp2grad");
+    gemm<T>(ctx, false, false, n, v * ov, o, one, Co, n, 0, G, o, 0, zero, Gh.as<T>(), n, 0, 1, "mp2grad_s2");
     permute<T, T>(ctx, H12, {n, n, o, v}, {1, 0, 2, 3}, H12p.as<T>(), 1.0, 0.0, "mp2grad_perm");
     permute<T, T>(ctx, Gh.as<T>(), {n, v, o, v}, {1, 0, 2, 3}, Ghp.as<T>(), 1.0, 0.0, "mp2grad_perm");
     gemm<T>(ctx, false, true, n, v, n * ov, one, H12p.as<T>(), n, 0, Ghp.as<T>(), v, 0, zero, dCv, n, 0, 1,
-            "mp2grad");
+            "mp2grad_s2");
   }
   // slot 3: G3[ia,j,sig] = G[iaj,b] Cv[sig,b] ; dC3[lam,j] = sum_{ia,sig} H34[ia,lam,sig] G3[ia,j,sig]
   {
     DevBuf G3(ov * o * n * sizeof(T)), H34p(n2 * ov * sizeof(T)), G3p(ov * o * n * sizeof(T));
-    gemm<T>(ctx, false, true, ov * o, n, v, one, G, ov * o, 0, Cv, n, 0, zero, G3.as<T>(), ov * o, 0, 1, "mp2grad");
+    gemm<T>(ctx, false, true, ov * o, n, v, one, G, ov * o, 0, Cv, n, 0, zero, G3.as<T>(), ov * o, 0, 1, "mp2grad_s3");
     permute<T, T>(ctx, H34, {ov, n, n}, {0, 2, 1}, H34p.as<T>(), 1.0, 0.0, "mp2grad_perm");
     permute<T, T>(ctx, G3.as<T>(), {ov, o, n}, {0, 2, 1}, G3p.as<T>(), 1.0, 0.0, "mp2grad_perm");
     gemm<T>(ctx, true, false, n, o, ov * n, one, H34p.as<T>(), ov * n, 0, G3p.as<T>(), ov * n, 0, one, dCo, n, 0, 1,
-            "mp2grad");
+            "mp2grad_s3");
   }
-  // slot 4: G4[ia,lam,b] = G[ia,j,b] Co[lam,j] ; dC4[sig,b] = sum_{ia,lam} H34[ia,lam,sig] G4[ia,lam,b]
+  // slot 4: dC4[sig,b] = sum_{ia,lam,j} H34[ia,lam,sig] Co[lam,j] G[ia,j,b], Co contracted into H34
+  // first (2n^2 o^2 v + 2n o^2 v^2 flops; contracting it into G first takes 2n o^2 v^2 + 2n^2 o v^2):
+  //   M4[ia,j,sig] = sum_lam H34[ia,lam,sig] Co[lam,j]   (batched over sig)
+  //   dC4[sig,b]  += sum_{(ia,j)} M4[(ia,j),sig] G[(ia,j),b]
   {
-    DevBuf G4(ov * n * v * sizeof(T));
-    gemm<T>(ctx, false, true, ov, n, o, one, G, ov, ov * o, Co, n, 0, zero, G4.as<T>(), ov, ov * n, v, "mp2grad");
-    gemm<T>(ctx, true, false, n, v, ov * n, one, H34, ov * n, 0, G4.as<T>(), ov * n, 0, one, dCv, n, 0, 1,
-            "mp2grad");
+    DevBuf M4(ov * o * n * sizeof(T));
+    gemm<T>(ctx, false, false, ov, o, n, one, H34, ov, ov * n, Co, n, 0, zero, M4.as<T>(), ov, ov * o, (int)n,
+            "mp2grad_s4");
+    gemm<T>(ctx, true, false, n, v, ov * o, one, M4.as<T>(), ov * o, 0, G, ov * o, 0, one, dCv, n, 0, 1,
+            "mp2grad_s4");
   }
   H12b.release();
   H34b.release();
@@ -278,13 +282,13 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
     const long on = (long)o * n;
     DevBuf P2(on * n * v * sizeof(T));
     gemm<T>(ctx, false, true, on, n, o, one, Gt.as<T>(), on, on * o, Co, n, 0, zero, P2.as<T>(), on, on * n, v,
-            "mp2grad_back");
+            "mp2grad_back_p2");
     Gt.release();
     DevBuf Q(on * n2 * sizeof(T));
     gemm<T>(ctx, false, true, on * n, n, v, one, P2.as<T>(), on * n, 0, Cv, n, 0, zero, Q.as<T>(), on * n, 0, 1,
-            "mp2grad_back");
+            "mp2grad_back_q");
     P2.release();
-    gemm<T>(ctx, false, false, n, n3, o, one, Co, n, 0, Q.as<T>(), o, 0, zero, dg, n, 0, 1, "mp2grad_back");
+    gemm<T>(ctx, false, false, n, n3, o, one, Co, n, 0, Q.as<T>(), o, 0, zero, dg, n, 0, 1, "mp2grad_back_dg");
   }
 }
 

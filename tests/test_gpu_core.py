@@ -122,7 +122,10 @@ def test_rmp2(cfg1, dense):
                                    (9000, 40, 256), (257, 300, 17), (640, 520, 4111), (130, 129, 31),
                                    (2048, 2048, 256), (2100, 2300, 1041), (4099, 2049, 513),
                                    # the packed ladder's M = o(o+1)/2 at o = 60: a last tile row of 38 rows
-                                   (1830, 8200, 4100)])
+                                   (1830, 8200, 4100),
+                                   # tall, narrow FP64 NN products (256x48 and 256x64 tiles): ragged last tile
+                                   # row, full-width tiles, one live column in the last 16-column block
+                                   (8300, 60, 272), (8192, 48, 160), (8193, 49, 144), (10000, 64, 256)])
 def test_gemm_nn_nt_occupancy_kernels(ctx, M, N, K, tb, prec, tol):
     """Products with A m-contiguous and B k- or n-contiguous on the high-occupancy kernels
     (FP32: four 256-thread workgroups per CU; FP64: two 512-thread ones): full 16-column
