@@ -1121,11 +1121,11 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   // product under those tags runs on the KIND 0 kernel of its shape)
   // (a skinny product with K <= 128 is bound by streaming op(B) and C: the 128x128 kernel, whose
   // extra MFMA work on the clamped rows costs nothing there, serves it)
-  // (a batched short product, M <= 64 with at least 1024 workgroups of 256 columns, such as the
-  // 40 x 216 blocks of the MP2 adjoint's second half-transform, takes the skinny-M tile too: its
-  // 16-row blocks waste less of the MFMA work than a 128-row tile)
+  // (a batched short product, M <= 64 with at least 1024 workgroups of 256 columns -- 256 when K is
+  // long -- such as the 40 x 216 blocks of the MP2 adjoint's second half-transform, takes the
+  // skinny-M tile too: its 16-row blocks waste less of the MFMA work than a 128-row tile)
   const bool short_batched = M <= 64 && !ta && !tb && K > 128 && K % 16 == 0 && N >= 128 &&
-                             (long)batch * ((N + 255) / 256) >= 1024;
+                             (long)batch * ((N + 255) / 256) >= (K >= 2048 ? 256 : 1024);
   const int shape = (M <= 64 && N >= 8192 && (ta || K > 128)) || short_batched ? 1
                     : (N <= 64 && M >= 8192 && (ta || K > 128)) ? 2
                                                                  : 0;

@@ -241,23 +241,39 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
   DevBuf Gt((long)o * n * ov * sizeof(T));
   gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, (long)o * n, (int)ov, "mp2grad_s1");
   gemm<T>(ctx, false, true, n, o, n * ov, one, H12, n, 0, Gt.as<T>(), o, 0, zero, dCo, n, 0, 1, "mp2grad_s1");
-  // slot 2: Gh[mu,a,jb] = Co . G ; dC2[nu,a] = sum_{mu,jb} H12[mu,nu,jb] Gh[mu,a,jb]
+  // slot 2: dC2[nu,a] = sum_{mu,i,jb} H12[mu,nu,jb] Co[mu,i] G[i,a,jb], Co contracted into H12 first
+  // (2n^2 o^2 v + 2n o^2 v^2 flops, no permutation of H12):
+  //   M2[(i,jb),nu] = sum_mu Co^T[i,mu] H12[mu,nu,jb]   (batched over nu)
+  //   dC2[nu,a]     = sum_{(i,jb)} M2[(i,jb),nu] Gp[(i,jb),a],  Gp[i,jb,a] = G[i,a,jb]
   {
-    DevBuf Gh(n * v * ov * sizeof(T)), H12p(n2 * ov * sizeof(T)), Ghp(n * v * ov * sizeof(T));
-    gemm<T>(ctx, false, false, n, v * ov, o, one, Co, n, 0, G, o, 0, zero, Gh.as<T>(), n, 0, 1, "mp2grad_s2");
-    permute<T, T>(ctx, H12, {n, n, o, v}, {1, 0, 2, 3}, H12p.as<T>(), 1.0, 0.0, "mp2grad_perm");
-    permute<T, T>(ctx, Gh.as<T>(), {n, v, o, v}, {1, 0, 2, 3}, Ghp.as<T>(), 1.0, 0.0, "mp2grad_perm");
-    gemm<T>(ctx, false, true, n, v, n * ov, one, H12p.as<T>(), n, 0, Ghp.as<T>(), v, 0, zero, dCv, n, 0, 1,
+    DevBuf CoT((long)o * n * sizeof(T)), M2(ov * o * n * sizeof(T)), Gp(N * sizeof(T));
+    permute<T, T>(ctx, Co, {n, (long)o}, {1, 0}, CoT.as<T>(), 1.0, 0.0, "mp2grad_s2");
+    permute<T, T>(ctx, G, {(long)o, (long)v, ov}, {0, 2, 1}, Gp.as<T>(), 1.0, 0.0, "mp2grad_s2");
+    gemm<T>(ctx, false, false, o, ov, n, one, CoT.as<T>(), o, 0, H12, n2, n, zero, M2.as<T>(), o, o * ov, (int)n,
+            "mp2grad_s2");
+    gemm<T>(ctx, true, false, n, v, o * ov, one, M2.as<T>(), o * ov, 0, Gp.as<T>(), o * ov, 0, zero, dCv, n, 0, 1,
             "mp2grad_s2");
   }
-  // slot 3: G3[ia,j,sig] = G[iaj,b] Cv[sig,b] ; dC3[lam,j] = sum_{ia,sig} H34[ia,lam,sig] G3[ia,j,sig]
+  // slot 3: dC3[lam,j] = sum_{ia,sig} H34[ia,lam,sig] G3[ia,j,sig], G3[ia,j,sig] = sum_b G[ia,j,b] Cv[sig,b];
+  // with G3 laid out j first, the products batched over sig are NN and read H34 in place (no
+  // permutation of H34):
+  //   Gq[j,ia,b]       = G[ia,j,b]
+  //   G3q[(j,ia),sig]  = sum_b Gq[(j,ia),b] Cv[sig,b]
+  //   W[j,lam,sig]     = sum_ia G3q[j,ia,sig] H34[ia,lam,sig]     (batched over sig)
+  //   dC3[lam,j]      += sum_sig W[j,lam,sig]                      (W times a vector of ones)
   {
-    DevBuf G3(ov * o * n * sizeof(T)), H34p(n2 * ov * sizeof(T)), G3p(ov * o * n * sizeof(T));
-    gemm<T>(ctx, false, true, ov * o, n, v, one, G, ov * o, 0, Cv, n, 0, zero, G3.as<T>(), ov * o, 0, 1, "mp2grad_s3");
-    permute<T, T>(ctx, H34, {ov, n, n}, {0, 2, 1}, H34p.as<T>(), 1.0, 0.0, "mp2grad_perm");
-    permute<T, T>(ctx, G3.as<T>(), {ov, o, n}, {0, 2, 1}, G3p.as<T>(), 1.0, 0.0, "mp2grad_perm");
-    gemm<T>(ctx, true, false, n, o, ov * n, one, H34p.as<T>(), ov * n, 0, G3p.as<T>(), ov * n, 0, one, dCo, n, 0, 1,
+    DevBuf Gq(N * sizeof(T)), G3q(ov * o * n * sizeof(T)), W((long)o * n2 * sizeof(T)), ones(n * sizeof(T)),
+        dC3t((long)o * n * sizeof(T));
+    const long on = (long)o * n;
+    permute<T, T>(ctx, G, {ov, (long)o, (long)v}, {1, 0, 2}, Gq.as<T>(), 1.0, 0.0, "mp2grad_s3");
+    gemm<T>(ctx, false, true, o * ov, n, v, one, Gq.as<T>(), o * ov, 0, Cv, n, 0, zero, G3q.as<T>(), o * ov, 0, 1,
             "mp2grad_s3");
+    gemm<T>(ctx, false, false, o, n, ov, one, G3q.as<T>(), o, o * ov, H34, ov, ov * n, zero, W.as<T>(), o, on, (int)n,
+            "mp2grad_s3");
+    fill<T>(ctx, ones.as<T>(), n, 1.0);
+    gemm<T>(ctx, false, false, on, 1, n, one, W.as<T>(), on, 0, ones.as<T>(), n, 0, zero, dC3t.as<T>(), on, 0, 1,
+            "mp2grad_s3");
+    permute<T, T>(ctx, dC3t.as<T>(), {(long)o, n}, {1, 0}, dCo, 1.0, 1.0, "mp2grad_s3");
   }
   // slot 4: dC4[sig,b] = sum_{ia,lam,j} H34[ia,lam,sig] Co[lam,j] G[ia,j,b], Co contracted into H34
   // first (2n^2 o^2 v + 2n o^2 v^2 flops; contracting it into G first takes 2n o^2 v^2 + 2n^2 o v^2):
