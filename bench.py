@@ -293,12 +293,11 @@ def mp2_bench() -> dict:
     timed("transform_iajb_fp32", lambda: Transformation.tei_transform(g, Co, Cv, Co, Cv, prec="fp32"), F_T, 157.3)
     E64 = timed("rmp2_fp64", lambda: MollerPlesset.do_rmp2(w), F_T, 78.6)
     E32 = timed("rmp2_fp32_mixed", lambda: MollerPlesset.do_rmp2(w, prec="fp32_mixed"), F_T, 157.3)
-    # adjoint as executed: forward (ia|jb) (F_T); the half-transforms H12, H34, each an AO index contracted
-    # with Co first (2 (2n^4 o + 2n^3 o v)); the four dE/dC slots' products with H12 / H34, 2n^2 o^2 v
-    # each (slots 2 and 4 contract Co into H12 / H34 first), and their G contractions (8 n o^2 v^2); the
+    # adjoint as executed: forward (ia|jb) with its (i nu|lam b) and (i nu|j b) intermediates kept (F_T);
+    # the half-transform (mu a|j sig) (2n^4 o + 2n^3 o v); the four dE/dC positions: three products of
+    # 2n^2 o^2 v (positions 1 and 3, the occupied quarter of position 4) and four of 2n o^2 v^2; the
     # back-transform to dE/dg, i contracted last (2n^4 o + 2n^3 o v + 2n^2 o^2 v)
-    F_G = (F_T + 6 * n**4 * o + 6 * n**3 * o * v + 8 * n**2 * o**2 * v + 2 * n**2 * o**2 * v
-           + 8 * n * o**2 * v**2)
+    F_G = F_T + 4 * n**4 * o + 4 * n**3 * o * v + 8 * n**2 * o**2 * v + 8 * n * o**2 * v**2
     res = timed("rmp2_grad_fp64_with_dEdg", lambda: MollerPlesset.rmp2_grad(w), F_G, 78.6)
     out["E_mp2_fp64"] = E64
     out["E_mp2_fp32_mixed_abs_diff"] = abs(E32 - E64)

@@ -311,7 +311,7 @@ __device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
 
 // Workgroups [0, nt0) run product p0 (nt0 padded up to a multiple of NXCD when a second
 // product follows), the rest p1.
-template <int KIND, bool TB>
+template <int KIND, bool TB, bool TA = false>
 __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, GemmSide<float> p1, int nt0,
                                                            float alpha, float beta) {
   constexpr int BM = 128, BN = 128, BK = 16;
@@ -340,6 +340,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   const int am = tid & 127, ac0 = __builtin_amdgcn_readfirstlane(tid >> 7);
   const int bn = tid >> 2, bc = tid & 3;
   float ra[8];
+  f4v rat[2];  // TA (A k-contiguous): shared like a k-contiguous B, rows bn and bn + 64, chunk bc
   f4v rb[2];
   // rows past the edge of M or N load the last row instead (their results are not stored).
   // B k-contiguous (TB false): 16-byte loads of 4 k; B n-contiguous (TB true, op(B) = B^T):
@@ -348,13 +349,26 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   const int bt_voff = 4 * (min(n0 + am, N - 1) - n0);
   const int b_voff0 = 4 * (4 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
   const int b_voff1 = 4 * (4 * bc + (min(n0 + bn + 64, N - 1) - n0) * (int)ldb);
+  // TA: k-contiguous rows addressed from two bases, rows m0.. and m1 = min(m0 + 64, M - 1).., so that a
+  // byte offset spans at most 63 rows (the same for B in the TA instantiations)
+  const int m1 = min(m0 + 64, M - 1), n1 = min(n0 + 64, N - 1);
+  const int at_voff0 = 4 * (4 * bc + (min(m0 + bn, M - 1) - m0) * (int)lda);
+  const int at_voff1 = 4 * (4 * bc + (min(m0 + bn + 64, M - 1) - m1) * (int)lda);
+  const int b2_voff1 = 4 * (4 * bc + (min(n0 + bn + 64, N - 1) - n1) * (int)ldb);
   auto load_seg = [&](const float* Ab, const float* Bb, int k0) {
-    const auto rA = make_rsrc(Ab + m0 + (long)k0 * lda);
+    if constexpr (TA) {
+      const auto rA0 = make_rsrc(Ab + k0 + (long)m0 * lda);
+      const auto rA1 = make_rsrc(Ab + k0 + (long)m1 * lda);
+      rat[0] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rA0, at_voff0, 0, 0));
+      rat[1] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rA1, at_voff1, 0, 0));
+    } else {
+      const auto rA = make_rsrc(Ab + m0 + (long)k0 * lda);
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        ra[4 * j + u] = bload<float>(rA, a_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)lda);
+        for (int u = 0; u < 4; ++u)
+          ra[4 * j + u] = bload<float>(rA, a_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)lda);
+    }
     if constexpr (TB) {
       const auto rB = make_rsrc(Bb + n0 + (long)k0 * ldb);
 #pragma unroll
@@ -362,6 +376,11 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           rb[j][u] = bload<float>(rB, bt_voff, 4 * (4 * (ac0 + 2 * j) + u) * (int)ldb);
+    } else if constexpr (TA) {
+      const auto rB0 = make_rsrc(Bb + k0 + (long)n0 * ldb);
+      const auto rB1 = make_rsrc(Bb + k0 + (long)n1 * ldb);
+      rb[0] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB0, b_voff0, 0, 0));
+      rb[1] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB1, b2_voff1, 0, 0));
     } else {
       const auto rB = make_rsrc(Bb + k0 + (long)n0 * ldb);
       rb[0] = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
@@ -377,9 +396,14 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   auto store = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int c = ac0 + 2 * j;
-      *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(c, am)]) =
-          f4v{ra[4 * j], ra[4 * j + 1], ra[4 * j + 2], ra[4 * j + 3]};
+      if constexpr (TA) {
+        const int m = bn + 64 * j;
+        *reinterpret_cast<f4v*>(&As[buf][m * BK + 4 * g32_chunk(bc, m)]) = rat[j];
+      } else {
+        const int c = ac0 + 2 * j;
+        *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(c, am)]) =
+            f4v{ra[4 * j], ra[4 * j + 1], ra[4 * j + 2], ra[4 * j + 3]};
+      }
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -405,15 +429,27 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   const int K16 = K / BK * BK, tail = K > K16 ? 1 : 0;
   const int nk = K16 / BK + tail + K1 / BK;
   if (tail) {
-    const float* Ar = A + m0 + (a_voff >> 2);
+    if constexpr (TA) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int gk = K16 + 4 * (ac0 + 2 * j) + u;
-        const float val = Ar[(long)min(gk, K - 1) * lda];
-        ra[4 * j + u] = gk < K ? val : 0.f;
-      }
+        for (int u = 0; u < 4; ++u) {
+          const int gk = K16 + 4 * bc + u;
+          const int m = min(m0 + bn + 64 * j, M - 1);
+          const float val = A[(long)min(gk, K - 1) + (long)m * lda];
+          rat[j][u] = gk < K ? val : 0.f;
+        }
+    } else {
+      const float* Ar = A + m0 + (a_voff >> 2);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int gk = K16 + 4 * (ac0 + 2 * j) + u;
+          const float val = Ar[(long)min(gk, K - 1) * lda];
+          ra[4 * j + u] = gk < K ? val : 0.f;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -513,7 +549,7 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 
 __device__ inline int g64_chunk(int q, int r) { return q ^ (r & 7); }
 
-template <int KIND, bool TB>
+template <int KIND, bool TB, bool TA = false>
 __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, GemmSide<double> p1, int nt0,
                                                         double alpha, double beta) {
   constexpr int BM = 128, BN = 128, BK = 16;
@@ -537,22 +573,42 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
     return;
   const int m0 = tm * BM, n0 = tn * BN;
   // A (and n-contiguous B) share: row am, k = 4 kq + u; k-contiguous B share: rows bn, bn + 64, chunk bc
+  // (a k-contiguous A, TA, is shared like a k-contiguous B)
   const int am = tid & 127, kq = __builtin_amdgcn_readfirstlane(tid >> 7);
   const int bn = tid >> 3, bc = tid & 7;
   double ra[4];
+  d2v rat[2];
   d2v rb[2];
   const int a_voff = 8 * (min(m0 + am, M - 1) - m0);
   const int bt_voff = 8 * (min(n0 + am, N - 1) - n0);
   const int b_voff0 = 8 * (2 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
   const int b_voff1 = 8 * (2 * bc + (min(n0 + bn + 64, N - 1) - n0) * (int)ldb);
+  // TA: k-contiguous rows addressed from two bases, rows m0.. and m1 = min(m0 + 64, M - 1).., so that a
+  // byte offset spans at most 63 rows (the same for B in the TA instantiations)
+  const int m1 = min(m0 + 64, M - 1), n1 = min(n0 + 64, N - 1);
+  const int at_voff0 = 8 * (2 * bc + (min(m0 + bn, M - 1) - m0) * (int)lda);
+  const int at_voff1 = 8 * (2 * bc + (min(m0 + bn + 64, M - 1) - m1) * (int)lda);
+  const int b2_voff1 = 8 * (2 * bc + (min(n0 + bn + 64, N - 1) - n1) * (int)ldb);
   auto load_seg = [&](const double* Ab, const double* Bb, int k0) {
-    const auto rA = make_rsrc(Ab + m0 + (long)k0 * lda);
+    if constexpr (TA) {
+      const auto rA0 = make_rsrc(Ab + k0 + (long)m0 * lda);
+      const auto rA1 = make_rsrc(Ab + k0 + (long)m1 * lda);
+      rat[0] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rA0, at_voff0, 0, 0));
+      rat[1] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rA1, at_voff1, 0, 0));
+    } else {
+      const auto rA = make_rsrc(Ab + m0 + (long)k0 * lda);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (4 * kq + u) * (int)lda);
+      for (int u = 0; u < 4; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (4 * kq + u) * (int)lda);
+    }
     if constexpr (TB) {
       const auto rB = make_rsrc(Bb + n0 + (long)k0 * ldb);
 #pragma unroll
       for (int u = 0; u < 4; ++u) rb[u >> 1][u & 1] = bload<double>(rB, bt_voff, 8 * (4 * kq + u) * (int)ldb);
+    } else if constexpr (TA) {
+      const auto rB0 = make_rsrc(Bb + k0 + (long)n0 * ldb);
+      const auto rB1 = make_rsrc(Bb + k0 + (long)n1 * ldb);
+      rb[0] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB0, b_voff0, 0, 0));
+      rb[1] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB1, b2_voff1, 0, 0));
     } else {
       const auto rB = make_rsrc(Bb + k0 + (long)n0 * ldb);
       rb[0] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff0, 0, 0));
@@ -566,8 +622,16 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
     else load_seg(A1, B1, (s - nk0) * BK);
   };
   auto store = [&](int buf) {
-    *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(2 * kq, am)]) = d2v{ra[0], ra[1]};
-    *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(2 * kq + 1, am)]) = d2v{ra[2], ra[3]};
+    if constexpr (TA) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int m = bn + 64 * j;
+        *reinterpret_cast<d2v*>(&As[buf][m * BK + 2 * g64_chunk(bc, m)]) = rat[j];
+      }
+    } else {
+      *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(2 * kq, am)]) = d2v{ra[0], ra[1]};
+      *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(2 * kq + 1, am)]) = d2v{ra[2], ra[3]};
+    }
     if constexpr (TB) {
       *reinterpret_cast<d2v*>(&Bs[buf][am * BK + 2 * g64_chunk(2 * kq, am)]) = rb[0];
       *reinterpret_cast<d2v*>(&Bs[buf][am * BK + 2 * g64_chunk(2 * kq + 1, am)]) = rb[1];
@@ -590,12 +654,24 @@ __global__ __launch_bounds__(512, 4) void gemm64_kernel(GemmSide<double> p0, Gem
   const int K16 = K / BK * BK, tail = K > K16 ? 1 : 0;
   const int nk = K16 / BK + tail + K1 / BK;
   if (tail) {
-    const double* Ar = A + m0 + (a_voff >> 3);
+    if constexpr (TA) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int gk = K16 + 4 * kq + u;
-      const double val = Ar[(long)min(gk, K - 1) * lda];
-      ra[u] = gk < K ? val : 0.0;
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int gk = K16 + 2 * bc + u;
+          const int m = min(m0 + bn + 64 * j, M - 1);
+          const double val = A[(long)min(gk, K - 1) + (long)m * lda];
+          rat[j][u] = gk < K ? val : 0.0;
+        }
+    } else {
+      const double* Ar = A + m0 + (a_voff >> 3);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gk = K16 + 4 * kq + u;
+        const double val = Ar[(long)min(gk, K - 1) * lda];
+        ra[u] = gk < K ? val : 0.0;
+      }
     }
     if constexpr (TB) {
 #pragma unroll
@@ -1051,11 +1127,15 @@ __global__ __launch_bounds__(512, 4) void gemm64_tall_kernel(GemmSide<double> g,
 
 static inline long tiles_m128(long x) { return (x + 127) / 128; }
 
-// Whether a product can run on the high-occupancy NN / NT kernels: A m-contiguous, 128x128
-// tiles, at least one full k-slice, and a tile's k-slice addressable with 32-bit offsets.
+// Whether a product can run on the high-occupancy kernels: 128x128 tiles, at least one full
+// k-slice, and a tile's k-slice addressable with 32-bit offsets (A m-contiguous: 128 rows and 16
+// columns of lda; a k-contiguous A, TN / TT, and its B in those instantiations: 64 rows of lda / ldb
+// from each of two bases).
 template <typename T>
 static bool occ_kernel_ok(bool ta, long K, long lda, long ldb) {
-  return !ta && K >= 16 && (long)sizeof(T) * (128 + 64) * (std::max(lda, ldb) + 1) < (1L << 31);
+  if (K < 16) return false;
+  if (ta) return (long)sizeof(T) * 64 * (std::max(lda, ldb) + 1) < (1L << 31);
+  return (long)sizeof(T) * (128 + 64) * (std::max(lda, ldb) + 1) < (1L << 31);
 }
 
 template <typename T>
@@ -1068,7 +1148,7 @@ static GemmSide<T> make_side(long M, long N, long K, const T* A, long lda, long 
 // One launch of the high-occupancy kernel over p0 (and p1 when `two`), nb batch entries.
 template <typename T>
 static void occ_launch(jues_ctx* ctx, bool tb, int kind, GemmSide<T> p0, GemmSide<T> p1, bool two, int nb, T alpha,
-                       T beta) {
+                       T beta, bool ta = false) {
   const long t0 = (long)p0.tiles_m * p0.tiles_n;
   const long nt0 = two ? (t0 + NXCD - 1) / NXCD * NXCD : t0;
   const long total = two ? nt0 + (long)p1.tiles_m * p1.tiles_n : t0;
@@ -1076,22 +1156,28 @@ static void occ_launch(jues_ctx* ctx, bool tb, int kind, GemmSide<T> p0, GemmSid
   if (!two) p1 = p0;
   const dim3 grid((unsigned)total, 1, nb);
   if constexpr (sizeof(T) == 4) {
-#define JUES_GEMM32(K_, TB_)                                                                                 \
-  hipLaunchKernelGGL((gemm32_nn_kernel<K_, TB_>), grid, dim3(NT), 0, ctx->stream, p0, p1, (int)nt0, (float)alpha, \
-                     (float)beta)
-    if (tb) JUES_GEMM32(0, true);
-    else if (kind == 1) JUES_GEMM32(1, false);
-    else if (kind == 2) JUES_GEMM32(2, false);
-    else JUES_GEMM32(0, false);
+#define JUES_GEMM32(K_, TB_, TA_)                                                                    \
+  hipLaunchKernelGGL((gemm32_nn_kernel<K_, TB_, TA_>), grid, dim3(NT), 0, ctx->stream, p0, p1, (int)nt0,    \
+                     (float)alpha, (float)beta)
+    if (ta) {
+      if (tb) JUES_GEMM32(0, true, true);
+      else JUES_GEMM32(0, false, true);
+    } else if (tb) JUES_GEMM32(0, true, false);
+    else if (kind == 1) JUES_GEMM32(1, false, false);
+    else if (kind == 2) JUES_GEMM32(2, false, false);
+    else JUES_GEMM32(0, false, false);
 #undef JUES_GEMM32
   } else {
-#define JUES_GEMM64(K_, TB_)                                                                               \
-  hipLaunchKernelGGL((gemm64_kernel<K_, TB_>), grid, dim3(512), 0, ctx->stream, p0, p1, (int)nt0, (double)alpha, \
-                     (double)beta)
-    if (tb) JUES_GEMM64(0, true);
-    else if (kind == 1) JUES_GEMM64(1, false);
-    else if (kind == 2) JUES_GEMM64(2, false);
-    else JUES_GEMM64(0, false);
+#define JUES_GEMM64(K_, TB_, TA_)                                                                  \
+  hipLaunchKernelGGL((gemm64_kernel<K_, TB_, TA_>), grid, dim3(512), 0, ctx->stream, p0, p1, (int)nt0,    \
+                     (double)alpha, (double)beta)
+    if (ta) {
+      if (tb) JUES_GEMM64(0, true, true);
+      else JUES_GEMM64(0, false, true);
+    } else if (tb) JUES_GEMM64(0, true, false);
+    else if (kind == 1) JUES_GEMM64(1, false, false);
+    else if (kind == 2) JUES_GEMM64(2, false, false);
+    else JUES_GEMM64(0, false, false);
 #undef JUES_GEMM64
   }
   JUES_HIP(hipGetLastError());
@@ -1164,7 +1250,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
       KTimer t1(ctx, name, 2.0 * M * N * K, (double)sizeof(T) * ((double)M * K + (double)K * N + (double)S * M * N));
       GemmSide<T> p = make_side<T>(M, N, kc, A, lda, sa, B, ldb, sb, Wk, M, M * N);
       p.Klast = (int)klast;
-      occ_launch<T>(ctx, tb, kind, p, p, false, S, alpha, T(0));
+      occ_launch<T>(ctx, tb, kind, p, p, false, S, alpha, T(0), ta);
     } else {
       if (S > 1)
         gemm_impl<T>(ctx, ta, tb, M, N, kc, alpha, A, lda, sa, B, ldb, sb, T(0), Wk, M, M * N, S - 1, tag, false);
@@ -1263,7 +1349,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
       const int nb = std::min(batch - b0, 65535);
       const GemmSide<T> p = make_side<T>(M, N, K, A + (long)b0 * sA, lda, sA, B + (long)b0 * sB, ldb, sB,
                                          C + (long)b0 * sC, ldc, sC);
-      occ_launch<T>(ctx, tb, kind, p, p, false, nb, alpha, beta);
+      occ_launch<T>(ctx, tb, kind, p, p, false, nb, alpha, beta, ta);
     }
     return;
   }

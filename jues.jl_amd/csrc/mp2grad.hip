@@ -4,14 +4,13 @@
 // the block of Transformation.jl:39-93).  With every entry of C and g
 // independent, G_iajb = dE/dK_iajb = 2 (2 K_iajb - K_ibja) / D_ijab and
 //   dE/dg[mu,nu,lam,sig] = sum G_iajb Co[mu,i] Cv[nu,a] Co[lam,j] Cv[sig,b],
-//   dE/dC = [dC1 + dC3 | dC2 + dC4] for the four slots C enters:
-//     dC1[mu,i]  = sum H12[mu,nu,j,b] Gt[i,nu,j,b],   Gt  = sum_a Cv[nu,a] G[i,a,j,b]
-//     dC2[nu,a]  = sum H12[mu,nu,j,b] Gh[mu,a,j,b],   Gh  = sum_i Co[mu,i] G[i,a,j,b]
-//     dC3[lam,j] = sum H34[i,a,lam,sig] G3[i,a,j,sig], G3 = sum_b Cv[sig,b] G[i,a,j,b]
-//     dC4[sig,b] = sum H34[i,a,lam,sig] G4[i,a,lam,b], G4 = sum_j Co[lam,j] G[i,a,j,b]
-//   with the half-transformed integrals recomputed (never stored between calls):
-//     H12[mu,nu,j,b] = sum g[mu,nu,lam,sig] Co[lam,j] Cv[sig,b]
-//     H34[i,a,lam,sig] = sum g[mu,nu,lam,sig] Co[mu,i] Cv[nu,a].
+//   dE/dC = [dC1 + dC3 | dC2 + dC4] for the four positions C enters, each g with the other
+//   three positions transformed, contracted with G:
+//     dC1[mu,i]  = sum (mu a|j sig) G[i,a,j,b] Cv[sig,b]     dC2[nu,a]  = sum (i nu|j b) G[i,a,j,b]
+//     dC3[lam,j] = sum (i nu|lam b) G[i,a,j,b] Cv[nu,a]      dC4[sig,b] = sum (i a|j sig) G[i,a,j,b]
+//   (i nu|lam b) and (i nu|j b) are intermediates of the forward (ia|jb) transform, kept for
+//   this; (mu a|j sig) is one more half-transform and (i a|j sig) its occupied quarter
+//   (mp2_grad_dev).  Nothing is stored between calls.
 // No symmetry of g is assumed.
 //
 // Factor route ((mu nu|lam sig) = sum_P B[mu,nu,P] B[lam,sig,P]): K = L L^T with
@@ -200,102 +199,101 @@ void mp2_grad_factors_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v,
 }
 
 // Dense route of the adjoint (the AO tensor g is on the device); dg may be null.
+//
+// E depends on C at four positions of (ia|jb) = sum g[mu,nu,lam,sig] Co[mu,i] Cv[nu,a] Co[lam,j]
+// Cv[sig,b].  The forward transform runs here with two of its intermediates kept:
+//   Z1[i,nu,lam,sig] = sum_mu Co[mu,i] g            X2[i,nu,lam,b] = sum_sig Z1 Cv[sig,b]
+//   X3[nu,i,b,lam]   = X2 permuted                  Y3[nu,i,b,j]   = sum_lam X3 Co[lam,j]
+//   K[i,a,j,b] = (X5[a,i,b,j] = sum_nu Cv[nu,a] Y3) permuted
+// X3 (positions 1 and 4 transformed) gives dE/dC at position 3 and Y3 (positions 1, 3, 4) at
+// position 2; positions 1 and 4 take one more half-transform, Y[mu,a,j,sig] = (mu a|j sig):
+//   pos 1: dCo[mu,i]  += sum_{a,j,sig} Y[mu,a,j,sig] T1[i,a,j,sig],  T1 = sum_b G[i,a,j,b] Cv[sig,b]
+//   pos 2: dCv[nu,a]   = sum_{i,b,j} Y3[nu,i,b,j] G[i,a,j,b]
+//   pos 3: dCo[lam,j]  = sum_{nu,i,b} X3[nu,i,b,lam] Gt[i,nu,j,b],  Gt = sum_a G[i,a,j,b] Cv[nu,a]
+//   pos 4: dCv[sig,b] += sum_{i,a,j} Yt[i,a,j,sig] G[i,a,j,b],     Yt = sum_mu Co[mu,i] Y
 template <typename T>
 void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const double* eps, double* E, T* dC,
                   T* dg, double* deps) {
   const long n = ctx->n, n2 = n * n, n3 = n2 * n;
-  const long ov = (long)o * v, N = ov * ov;
-  // K = (ia|jb), G, energy
-  DevBuf Kb(N * sizeof(T)), Gb(N * sizeof(T));
-  T* K = Kb.as<T>();
-  T* G = Gb.as<T>();
-  tei_transform_dev<T>(ctx, Co, o, Cv, v, Co, o, Cv, v, K);
-  mp2_g<T>(ctx, K, eps, o, v, G, E, deps);
-  Kb.release();
+  const long ov = (long)o * v, N = ov * ov, on = (long)o * n, nov = n * ov;
   const T one = T(1), zero = T(0);
-  // half-transformed integrals, recomputed here
-  DevBuf H12b(n2 * ov * sizeof(T)), H34b(n2 * ov * sizeof(T));
-  T* H12 = H12b.as<T>();  // [mu,nu,j,b]
-  T* H34 = H34b.as<T>();  // [i,a,lam,sig]
   const T* g = ao_dense<T>(ctx);
+  T* dCo = dC;           // n x o  (columns 0..o-1 of dC, leading dimension n)
+  T* dCv = dC + n * o;   // n x v
+  DevBuf Gb(N * sizeof(T)), X3b(nov * n * sizeof(T)), Y3b(nov * o * sizeof(T));
+  T* G = Gb.as<T>();
+  T* X3 = X3b.as<T>();
+  T* Y3 = Y3b.as<T>();
   {
-    // X[mu,nu,j,sig] = sum_lam g[mu,nu,lam,sig] Co[lam,j] (batched over sig), then
-    // H12[(mu,nu,j),b] = sum_sig X[(mu,nu,j),sig] Cv[sig,b]: 2n^4 o + 2n^3 o v flops and an n^3 o
-    // intermediate (contracting sig with Cv first takes 2n^4 v and n^3 v)
+    // forward (ia|jb), the same four quarter-transforms as the (ia|jb) transform
+    DevBuf X2(nov * n * sizeof(T));
+    {
+      DevBuf Z1(o * n3 * sizeof(T));
+      gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z1.as<T>(), o, 0, 1, "xform_q1");
+      gemm<T>(ctx, false, false, o * n2, v, n, one, Z1.as<T>(), o * n2, 0, Cv, n, 0, zero, X2.as<T>(), o * n2, 0,
+              1, "xform_q2");
+    }
+    permute<T, T>(ctx, X2.as<T>(), {(long)o, n, n, (long)v}, {1, 0, 3, 2}, X3, 1.0, 0.0, "xform_perm");
+    X2.release();
+    gemm<T>(ctx, false, false, nov, o, n, one, X3, nov, 0, Co, n, 0, zero, Y3, nov, 0, 1, "xform_q3");
+    DevBuf X5(N * sizeof(T)), Kb(N * sizeof(T));
+    gemm<T>(ctx, true, false, v, ov * o, n, one, Cv, n, 0, Y3, n, 0, zero, X5.as<T>(), v, 0, 1, "xform_q4");
+    permute<T, T>(ctx, X5.as<T>(), {(long)v, (long)o, (long)v, (long)o}, {1, 0, 3, 2}, Kb.as<T>(), 1.0, 0.0,
+                  "xform_perm");
+    X5.release();
+    mp2_g<T>(ctx, Kb.as<T>(), eps, o, v, G, E, deps);
+  }
+  // position 2: Gp2[i,b,j,a] = G[i,a,j,b]; dCv = Y3[nu,(i,b,j)] Gp2[(i,b,j),a]
+  {
+    DevBuf Gp2(N * sizeof(T));
+    permute<T, T>(ctx, G, {(long)o, (long)v, (long)o, (long)v}, {0, 3, 2, 1}, Gp2.as<T>(), 1.0, 0.0, "mp2grad_s2");
+    gemm<T>(ctx, false, false, n, v, ov * o, one, Y3, n, 0, Gp2.as<T>(), ov * o, 0, zero, dCv, n, 0, 1, "mp2grad_s2");
+  }
+  Y3b.release();
+  // position 3: Gt[i,nu,j,b] (batched over jb; also the first step of the dE/dg back-transform),
+  // Gtp[nu,i,b,j] = Gt permuted; dCo = X3[(nu,i,b),lam]^T Gtp[(nu,i,b),j]
+  DevBuf Gt(on * ov * sizeof(T));
+  gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, on, (int)ov, "mp2grad_s3");
+  {
+    DevBuf Gtp(on * ov * sizeof(T));
+    permute<T, T>(ctx, Gt.as<T>(), {(long)o, n, (long)o, (long)v}, {1, 0, 3, 2}, Gtp.as<T>(), 1.0, 0.0,
+                  "mp2grad_s3");
+    gemm<T>(ctx, true, false, n, o, nov, one, X3, nov, 0, Gtp.as<T>(), nov, 0, zero, dCo, n, 0, 1, "mp2grad_s3");
+  }
+  X3b.release();
+  // Y[mu,a,j,sig] = (mu a|j sig): X[mu,nu,j,sig] = sum_lam g[mu,nu,lam,sig] Co[lam,j] (batched over sig),
+  // then Y[mu,a,(j,sig)] = sum_nu X[mu,nu,(j,sig)] Cv[nu,a] (batched over (j,sig))
+  DevBuf Yb(nov * n * sizeof(T));
+  T* Y = Yb.as<T>();
+  {
     DevBuf X(n3 * o * sizeof(T));
     gemm<T>(ctx, false, false, n2, o, n, one, g, n2, n3, Co, n, 0, zero, X.as<T>(), n2, n2 * o, (int)n,
             "mp2grad_half_x");
-    gemm<T>(ctx, false, false, n2 * o, v, n, one, X.as<T>(), n2 * o, 0, Cv, n, 0, zero, H12, n2 * o, 0, 1,
-            "mp2grad_half_h12");
+    gemm<T>(ctx, false, false, n, v, n, one, X.as<T>(), n, n2, Cv, n, 0, zero, Y, n, n * v, (int)on,
+            "mp2grad_half_y");
   }
+  // position 1: T1[(i,a,j),sig] = sum_b G[(i,a,j),b] Cv[sig,b]; dCo += Y[mu,(a,j,sig)] T1[i,(a,j,sig)]^T
   {
-    DevBuf Z(o * n3 * sizeof(T));  // Z[i,nu,lam,sig] = sum_mu Co[mu,i] g
-    gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z.as<T>(), o, 0, 1, "mp2grad_half_z");
-    gemm<T>(ctx, false, false, o, v, n, one, Z.as<T>(), o, (long)o * n, Cv, n, 0, zero, H34, o, ov, (int)n2,
-            "mp2grad_half_h34");
+    DevBuf T1(ov * on * sizeof(T));
+    gemm<T>(ctx, false, true, ov * o, n, v, one, G, ov * o, 0, Cv, n, 0, zero, T1.as<T>(), ov * o, 0, 1,
+            "mp2grad_s1");
+    gemm<T>(ctx, false, true, n, o, nov, one, Y, n, 0, T1.as<T>(), o, 0, one, dCo, n, 0, 1, "mp2grad_s1");
   }
-  T* dCo = dC;           // n x o  (columns 0..o-1 of dC, leading dimension n)
-  T* dCv = dC + n * o;   // n x v
-  // slot 1: Gt[i,nu,jb] = G[i,a,jb] Cv[nu,a]^T ; dC1 = H12 (n x n ov) . Gt^T  (Gt is also the first
-  // step of the dE/dg back-transform)
-  DevBuf Gt((long)o * n * ov * sizeof(T));
-  gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, (long)o * n, (int)ov, "mp2grad_s1");
-  gemm<T>(ctx, false, true, n, o, n * ov, one, H12, n, 0, Gt.as<T>(), o, 0, zero, dCo, n, 0, 1, "mp2grad_s1");
-  // slot 2: dC2[nu,a] = sum_{mu,i,jb} H12[mu,nu,jb] Co[mu,i] G[i,a,jb], Co contracted into H12 first
-  // (2n^2 o^2 v + 2n o^2 v^2 flops, no permutation of H12):
-  //   M2[(i,jb),nu] = sum_mu Co^T[i,mu] H12[mu,nu,jb]   (batched over nu)
-  //   dC2[nu,a]     = sum_{(i,jb)} M2[(i,jb),nu] Gp[(i,jb),a],  Gp[i,jb,a] = G[i,a,jb]
+  // position 4: Yt[i,(a,j,sig)] = sum_mu Co[mu,i] Y[mu,(a,j,sig)]; dCv += Yt[(i,a,j),sig]^T G[(i,a,j),b]
   {
-    DevBuf CoT((long)o * n * sizeof(T)), M2(ov * o * n * sizeof(T)), Gp(N * sizeof(T));
-    permute<T, T>(ctx, Co, {n, (long)o}, {1, 0}, CoT.as<T>(), 1.0, 0.0, "mp2grad_s2");
-    permute<T, T>(ctx, G, {(long)o, (long)v, ov}, {0, 2, 1}, Gp.as<T>(), 1.0, 0.0, "mp2grad_s2");
-    gemm<T>(ctx, false, false, o, ov, n, one, CoT.as<T>(), o, 0, H12, n2, n, zero, M2.as<T>(), o, o * ov, (int)n,
-            "mp2grad_s2");
-    gemm<T>(ctx, true, false, n, v, o * ov, one, M2.as<T>(), o * ov, 0, Gp.as<T>(), o * ov, 0, zero, dCv, n, 0, 1,
-            "mp2grad_s2");
-  }
-  // slot 3: dC3[lam,j] = sum_{ia,sig} H34[ia,lam,sig] G3[ia,j,sig], G3[ia,j,sig] = sum_b G[ia,j,b] Cv[sig,b];
-  // with G3 laid out j first, the products batched over sig are NN and read H34 in place (no
-  // permutation of H34):
-  //   Gq[j,ia,b]       = G[ia,j,b]
-  //   G3q[(j,ia),sig]  = sum_b Gq[(j,ia),b] Cv[sig,b]
-  //   W[j,lam,sig]     = sum_ia G3q[j,ia,sig] H34[ia,lam,sig]     (batched over sig)
-  //   dC3[lam,j]      += sum_sig W[j,lam,sig]                      (W times a vector of ones)
-  {
-    DevBuf Gq(N * sizeof(T)), G3q(ov * o * n * sizeof(T)), W((long)o * n2 * sizeof(T)), ones(n * sizeof(T)),
-        dC3t((long)o * n * sizeof(T));
-    const long on = (long)o * n;
-    permute<T, T>(ctx, G, {ov, (long)o, (long)v}, {1, 0, 2}, Gq.as<T>(), 1.0, 0.0, "mp2grad_s3");
-    gemm<T>(ctx, false, true, o * ov, n, v, one, Gq.as<T>(), o * ov, 0, Cv, n, 0, zero, G3q.as<T>(), o * ov, 0, 1,
-            "mp2grad_s3");
-    gemm<T>(ctx, false, false, o, n, ov, one, G3q.as<T>(), o, o * ov, H34, ov, ov * n, zero, W.as<T>(), o, on, (int)n,
-            "mp2grad_s3");
-    fill<T>(ctx, ones.as<T>(), n, 1.0);
-    gemm<T>(ctx, false, false, on, 1, n, one, W.as<T>(), on, 0, ones.as<T>(), n, 0, zero, dC3t.as<T>(), on, 0, 1,
-            "mp2grad_s3");
-    permute<T, T>(ctx, dC3t.as<T>(), {(long)o, n}, {1, 0}, dCo, 1.0, 1.0, "mp2grad_s3");
-  }
-  // slot 4: dC4[sig,b] = sum_{ia,lam,j} H34[ia,lam,sig] Co[lam,j] G[ia,j,b], Co contracted into H34
-  // first (2n^2 o^2 v + 2n o^2 v^2 flops; contracting it into G first takes 2n o^2 v^2 + 2n^2 o v^2):
-  //   M4[ia,j,sig] = sum_lam H34[ia,lam,sig] Co[lam,j]   (batched over sig)
-  //   dC4[sig,b]  += sum_{(ia,j)} M4[(ia,j),sig] G[(ia,j),b]
-  {
-    DevBuf M4(ov * o * n * sizeof(T));
-    gemm<T>(ctx, false, false, ov, o, n, one, H34, ov, ov * n, Co, n, 0, zero, M4.as<T>(), ov, ov * o, (int)n,
-            "mp2grad_s4");
-    gemm<T>(ctx, true, false, n, v, ov * o, one, M4.as<T>(), ov * o, 0, G, ov * o, 0, one, dCv, n, 0, 1,
+    DevBuf Yt(ov * on * sizeof(T));
+    gemm<T>(ctx, true, false, o, nov, n, one, Co, n, 0, Y, n, 0, zero, Yt.as<T>(), o, 0, 1, "mp2grad_s4");
+    Yb.release();
+    gemm<T>(ctx, true, false, n, v, ov * o, one, Yt.as<T>(), ov * o, 0, G, ov * o, 0, one, dCv, n, 0, 1,
             "mp2grad_s4");
   }
-  H12b.release();
-  H34b.release();
   if (dg) {
     // back-transform G -> dE/dg with the occupied index i contracted last, so that the n^4 result
     // comes out of one GEMM already in [mu,nu,lam,sig] order (no permutation pass):
-    //   Gt[(i,nu),j,b]   = sum_a G[i,a,j,b] Cv[nu,a]             (slot 1)
+    //   Gt[(i,nu),j,b]   = sum_a G[i,a,j,b] Cv[nu,a]             (position 3)
     //   P2[(i,nu),lam,b] = sum_j Gt[(i,nu),j,b] Co[lam,j]        (batched over b)
     //   Q[(i,nu,lam),sig] = sum_b P2[(i,nu,lam),b] Cv[sig,b]
     //   dg[mu,(nu,lam,sig)] = sum_i Co[mu,i] Q[i,(nu,lam,sig)]
-    const long on = (long)o * n;
     DevBuf P2(on * n * v * sizeof(T));
     gemm<T>(ctx, false, true, on, n, o, one, Gt.as<T>(), on, on * o, Co, n, 0, zero, P2.as<T>(), on, on * n, v,
             "mp2grad_back_p2");

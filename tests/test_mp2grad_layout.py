@@ -1,10 +1,10 @@
 """CPU check of the dense MP2 adjoint's product sequence (csrc/mp2grad.hip, mp2_grad_dev).
 
 Every GEMM and permutation the device issues is restated here on flat column-major arrays with the
-same M, N, K, leading dimensions and batch strides, so the buffer layouts of the half-transforms
-(H12 = (mu nu|jb), H34 = (ia|lam sig)), the four dE/dC slots and the back-transform to dE/dg are
-checked against the oracle's einsum gradient (RMP2.jl:33-37 differentiated; ``rmp2_grad``) on a
-small tensor with no index symmetry."""
+same M, N, K, leading dimensions and batch strides: the forward (ia|jb) transform with its kept
+intermediates (i nu|lam b) and (i nu|j b), the extra half-transform (mu a|j sig), the four dE/dC
+positions and the back-transform to dE/dg, checked against the oracle's einsum gradient (RMP2.jl:33-37
+differentiated; ``rmp2_grad``) on small tensors with no index symmetry."""
 import numpy as np
 import pytest
 from numpy.lib.stride_tricks import as_strided
@@ -34,97 +34,81 @@ def permute(src, dims, perm, dst, alpha=1.0, beta=0.0):
     out[...] = alpha * x + beta * out
 
 
-@pytest.mark.parametrize("n,o,seed", [(12, 3, 0), (11, 4, 1)])
-def test_dense_adjoint_sequence_matches_oracle(n, o, seed):
+def _system(n, o, seed):
     rng = np.random.default_rng(seed)
-    v = n - o
     g = rng.standard_normal((n, n, n, n))                 # no (mu nu) / (lam sig) / pair symmetry
     C = rng.standard_normal((n, n))
     eps = np.sort(rng.uniform(-2, 2, n))
     eps[o:] += 3.0
-    E, dC_ref, dg_ref = O.rmp2_grad(g, C, eps, o)
-    Co, Cv = np.asfortranarray(C[:, :o]).ravel(order="F"), np.asfortranarray(C[:, o:]).ravel(order="F")
-    gf = g.ravel(order="F")
-    n2, n3, ov = n * n, n ** 3, o * v
-    # G = dE/dK, [i,a,j,b]
-    K = O.tei_transform(g, C[:, :o], C[:, o:], C[:, :o], C[:, o:])
-    D = O.form_Dijab(eps, o, v).transpose(0, 2, 1, 3)
-    G = (2.0 * (2.0 * K - K.transpose(0, 3, 2, 1)) / D).ravel(order="F")
-    z = lambda m: np.zeros(m)
+    return g, C, eps
 
-    # H12[mu,nu,j,b]: X[mu,nu,j,sig] = sum_lam g Co (batched over sig), then X Cv
-    X, H12 = z(n2 * o * n), z(n2 * ov)
-    gemm(0, 0, n2, o, n, 1.0, gf, n2, n3, Co, n, 0, 0.0, X, n2, n2 * o, n)
-    gemm(0, 0, n2 * o, v, n, 1.0, X, n2 * o, 0, Cv, n, 0, 0.0, H12, n2 * o, 0)
-    # H34[i,a,lam,sig]: Z[i,nu,lam,sig] = Co^T g, then Z Cv batched over (lam,sig)
-    Z, H34 = z(o * n3), z(n2 * ov)
-    gemm(1, 0, o, n3, n, 1.0, Co, n, 0, gf, n, 0, 0.0, Z, o, 0)
-    gemm(0, 0, o, v, n, 1.0, Z, o, o * n, Cv, n, 0, 0.0, H34, o, ov, n2)
+
+def device_sequence(g, C, eps, o, pos4_stride_error=0):
+    """dE/dC and dE/dg through the device's products; G from the forward K as mp2_g_kernel forms it."""
+    n = C.shape[0]
+    v = n - o
+    n2, n3, ov = n * n, n ** 3, o * v
+    nov, on, N = n * ov, o * n, ov * ov
+    gf = g.ravel(order="F")
+    Co, Cv = C[:, :o].ravel(order="F").copy(), C[:, o:].ravel(order="F").copy()
+    z = np.zeros
+    # forward (ia|jb) with X3 = (i nu|lam b) as [nu,i,b,lam] and Y3 = (i nu|j b) as [nu,i,b,j] kept
+    Z1, X2, X3, Y3, X5, K = z(o * n3), z(nov * n), z(nov * n), z(nov * o), z(N), z(N)
+    gemm(1, 0, o, n3, n, 1.0, Co, n, 0, gf, n, 0, 0.0, Z1, o, 0)
+    gemm(0, 0, o * n2, v, n, 1.0, Z1, o * n2, 0, Cv, n, 0, 0.0, X2, o * n2, 0)
+    permute(X2, (o, n, n, v), (1, 0, 3, 2), X3)
+    gemm(0, 0, nov, o, n, 1.0, X3, nov, 0, Co, n, 0, 0.0, Y3, nov, 0)
+    gemm(1, 0, v, ov * o, n, 1.0, Cv, n, 0, Y3, n, 0, 0.0, X5, v, 0)
+    permute(X5, (v, o, v, o), (1, 0, 3, 2), K)
+    K4 = K.reshape(o, v, o, v, order="F")
+    D = O.form_Dijab(eps, o, v).transpose(0, 2, 1, 3)
+    G = (2.0 * (2.0 * K4 - K4.transpose(0, 3, 2, 1)) / D).ravel(order="F")
     dCo, dCv = z(n * o), z(n * v)
-    # slot 1
-    Gt = z(o * n * ov)
-    gemm(0, 1, o, n, v, 1.0, G, o, ov, Cv, n, 0, 0.0, Gt, o, o * n, ov)
-    gemm(0, 1, n, o, n * ov, 1.0, H12, n, 0, Gt, o, 0, 0.0, dCo, n, 0)
-    # slot 2
-    CoT, Gp, M2 = z(o * n), z(ov * ov), z(ov * o * n)
-    permute(Co, (n, o), (1, 0), CoT)
-    permute(G, (o, v, ov), (0, 2, 1), Gp)
-    gemm(0, 0, o, ov, n, 1.0, CoT, o, 0, H12, n2, n, 0.0, M2, o, o * ov, n)
-    gemm(1, 0, n, v, o * ov, 1.0, M2, o * ov, 0, Gp, o * ov, 0, 0.0, dCv, n, 0)
-    # slot 3
-    on = o * n
-    Gq, G3q, W, ones, dC3t = z(ov * ov), z(ov * o * n), z(o * n2), np.ones(n), z(on)
-    permute(G, (ov, o, v), (1, 0, 2), Gq)
-    gemm(0, 1, o * ov, n, v, 1.0, Gq, o * ov, 0, Cv, n, 0, 0.0, G3q, o * ov, 0)
-    gemm(0, 0, o, n, ov, 1.0, G3q, o, o * ov, H34, ov, ov * n, 0.0, W, o, on, n)
-    gemm(0, 0, on, 1, n, 1.0, W, on, 0, ones, n, 0, 0.0, dC3t, on, 0)
-    permute(dC3t, (o, n), (1, 0), dCo, 1.0, 1.0)
-    # slot 4
-    M4 = z(ov * o * n)
-    gemm(0, 0, ov, o, n, 1.0, H34, ov, ov * n, Co, n, 0, 0.0, M4, ov, ov * o, n)
-    gemm(1, 0, n, v, ov * o, 1.0, M4, ov * o, 0, G, ov * o, 0, 1.0, dCv, n, 0)
-    dC = np.concatenate([dCo.reshape(n, o, order="F"), dCv.reshape(n, v, order="F")], axis=1)
-    assert np.max(np.abs(dC - dC_ref)) < 1e-10 * np.max(np.abs(dC_ref))
-    # back-transform: P2[(i,nu),lam,b] batched over b, Q, then dg = Co Q
+    # position 2
+    Gp2 = z(N)
+    permute(G, (o, v, o, v), (0, 3, 2, 1), Gp2)
+    gemm(0, 0, n, v, ov * o, 1.0, Y3, n, 0, Gp2, ov * o, 0, 0.0, dCv, n, 0)
+    # position 3
+    Gt, Gtp = z(on * ov), z(on * ov)
+    gemm(0, 1, o, n, v, 1.0, G, o, ov, Cv, n, 0, 0.0, Gt, o, on, ov)
+    permute(Gt, (o, n, o, v), (1, 0, 3, 2), Gtp)
+    gemm(1, 0, n, o, nov, 1.0, X3, nov, 0, Gtp, nov, 0, 0.0, dCo, n, 0)
+    # (mu a|j sig): X[mu,nu,j,sig] batched over sig, then Y[mu,a,(j,sig)] batched over (j,sig)
+    X, Y = z(n3 * o), z(nov * n)
+    gemm(0, 0, n2, o, n, 1.0, gf, n2, n3, Co, n, 0, 0.0, X, n2, n2 * o, n)
+    gemm(0, 0, n, v, n, 1.0, X, n, n2, Cv, n, 0, 0.0, Y, n, n * v, on)
+    # position 1
+    T1 = z(ov * on)
+    gemm(0, 1, ov * o, n, v, 1.0, G, ov * o, 0, Cv, n, 0, 0.0, T1, ov * o, 0)
+    gemm(0, 1, n, o, nov, 1.0, Y, n, 0, T1, o, 0, 1.0, dCo, n, 0)
+    # position 4 (pos4_stride_error shifts the second product's leading dimension: a planted mistake)
+    Yt = z(ov * on + n * v)
+    gemm(1, 0, o, nov, n, 1.0, Co, n, 0, Y, n, 0, 0.0, Yt, o, 0)
+    gemm(1, 0, n, v, ov * o, 1.0, Yt, ov * o + pos4_stride_error, 0, G, ov * o, 0, 1.0, dCv, n, 0)
+    # back-transform
     P2, Q, dg = z(on * n * v), z(on * n2), z(n2 * n2)
     gemm(0, 1, on, n, o, 1.0, Gt, on, on * o, Co, n, 0, 0.0, P2, on, on * n, v)
     gemm(0, 1, on * n, n, v, 1.0, P2, on * n, 0, Cv, n, 0, 0.0, Q, on * n, 0)
     gemm(0, 0, n, n3, o, 1.0, Co, n, 0, Q, o, 0, 0.0, dg, n, 0)
-    assert np.max(np.abs(dg.reshape(n, n, n, n, order="F") - dg_ref)) < 1e-10 * np.max(np.abs(dg_ref))
+    dC = np.concatenate([dCo.reshape(n, o, order="F"), dCv.reshape(n, v, order="F")], axis=1)
+    return dC, dg.reshape(n, n, n, n, order="F")
+
+
+@pytest.mark.parametrize("n,o,seed", [(12, 3, 0), (11, 4, 1), (9, 6, 2)])
+def test_dense_adjoint_sequence_matches_oracle(n, o, seed):
+    g, C, eps = _system(n, o, seed)
+    _, dC_ref, dg_ref = O.rmp2_grad(g, C, eps, o)
+    dC, dg = device_sequence(g, C, eps, o)
+    assert np.max(np.abs(dC - dC_ref)) < 1e-10 * np.max(np.abs(dC_ref))
+    assert np.max(np.abs(dg - dg_ref)) < 1e-10 * np.max(np.abs(dg_ref))
 
 
 def test_sequence_check_catches_a_wrong_stride():
-    """The restatement is not vacuous: slot 4 with the batch stride of H34 off by one block of
-    (ia) misses the oracle's dE/dCv."""
-    rng = np.random.default_rng(3)
+    """The restatement is not vacuous: position 4's product with a leading dimension off by one
+    misses the oracle's dE/dCv."""
     n, o = 10, 3
-    v = n - o
-    g = rng.standard_normal((n, n, n, n))
-    C = rng.standard_normal((n, n))
-    eps = np.sort(rng.uniform(-2, 2, n))
-    eps[o:] += 3.0
-    ov = o * v
-    gf, Co, Cv = g.ravel(order="F"), C[:, :o].ravel(order="F"), C[:, o:].ravel(order="F")
-    K = O.tei_transform(g, C[:, :o], C[:, o:], C[:, :o], C[:, o:])
-    D = O.form_Dijab(eps, o, v).transpose(0, 2, 1, 3)
-    G = (2.0 * (2.0 * K - K.transpose(0, 3, 2, 1)) / D).ravel(order="F")
-    Z, H34 = np.zeros(o * n ** 3), np.zeros(n * n * ov)
-    gemm(1, 0, o, n ** 3, n, 1.0, Co, n, 0, gf, n, 0, 0.0, Z, o, 0)
-    gemm(0, 0, o, v, n, 1.0, Z, o, o * n, Cv, n, 0, 0.0, H34, o, ov, n * n)
-
-    def slot4(sA):
-        M4, dCv = np.zeros(ov * o * n), np.zeros(n * v)
-        gemm(0, 0, ov, o, n, 1.0, H34, ov, sA, Co, n, 0, 0.0, M4, ov, ov * o, n - 1)
-        gemm(1, 0, n, v, ov * o, 1.0, M4, ov * o, 0, G, ov * o, 0, 0.0, dCv, n, 0)
-        return dCv
-
-    # the oracle's slot-4 term alone: h4 contracted with g
-    h4 = np.einsum("iajb,mi,na,lj->mnlb", G.reshape(o, v, o, v, order="F"), C[:, :o], C[:, o:], C[:, :o],
-                   optimize=True)
-    ref = np.einsum("mnlb,mnls->sb", h4, g, optimize=True)
-    # (n - 1 batches of sig: the last sig is dropped in both so the comparison stays one-to-one)
-    good, bad = slot4(ov * n), slot4(ov * n - ov)
-    ref_trunc = ref.copy()
-    ref_trunc[n - 1] = 0.0
-    assert np.max(np.abs(good.reshape(n, v, order="F") - ref_trunc)) < 1e-10 * np.max(np.abs(ref))
-    assert np.max(np.abs(bad.reshape(n, v, order="F") - ref_trunc)) > 1e-3 * np.max(np.abs(ref))
+    g, C, eps = _system(n, o, 3)
+    _, dC_ref, _ = O.rmp2_grad(g, C, eps, o)
+    dC, _ = device_sequence(g, C, eps, o, pos4_stride_error=1)
+    assert np.max(np.abs(dC[:, o:] - dC_ref[:, o:])) > 1e-3 * np.max(np.abs(dC_ref))
+    assert np.max(np.abs(dC[:, :o] - dC_ref[:, :o])) < 1e-10 * np.max(np.abs(dC_ref))
