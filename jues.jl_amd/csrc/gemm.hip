@@ -241,9 +241,8 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
   load(0);
   store(0);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+  // the last slice is peeled off, so that the loop body is one block without conditions
+  auto compute = [&](int cur) {
     if constexpr (sizeof(T) == 4) {
       // FP32: the 32-cycle MFMAs cannot hide a fragment read, so fragments for
       // step s+1 are fetched while the MFMAs of step s issue
@@ -267,9 +266,15 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
           for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(bf[0][j], af[0][i], acc[i][j]);
       }
     }
-    if (kt + 1 < nk) store(cur ^ 1);
+  };
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int cur = kt & 1;
+    load((kt + 1) * BK);
+    compute(cur);
+    store(cur ^ 1);
     __syncthreads();
   }
+  compute((nk - 1) & 1);
 
   // acc[i][j][r] holds C(m, n) with m = lane%16 of block i, n = Mfma<T>::row(lane, r) of block j.
 #pragma unroll
@@ -877,9 +882,8 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
   load(0);
   store(0);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+  // the last slice is peeled off, so that the loop body is one block without conditions
+  auto compute = [&](int cur) {
     f4v bf[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -896,9 +900,15 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
         for (int j = 0; j < 4; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j][s2], a0[s2], acc[i][j], 0, 0, 0);
     }
-    if (kt + 1 < nk) store(cur ^ 1);
+  };
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int cur = kt & 1;
+    load((kt + 1) * BK);
+    compute(cur);
+    store(cur ^ 1);
     __syncthreads();
   }
+  compute((nk - 1) & 1);
 
 #pragma unroll
   for (int i = 0; i < IM; ++i) {
@@ -982,9 +992,8 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
   load(0);
   store(0);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+  // the last slice is peeled off, so that the loop body is one block without conditions
+  auto compute = [&](int cur) {
     d2v bf[2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -1007,9 +1016,15 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j][s2 >> 1][s2 & 1], a0[s2 >> 1][s2 & 1], acc[i][j],
                                                            0, 0, 0);
     }
-    if (kt + 1 < nk) store(cur ^ 1);
+  };
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int cur = kt & 1;
+    load((kt + 1) * BK);
+    compute(cur);
+    store(cur ^ 1);
     __syncthreads();
   }
+  compute((nk - 1) & 1);
 
   // acc[i][j][r]: C(m, n) with m = 16 i + lane % 16, n = n0 + 32 wn + 16 j + lane / 16 + 4 r
 #pragma unroll
@@ -1085,9 +1100,8 @@ __global__ __launch_bounds__(512, 4) void gemm64_tall_kernel(GemmSide<double> g,
   load(0);
   store(0);
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+  // the last slice is peeled off, so that the loop body is one block without conditions
+  auto compute = [&](int cur) {
     // A's two blocks are held for the slice, B's blocks streamed one ahead of their MFMAs (for
     // IN = 4 read as they are used: the one-ahead copy would not fit in 128 VGPRs)
     constexpr bool ahead = IN < 4;
@@ -1121,9 +1135,15 @@ __global__ __launch_bounds__(512, 4) void gemm64_tall_kernel(GemmSide<double> g,
                                                            0, 0, 0);
       if (ahead && j + 1 < IN) { b0[0] = b1[0]; b0[1] = b1[1]; }
     }
-    if (kt + 1 < nk) store(cur ^ 1);
+  };
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int cur = kt & 1;
+    load((kt + 1) * BK);
+    compute(cur);
+    store(cur ^ 1);
     __syncthreads();
   }
+  compute((nk - 1) & 1);
 
   // acc[i][j][r]: C(m, n) with m = m0 + 32 wm + 16 i + lane % 16, n = n0 + 16 j + lane / 16 + 4 r
 #pragma unroll
