@@ -1169,14 +1169,17 @@ __global__ __launch_bounds__(512, 4) void gemm64_tall_kernel(GemmSide<double> g,
 static inline long tiles_m128(long x) { return (x + 127) / 128; }
 
 // Whether a product can run on the high-occupancy kernels: 128x128 tiles, at least one full
-// k-slice, and a tile's k-slice addressable with 32-bit offsets (A m-contiguous: 128 rows and 16
-// columns of lda; a k-contiguous A, TN / TT, and its B in those instantiations: 64 rows of lda / ldb
-// from each of two bases).
+// k-slice, and a tile's k-slice addressable with 32-bit byte offsets from one buffer base:
+// an m-contiguous A (n-contiguous B) spans 16 columns of lda (ldb); a k-contiguous B spans 128 rows of
+// ldb; in the TN / TT instantiations a k-contiguous A, and its k-contiguous B, span 64 rows of lda /
+// ldb from each of two bases.
 template <typename T>
-static bool occ_kernel_ok(bool ta, long K, long lda, long ldb) {
+static bool occ_kernel_ok(bool ta, bool tb, long K, long lda, long ldb) {
   if (K < 16) return false;
-  if (ta) return (long)sizeof(T) * 64 * (std::max(lda, ldb) + 1) < (1L << 31);
-  return (long)sizeof(T) * (128 + 64) * (std::max(lda, ldb) + 1) < (1L << 31);
+  const auto fits = [](long rows, long ld) { return (long)sizeof(T) * rows * (ld + 1) < (1L << 31); };
+  const bool a_ok = ta ? fits(64, lda) : fits(16, lda);
+  const bool b_ok = tb ? fits(16, ldb) : fits(ta ? 64 : 128, ldb);
+  return a_ok && b_ok;
 }
 
 template <typename T>
@@ -1292,7 +1295,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
     T* Wk = static_cast<T*>(scratch(ctx, 9, sizeof(T) * (size_t)S * M * N));
     const long sa = ta ? kc : kc * lda, sb = tb ? kc * ldb : kc;
     const long klast = K - (long)(S - 1) * kc;
-    if (shape == 0 && occ_kernel_ok<T>(ta, std::min(kc, klast), lda, ldb)) {
+    if (shape == 0 && occ_kernel_ok<T>(ta, tb, std::min(kc, klast), lda, ldb)) {
       // all chunks in one launch, the last one shorter
       KTimer t1(ctx, name, 2.0 * M * N * K, (double)sizeof(T) * ((double)M * K + (double)K * N + (double)S * M * N));
       GemmSide<T> p = make_side<T>(M, N, kc, A, lda, sa, B, ldb, sb, Wk, M, M * N);
@@ -1390,7 +1393,7 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
   }
   // FP32 NN and NT: gemm32_nn_kernel (four workgroups per CU); FP64: gemm64_kernel (two
   // 512-thread workgroups per CU)
-  if (shape == 0 && occ_kernel_ok<T>(ta, K, lda, ldb)) {
+  if (shape == 0 && occ_kernel_ok<T>(ta, tb, K, lda, ldb)) {
     KTimer timer(ctx, name, flops, bytes);
     for (int b0 = 0; b0 < batch; b0 += 65535) {
       const int nb = std::min(batch - b0, 65535);
@@ -1445,8 +1448,8 @@ void gemm_pair(jues_ctx* ctx, bool tb, long M0, long N0, long K0, const T* A0, l
   const bool skinny0 = (M0 <= 64 && N0 >= 8192) || (N0 <= 64 && M0 >= 8192);
   const bool skinny1 = (M1 <= 64 && N1 >= 8192) || (N1 <= 64 && M1 >= 8192);
   const bool empty0 = M0 <= 0 || N0 <= 0, empty1 = M1 <= 0 || N1 <= 0;
-  if (empty0 || empty1 || skinny0 || skinny1 || !occ_kernel_ok<T>(false, K0, lda0, ldb0) ||
-      !occ_kernel_ok<T>(false, K1, lda1, ldb1)) {
+  if (empty0 || empty1 || skinny0 || skinny1 || !occ_kernel_ok<T>(false, tb, K0, lda0, ldb0) ||
+      !occ_kernel_ok<T>(false, tb, K1, lda1, ldb1)) {
     gemm<T>(ctx, false, tb, M0, N0, K0, alpha, A0, lda0, 0, B0, ldb0, 0, beta, C0, ldc0, 0, 1, tag);
     gemm<T>(ctx, false, tb, M1, N1, K1, alpha, A1, lda1, 0, B1, ldb1, 0, beta, C1, ldc1, 0, 1, tag);
     return;
@@ -1503,7 +1506,7 @@ void gemm_ksum(jues_ctx* ctx, bool tb, long M, long N, long K0, const T* A0, con
   const bool skinny = (M <= 64 && N >= 8192) || (N <= 64 && M >= 8192);
   if (M <= 0 || N <= 0 || batch <= 0) return;
   if (skinny || K0 < 16 || K1 < 16 || K0 % 16 != 0 || K1 % 16 != 0 || batch > 65535 ||
-      !occ_kernel_ok<T>(false, std::max(K0, K1), lda, ldb)) {
+      !occ_kernel_ok<T>(false, tb, std::max(K0, K1), lda, ldb)) {
     gemm<T>(ctx, false, tb, M, N, K0, alpha, A0, lda, sA, B0, ldb, sB, beta, C, ldc, sC, batch, tag);
     gemm<T>(ctx, false, tb, M, N, K1, alpha, A1, lda, sA, B1, ldb, sB, T(1), C, ldc, sC, batch, tag);
     return;
