@@ -207,7 +207,9 @@ jues_status jues_rmp2_noalloc(jues_ctx* ctx, int prec, const double* Cocc, int o
  * NULL: dE_dC (n x n), dE_dg (n^4, only with a dense AO tensor) and dE_deps (n).
  * Every entry of C, g and eps is an independent variable.  With AO factors the
  * gradient takes the factor route (dL = 2 G L through L = Co^T B_P Cv; no n^2 o v
- * intermediate); with a dense AO tensor the half-transforms are recomputed. */
+ * intermediate); with a dense AO tensor the call recomputes the half-transformed
+ * integrals it needs (the forward transform's and (mu a|j sig)), storing nothing
+ * between calls. */
 jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, int o, int n,
                            const double* eps, double* E, double* dE_dC, double* dE_dg,
                            double* dE_deps);
