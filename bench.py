@@ -299,6 +299,19 @@ def mp2_bench() -> dict:
     # back-transform to dE/dg, i contracted last (2n^4 o + 2n^3 o v + 2n^2 o^2 v)
     F_G = F_T + 4 * n**4 * o + 4 * n**3 * o * v + 8 * n**2 * o**2 * v + 8 * n * o**2 * v**2
     res = timed("rmp2_grad_fp64_with_dEdg", lambda: MollerPlesset.rmp2_grad(w), F_G, 78.6)
+    res32 = timed("rmp2_grad_fp32_mixed_with_dEdg", lambda: MollerPlesset.rmp2_grad(w, prec="fp32_mixed"), F_G, 157.3)
+    # FP32-mixed adjoint against FP64: max-abs differences relative to the FP64 maxima (dE/dg in chunks
+    # of its 4.3e9 entries, so no second n^4 temporary is formed on the host)
+    a, b = res[2].ravel(order="K"), res32[2].ravel(order="K")
+    step = 1 << 26
+    dg_diff = max(float(np.max(np.abs(a[i:i + step] - b[i:i + step]))) for i in range(0, a.size, step))
+    dg_max = max(float(np.max(np.abs(a[i:i + step]))) for i in range(0, a.size, step))
+    out["grad_fp32_mixed_vs_fp64"] = {
+        "E_rel": abs(res32[0] - res[0]) / abs(res[0]),
+        "dC_rel": float(np.max(np.abs(res32[1] - res[1])) / np.max(np.abs(res[1]))),
+        "dg_rel": dg_diff / dg_max,
+        "deps_rel": float(np.max(np.abs(res32[3] - res[3])) / np.max(np.abs(res[3])))}
+    del a, b, res32
     out["E_mp2_fp64"] = E64
     out["E_mp2_fp32_mixed_abs_diff"] = abs(E32 - E64)
     out["grad_E"] = res[0]

@@ -1166,6 +1166,114 @@ __global__ __launch_bounds__(512, 4) void gemm64_tall_kernel(GemmSide<double> g,
   }
 }
 
+// FP32 NN C = alpha A B + beta C for a tall, narrow product (N <= 64, M large): a 256x(16 IN) tile over
+// four wavefronts of 64x(16 IN) stacked in m, four workgroups per CU.  A (m-contiguous) is staged as in
+// gemm32_nn_kernel, each thread holding one row's four 16-byte k-chunks; B (k-contiguous, at most 64
+// rows) as gemm32_nn_kernel's B.  K must be a multiple of 16.
+template <int IN>
+__global__ __launch_bounds__(256, 4) void gemm32_tall_kernel(GemmSide<float> g, float alpha, float beta) {
+  constexpr int BM = 256, BN = 16 * IN, BK = 16;
+  __shared__ __attribute__((aligned(16))) float As[2][BM * BK];
+  __shared__ __attribute__((aligned(16))) float Bs[2][BN * BK];
+  const int tid = threadIdx.x, lane = tid & 63, wm = tid >> 6;
+  const int M = g.M, N = g.N, K = g.K;
+  const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+  const float* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
+  const float* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
+  float* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
+  int tm, tn;
+  if (!tile_coords_bounded((int)blockIdx.x, (int)gridDim.x, g.tiles_m, g.tiles_n, tm, tn)) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  // A share: row am = tid, k = 4 c + u (c, u = 0..3); B share (threads below 4 BN): row bn, chunk bc
+  const int am = tid;
+  const int bn = tid >> 2, bc = tid & 3;
+  const bool b_share = tid < 4 * BN;
+  float ra[16];
+  f4v rb;
+  const int a_voff = 4 * (min(m0 + am, M - 1) - m0);
+  const int b_voff = 4 * (4 * bc + (min(n0 + bn, N - 1) - n0) * (int)ldb);
+  auto load = [&](int k0) {
+    const auto rA = make_rsrc(A + m0 + (long)k0 * lda);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) ra[q] = bload<float>(rA, a_voff, 4 * q * (int)lda);
+    if (b_share) {
+      const auto rB = make_rsrc(B + k0 + (long)n0 * ldb);
+      rb = __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, b_voff, 0, 0));
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(c, am)]) =
+          f4v{ra[4 * c], ra[4 * c + 1], ra[4 * c + 2], ra[4 * c + 3]};
+    if (b_share) *reinterpret_cast<f4v*>(&Bs[buf][bn * BK + 4 * g32_chunk(bc, bn)]) = rb;
+  };
+
+  fvec4 acc[4][IN];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < IN; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
+  const int fr = lane & 15, fk = lane >> 4;
+  const int nk = K / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  // B's blocks are held for the slice, A's streamed one ahead of their MFMAs
+  auto compute = [&](int cur) {
+    f4v bf[IN];
+#pragma unroll
+    for (int j = 0; j < IN; ++j) {
+      const int r = j * 16 + fr;
+      bf[j] = *reinterpret_cast<const f4v*>(&Bs[cur][r * BK + 4 * g32_chunk(fk, r)]);
+    }
+    f4v a0 = *reinterpret_cast<const f4v*>(&As[cur][(wm * 64 + fr) * BK + 4 * g32_chunk(fk, wm * 64 + fr)]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      f4v a1;
+      if (i + 1 < 4) {
+        const int r = wm * 64 + (i + 1) * 16 + fr;
+        a1 = *reinterpret_cast<const f4v*>(&As[cur][r * BK + 4 * g32_chunk(fk, r)]);
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int j = 0; j < IN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j][s2], a0[s2], acc[i][j], 0, 0, 0);
+      if (i + 1 < 4) a0 = a1;
+    }
+  };
+  // the last slice is peeled off, so that the loop body is one block without conditions
+  for (int kt = 0; kt + 1 < nk; ++kt) {
+    const int cur = kt & 1;
+    load((kt + 1) * BK);
+    compute(cur);
+    store(cur ^ 1);
+    __syncthreads();
+  }
+  compute((nk - 1) & 1);
+
+  // acc[i][j][r]: C(m, n) with m = m0 + 64 wm + 16 i + lane % 16, n = n0 + 16 j + 4 (lane / 16) + r
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + wm * 64 + i * 16 + fr;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < IN; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + j * 16 + 4 * fk + r;
+        if (n < N) {
+          const long off = (long)m + (long)n * ldc;
+          float v = alpha * acc[i][j][r];
+          if (beta != 0.f) v += beta * C[off];
+          C[off] = v;
+        }
+      }
+    }
+  }
+}
+
 static inline long tiles_m128(long x) { return (x + 127) / 128; }
 
 // Whether a product can run on the high-occupancy kernels: 128x128 tiles, at least one full
@@ -1367,8 +1475,30 @@ static void gemm_impl(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T
       return;
     }
   }
-  // Tall, narrow FP64 NN products (N <= 64): gemm64_tall_kernel, 256x48 or 256x64 tiles (a k-slice
-  // of A spans 16 lda, B's 64 rows 64 ldb, within 32-bit byte offsets)
+  // Tall, narrow NN products (N <= 64): gemm64_tall_kernel / gemm32_tall_kernel, 256x48 or 256x64
+  // tiles (a k-slice of A spans 16 lda, B's 64 rows 64 ldb, within 32-bit byte offsets)
+  if constexpr (sizeof(T) == 4) {
+    if (shape == 2 && !ta && !tb && K % 16 == 0 && 4L * 16 * (lda + 1) < (1L << 31) &&
+        4L * 64 * (ldb + 1) < (1L << 31)) {
+      KTimer timer(ctx, name, flops, bytes);
+      for (int b0 = 0; b0 < batch; b0 += 65535) {
+        const int nb = std::min(batch - b0, 65535);
+        GemmSide<float> p = make_side<float>(M, N, K, reinterpret_cast<const float*>(A + (long)b0 * sA), lda, sA,
+                                             reinterpret_cast<const float*>(B + (long)b0 * sB), ldb, sB,
+                                             reinterpret_cast<float*>(C + (long)b0 * sC), ldc, sC);
+        const int bn = N <= 48 ? 48 : 64;  // 16-column blocks: 3 or 4
+        p.tiles_m = (int)((M + 255) / 256);
+        p.tiles_n = (int)((N + bn - 1) / bn);
+        const dim3 grid((unsigned)((long)p.tiles_m * p.tiles_n), 1, nb);
+        if (bn == 48)
+          hipLaunchKernelGGL(gemm32_tall_kernel<3>, grid, dim3(256), 0, ctx->stream, p, (float)alpha, (float)beta);
+        else
+          hipLaunchKernelGGL(gemm32_tall_kernel<4>, grid, dim3(256), 0, ctx->stream, p, (float)alpha, (float)beta);
+        JUES_HIP(hipGetLastError());
+      }
+      return;
+    }
+  }
   if constexpr (sizeof(T) == 8) {
     if (shape == 2 && !ta && !tb && K % 16 == 0 && 8L * 16 * (lda + 1) < (1L << 31) &&
         8L * 64 * (ldb + 1) < (1L << 31)) {

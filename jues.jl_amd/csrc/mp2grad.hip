@@ -211,9 +211,10 @@ void mp2_grad_factors_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v,
 //   pos 2: dCv[nu,a]   = sum_{i,b,j} Y3[nu,i,b,j] G[i,a,j,b]
 //   pos 3: dCo[lam,j]  = sum_{nu,i,b} X3[nu,i,b,lam] Gt[i,nu,j,b],  Gt = sum_a G[i,a,j,b] Cv[nu,a]
 //   pos 4: dCv[sig,b] += sum_{i,a,j} Yt[i,a,j,sig] G[i,a,j,b],     Yt = sum_mu Co[mu,i] Y
+// dg (FP64 in both forms) may be null.
 template <typename T>
 void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const double* eps, double* E, T* dC,
-                  T* dg, double* deps) {
+                  double* dg, double* deps) {
   const long n = ctx->n, n2 = n * n, n3 = n2 * n;
   const long ov = (long)o * v, N = ov * ov, on = (long)o * n, nov = n * ov;
   const T one = T(1), zero = T(0);
@@ -302,7 +303,20 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
     gemm<T>(ctx, false, true, on * n, n, v, one, P2.as<T>(), on * n, 0, Cv, n, 0, zero, Q.as<T>(), on * n, 0, 1,
             "mp2grad_back_q");
     P2.release();
-    gemm<T>(ctx, false, false, n, n3, o, one, Co, n, 0, Q.as<T>(), o, 0, zero, dg, n, 0, 1, "mp2grad_back_dg");
+    if constexpr (sizeof(T) == sizeof(double)) {
+      gemm<double>(ctx, false, false, n, n3, o, 1.0, Co, n, 0, Q.as<double>(), o, 0, 0.0, dg, n, 0, 1,
+                   "mp2grad_back_dg");
+    } else {
+      // the last product writes the n^4 result; with K = o it is bound by that write, so it runs in FP64 on
+      // FP64 copies of Q and Co and writes dE/dg once, in the output type (an FP32 product and a
+      // conversion would write and re-read an FP32 copy of the n^4 tensor)
+      DevBuf Q64(on * n2 * sizeof(double)), Co64((long)n * o * sizeof(double));
+      convert<float, double>(ctx, Q.as<float>(), Q64.as<double>(), on * n2);
+      convert<float, double>(ctx, Co, Co64.as<double>(), (long)n * o);
+      Q.release();
+      gemm<double>(ctx, false, false, n, n3, o, 1.0, Co64.as<double>(), n, 0, Q64.as<double>(), o, 0, 0.0, dg, n,
+                   0, 1, "mp2grad_back_dg");
+    }
   }
 }
 
@@ -329,11 +343,12 @@ extern "C" jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, 
     using T = decltype(zero);
     DeviceArray<T> c(ctx, C, nn);
     DevBuf dC(nn * sizeof(T));
-    DevBuf dg(dE_dg ? n4 * sizeof(T) : 0);
+    DevBuf dg(dE_dg ? n4 * sizeof(double) : 0);  // dE/dg is formed in FP64 in both forms
     if (factor_route)
       mp2_grad_factors_dev<T>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<T>(), deps);
     else
-      mp2_grad_dev<T>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<T>(), dE_dg ? dg.as<T>() : nullptr, deps);
+      mp2_grad_dev<T>(ctx, c.p, c.p + (size_t)n * o, o, v, e.p, E, dC.as<T>(), dE_dg ? dg.as<double>() : nullptr,
+                      deps);
     // results back in FP64 (an FP32 stage converts on the device first)
     auto fetch = [&](double* host, const DevBuf& dev, size_t count) {
       if constexpr (sizeof(T) == sizeof(double)) {
@@ -345,7 +360,7 @@ extern "C" jues_status jues_rmp2_grad(jues_ctx* ctx, int prec, const double* C, 
       }
     };
     if (dE_dC) fetch(dE_dC, dC, nn);
-    if (dE_dg) fetch(dE_dg, dg, n4);
+    if (dE_dg) d2h(ctx, dE_dg, dg.p, n4 * sizeof(double));
   };
   if (prec == JUES_FP64) run(0.0);
   else run(0.0f);
