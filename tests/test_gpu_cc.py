@@ -105,3 +105,28 @@ def test_rccsd_fp32_mixed(cfg1):
     w = Wfn.from_synthetic(s, dense=False)
     E = CC.mRCCSD.do_rccsd(w, prec="fp32_mixed", cc_e_conv=1e-12, cc_max_rms=1e-11, maxit=300)
     assert abs(E - Eref) < 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32_mixed", 1e-5)])
+def test_rccsd_steps_factored_batched_short_products(prec, tol):
+    """Two Jacobi steps of factored RCCSD at n=96, o=32, v=64, naux=800 against the oracle's
+    mRCCSD.jl update (:118-241).  The batched W2[i,f,P] = sum_(em) T[i,(em),f] LovT[(em),P] product
+    (M = o = 32, K = ov = 2048, 64 batches of 800 columns) takes the skinny-M tile through the
+    batched short-product route of the GEMM dispatch; the cfg1 and cfg2 systems are too small for it."""
+    from jues import synthetic
+    from jues.CoupledCluster import Solver
+    from jues.Wavefunction import Wfn, FactoredERI
+    o, v = 32, 64
+    s = synthetic.make_system(o + v, o, v, 800, 11, name="w2")
+    B = np.asfortranarray(0.3 * s.B)
+    I = O.rccsd_integrals_from_factors(B, s.Co, s.Cv)
+    Dia, Dijab = O.form_Dia(s.eps, o, v), O.form_Dijab(s.eps, o, v)
+    t1, t2 = np.zeros((o, v)), I["oovv"] / Dijab
+    sv = Solver(Wfn.from_orbitals(s.C, s.eps, o, FactoredERI(B)), method="rccsd", prec=prec, guess="mp2", diis=False)
+    for _ in range(2):
+        n1, n2 = O.rccsd_update(t1, t2, I)
+        t1, t2 = n1 / Dia, n2 / Dijab
+        sv.iterate(1, prec=prec)
+        _, T1, T2 = sv.result()
+        assert np.max(np.abs(T1 - t1)) < tol * np.max(np.abs(t1))
+        assert np.max(np.abs(T2 - t2)) < tol * np.max(np.abs(t2))
