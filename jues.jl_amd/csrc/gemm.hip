@@ -301,7 +301,8 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
 // n-contiguous (NT), 128x128 tile, 4 wavefronts of 64x64.
 // The K slice is 16 deep, so a workgroup needs 32 KB of LDS and at most 128 VGPRs per
 // lane: four workgroups (16 wavefronts) share a CU and cover each other's load and
-// barrier latency (two workgroups of 32-deep slices reach 73% MFMA-busy, this form ~88%).
+// barrier latency (two workgroups of 32-deep slices reach 73% MFMA-busy, this form ~93% with the
+// last slice peeled off the main loop, ~89% with a conditional loop body).
 // LDS holds each slice as 64-byte rows of 16 k-values per m (A) or n (B); the 16-byte
 // chunk q of row r sits at chunk q ^ ((r >> 2) & 3), so slices are written with 16-byte
 // stores and fragments read with 16-byte loads.  Lane l of a 16x16x4 MFMA supplies
@@ -309,7 +310,9 @@ __global__ __launch_bounds__(256, (BM == 128 && BN == 128) ? 2 : 1) void gemm_ke
 // 16-byte read per 16-row block.  B's four fragment blocks are held for the slice and A's
 // are streamed one block ahead of the MFMAs that use them.  Rows of the last tile row or
 // column beyond M or N load a clamped row and are not stored.  A K tail (16 not dividing K)
-// is the first slice: zero-padded, loaded element by element once in the prologue.
+// is the first slice: zero-padded, loaded element by element once in the prologue.  TA (A
+// k-contiguous, TN / TT) stages A like a k-contiguous B; SEG instantiations add a second contraction
+// segment (A1, B1 over K1 columns).
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
