@@ -3,8 +3,9 @@
 //
 // Dense route: four quarter transforms, each one MFMA GEMM over the full AO
 // tensor.  The outer index (mu or sigma) with the smaller target dimension is
-// contracted first, then the other outer index, then one tiled permutation
-// brings nu and lambda to the outside for the last two quarters.  Factor route:
+// contracted first, then the other outer index; the last two quarters contract
+// nu and lambda in place as batched products (the smaller of d2, d3 first), so
+// the result leaves the last GEMM in [p, q, r, s] order.  Factor route:
 // (pq|rs) = sum_P (C1^T B_P C2)_pq (C3^T B_P C4)_rs, two batched GEMMs per side
 // and one GEMM over P; the n^4 AO tensor is never formed.
 #include "jues_internal.h"
@@ -90,23 +91,26 @@ void tei_transform_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, 
       gemm<T>(ctx, false, false, n3, d4, n, T(1), G, n3, 0, C4, n, 0, T(0), Y1, n3, 0, 1, "xform_q1");
       gemm<T>(ctx, true, false, d1, n2 * d4, n, T(1), C1, n, 0, Y1, n, 0, T(0), X2, d1, 0, 1, "xform_q2");
     }
-    // X2[p, nu, lam, s] -> X3[nu, p, s, lam]
-    T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)(d1 * n2 * d4)));
-    permute<T, T>(ctx, X2, {d1, n, n, d4}, {1, 0, 3, 2}, X3, 1.0, 0.0, "xform_perm");
-    T* X5 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)((long)d1 * d2 * d3 * d4)));
-    const long pd = (long)d1 * d4;
-    if (d2 <= d3) {
-      T* X4 = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(d2 * pd * n)));
-      gemm<T>(ctx, true, false, d2, pd * n, n, T(1), C2, n, 0, X3, n, 0, T(0), X4, d2, 0, 1, "xform_q3");
-      gemm<T>(ctx, false, false, d2 * pd, d3, n, T(1), X4, d2 * pd, 0, C3, n, 0, T(0), X5, d2 * pd, 0, 1,
+    // X2[p, nu, lam, s]: the last two quarters contract nu and lam where they lie, as batched
+    // products, so that the result is written in [p, q, r, s] order with no permutation pass
+    const long d12 = (long)d1 * d2;
+    if (d3 <= d2) {
+      // X3[(p,nu), r, s] = sum_lam X2[(p,nu), lam, s] C3[lam, r]      (batched over s)
+      // out[p, q, (r,s)] = sum_nu X3[p, nu, (r,s)] C2[nu, q]         (batched over (r,s))
+      T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)((long)d1 * n * d3 * d4)));
+      gemm<T>(ctx, false, false, (long)d1 * n, d3, n, T(1), X2, (long)d1 * n, (long)d1 * n2, C3, n, 0, T(0), X3,
+              (long)d1 * n, (long)d1 * n * d3, d4, "xform_q3");
+      gemm<T>(ctx, false, false, d1, d2, n, T(1), X3, d1, (long)d1 * n, C2, n, 0, T(0), out, d1, d12, d3 * d4,
               "xform_q4");
     } else {
-      T* Y = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(n * pd * d3)));
-      gemm<T>(ctx, false, false, n * pd, d3, n, T(1), X3, n * pd, 0, C3, n, 0, T(0), Y, n * pd, 0, 1, "xform_q3");
-      gemm<T>(ctx, true, false, d2, pd * d3, n, T(1), C2, n, 0, Y, n, 0, T(0), X5, d2, 0, 1, "xform_q4");
+      // X3[p, q, (lam,s)] = sum_nu X2[p, nu, (lam,s)] C2[nu, q]      (batched over (lam,s))
+      // out[(p,q), r, s] = sum_lam X3[(p,q), lam, s] C3[lam, r]      (batched over s)
+      T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)(d12 * n * d4)));
+      gemm<T>(ctx, false, false, d1, d2, n, T(1), X2, d1, (long)d1 * n, C2, n, 0, T(0), X3, d1, d12, (int)(n * d4),
+              "xform_q3");
+      gemm<T>(ctx, false, false, d12, d3, n, T(1), X3, d12, d12 * n, C3, n, 0, T(0), out, d12, d12 * d3, d4,
+              "xform_q4");
     }
-    // X5[q, p, s, r] -> out[p, q, r, s]
-    permute<T, T>(ctx, X5, {d2, d1, d4, d3}, {1, 0, 3, 2}, out, 1.0, 0.0, "xform_perm");
   } else {
     const long naux = ctx->naux;
     JUES_REQUIRE(ctx->Bfac.p, "no AO integrals set");

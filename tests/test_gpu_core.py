@@ -65,6 +65,28 @@ def test_tei_transform_general_tensor(ctx):
     assert rel_err(out, ref) < 1e-12
 
 
+def test_tei_transform_dense_n144(ctx):
+    """Dense route at n = 144 (K = 144 > 128, a multiple of 16), where the quarter products run on
+    the tall and the short batched kernels, with four coefficient blocks of no symmetry: both orders
+    of the first two quarters (d1 <= d4 and d1 > d4) and of the last two (d3 <= d2 and d3 > d2), in
+    FP64 and FP32-mixed.  The dense tensor is g = sum_P B_P (x) B_P with B_P not symmetric in
+    (mu, nu), so the oracle's exact 3-index route gives the reference in seconds at this size."""
+    from jues import Transformation
+    n, o, v, naux = 144, 24, 120, 48
+    rng = np.random.default_rng(144)
+    B = rng.standard_normal((n, n, naux))
+    B2 = B.reshape((n * n, naux), order="F")
+    g = (B2 @ B2.T).T.reshape((n,) * 4, order="F")    # symmetric n^2 x n^2: its transpose is the same data
+    Co, Cv = rng.normal(size=(n, o)), rng.normal(size=(n, v))
+    Co2, Cv2 = rng.normal(size=(n, o)), rng.normal(size=(n, v))
+    for Cs in [(Co, Cv, Co2, Cv2), (Co, Co2, Cv, Cv2), (Cv, Co, Cv2, Co2), (Cv, Cv2, Co, Co2)]:
+        ref = O.mo_from_factors(B, *Cs)
+        for prec, tol in [("fp64", 1e-12), ("fp32", 2e-5)]:
+            out = Transformation.tei_transform(g, *Cs, prec=prec)
+            print(f"dense n=144 dims={[c.shape[1] for c in Cs]} {prec}: rel err {rel_err(out, ref):.2e}", flush=True)
+            assert rel_err(out, ref) < tol, ([c.shape[1] for c in Cs], prec)
+
+
 @pytest.mark.parametrize("dims", [(3, 4, 5, 2), (4, 3, 2, 5), (6, 6, 6, 6)])
 def test_tei_transform_nonsymmetric_factors(ctx, dims):
     """Factor route with B[mu,nu,P] != B[nu,mu,P]: both contraction orders of the
