@@ -848,30 +848,36 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
   const int am = lane, ac = __builtin_amdgcn_readfirstlane(wn);
   const bool a_share = lane < BM;
   const int bn = tid >> 2, bc = tid & 3;
+  // A (at most 64 rows, L2-resident) is loaded one slice ahead.  B streams from HBM: for IM <= 3
+  // it is loaded two slices ahead into two register stages, so that its loads have two slices of
+  // MFMA work to arrive; IM = 4 has the registers for one stage.
+  constexpr int NSB = IM <= 3 ? 2 : 1;
   float ra[4];
-  f4v rb[4];
+  f4v rb[NSB][4];
   const int a_voff = 4 * (min(m0 + am, M - 1) - m0);
   const int nlim = N - 1 - n0, ldbi = (int)ldb;
   const float* At = A + m0;
   const float* Bt = B + (long)n0 * ldb;
-  auto load = [&](int k0) {
+  auto load_a = [&](int k0) {
     if (a_share) {
       const auto rA = make_rsrc(At + (long)k0 * lda);
 #pragma unroll
       for (int u = 0; u < 4; ++u) ra[u] = bload<float>(rA, a_voff, 4 * (4 * ac + u) * (int)lda);
     }
+  };
+  auto load_b = [&](int k0, f4v (&r)[4]) {
     const auto rB = make_rsrc(Bt + k0);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      rb[j] = __builtin_bit_cast(
+      r[j] = __builtin_bit_cast(
           f4v, __builtin_amdgcn_raw_buffer_load_b128(rB, 4 * (4 * bc + min(bn + 64 * j, nlim) * ldbi), 0, 0));
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, const f4v (&r)[4]) {
     if (a_share) *reinterpret_cast<f4v*>(&As[buf][am * BK + 4 * g32_chunk(ac, am)]) = f4v{ra[0], ra[1], ra[2], ra[3]};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int n = bn + 64 * j;
-      *reinterpret_cast<f4v*>(&Bs[buf][n * BK + 4 * g32_chunk(bc, n)]) = rb[j];
+      *reinterpret_cast<f4v*>(&Bs[buf][n * BK + 4 * g32_chunk(bc, n)]) = r[j];
     }
   };
 
@@ -882,10 +888,6 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
     for (int j = 0; j < 4; ++j) acc[i][j] = fvec4{0, 0, 0, 0};
   const int fr = lane & 15, fk = lane >> 4;
   const int nk = K / BK;
-  load(0);
-  store(0);
-  __syncthreads();
-  // the last slice is peeled off, so that the loop body is one block without conditions
   auto compute = [&](int cur) {
     f4v bf[4];
 #pragma unroll
@@ -904,14 +906,60 @@ __global__ __launch_bounds__(256, 4) void gemm32_sk_kernel(GemmSide<float> g, fl
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j][s2], a0[s2], acc[i][j], 0, 0, 0);
     }
   };
-  for (int kt = 0; kt + 1 < nk; ++kt) {
-    const int cur = kt & 1;
-    load((kt + 1) * BK);
-    compute(cur);
-    store(cur ^ 1);
+  if constexpr (NSB == 2) {
+    // On entry to slice t (t even in the loop): LDS buffer 0 holds slice t, rb[1] holds slice
+    // t + 1 (in flight) and rb[0] is free.  A is issued before B each time, so that waiting for A
+    // leaves the newer B loads in flight.  Two slices per trip, so that the stages are named
+    // statically; the last one to three slices are peeled off, so that the loop body has no
+    // conditions.
+    load_a(0);
+    load_b(0, rb[0]);
+    if (nk > 1) load_b(BK, rb[1]);
+    store(0, rb[0]);
     __syncthreads();
+    int t = 0;
+    for (; t + 3 < nk; t += 2) {
+      load_a((t + 1) * BK);
+      load_b((t + 2) * BK, rb[0]);
+      compute(0);
+      store(1, rb[1]);
+      __syncthreads();
+      load_a((t + 2) * BK);
+      load_b((t + 3) * BK, rb[1]);
+      compute(1);
+      store(0, rb[0]);
+      __syncthreads();
+    }
+    if (t + 1 < nk) load_a((t + 1) * BK);
+    if (t + 2 < nk) load_b((t + 2) * BK, rb[0]);
+    compute(0);
+    if (t + 1 < nk) {
+      store(1, rb[1]);
+      __syncthreads();
+      if (t + 2 < nk) load_a((t + 2) * BK);
+      compute(1);
+      if (t + 2 < nk) {
+        store(0, rb[0]);
+        __syncthreads();
+        compute(0);
+      }
+    }
+  } else {
+    load_a(0);
+    load_b(0, rb[0]);
+    store(0, rb[0]);
+    __syncthreads();
+    // the last slice is peeled off, so that the loop body is one block without conditions
+    for (int kt = 0; kt + 1 < nk; ++kt) {
+      const int cur = kt & 1;
+      load_a((kt + 1) * BK);
+      load_b((kt + 1) * BK, rb[0]);
+      compute(cur);
+      store(cur ^ 1, rb[0]);
+      __syncthreads();
+    }
+    compute((nk - 1) & 1);
   }
-  compute((nk - 1) & 1);
 
 #pragma unroll
   for (int i = 0; i < IM; ++i) {
@@ -956,32 +1004,38 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
   const int am = lane, ka = __builtin_amdgcn_readfirstlane(wn);
   const bool a_share = lane < BM;
   const int bn = tid >> 3, bc = tid & 7;
+  // A (at most 64 rows, L2-resident) is loaded one slice ahead.  B streams from HBM: for IM <= 3
+  // it is loaded two slices ahead into two register stages, so that its loads have two slices of
+  // MFMA work to arrive (K is short in these products); IM = 4 has the registers for one stage.
+  constexpr int NSB = IM <= 3 ? 2 : 1;
   double ra[2];
-  d2v rb[4];
+  d2v rb[NSB][4];
   const int a_voff = 8 * (min(m0 + am, M - 1) - m0);
   // B rows past N load row N - 1 (uniform limit; the offsets are recomputed per load rather
   // than held in registers)
   const int nlim = N - 1 - n0, ldbi = (int)ldb;
   const double* At = A + m0;
   const double* Bt = B + (long)n0 * ldb;
-  auto load = [&](int k0) {
+  auto load_a = [&](int k0) {
     if (a_share) {
       const auto rA = make_rsrc(At + (long)k0 * lda);
 #pragma unroll
       for (int u = 0; u < 2; ++u) ra[u] = bload<double>(rA, a_voff, 8 * (2 * ka + u) * (int)lda);
     }
+  };
+  auto load_b = [&](int k0, d2v (&r)[4]) {
     const auto rB = make_rsrc(Bt + k0);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      rb[j] = __builtin_bit_cast(
+      r[j] = __builtin_bit_cast(
           d2v, __builtin_amdgcn_raw_buffer_load_b128(rB, 8 * (2 * bc + min(bn + 64 * j, nlim) * ldbi), 0, 0));
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, const d2v (&r)[4]) {
     if (a_share) *reinterpret_cast<d2v*>(&As[buf][am * BK + 2 * g64_chunk(ka, am)]) = d2v{ra[0], ra[1]};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int n = bn + 64 * j;
-      *reinterpret_cast<d2v*>(&Bs[buf][n * BK + 2 * g64_chunk(bc, n)]) = rb[j];
+      *reinterpret_cast<d2v*>(&Bs[buf][n * BK + 2 * g64_chunk(bc, n)]) = r[j];
     }
   };
 
@@ -992,10 +1046,6 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
     for (int j = 0; j < 2; ++j) acc[i][j] = dvec4{0, 0, 0, 0};
   const int fr = lane & 15, fk = lane >> 4;
   const int nk = K / BK;
-  load(0);
-  store(0);
-  __syncthreads();
-  // the last slice is peeled off, so that the loop body is one block without conditions
   auto compute = [&](int cur) {
     d2v bf[2][2];
 #pragma unroll
@@ -1020,14 +1070,60 @@ __global__ __launch_bounds__(512, 4) void gemm64_sk_kernel(GemmSide<double> g, d
                                                            0, 0, 0);
     }
   };
-  for (int kt = 0; kt + 1 < nk; ++kt) {
-    const int cur = kt & 1;
-    load((kt + 1) * BK);
-    compute(cur);
-    store(cur ^ 1);
+  if constexpr (NSB == 2) {
+    // On entry to slice t (t even in the loop): LDS buffer 0 holds slice t, rb[1] holds slice
+    // t + 1 (in flight) and rb[0] is free.  A is issued before B each time, so that waiting for A
+    // leaves the newer B loads in flight.  Two slices per trip, so that the stages are named
+    // statically; the last one to three slices are peeled off, so that the loop body has no
+    // conditions.
+    load_a(0);
+    load_b(0, rb[0]);
+    if (nk > 1) load_b(BK, rb[1]);
+    store(0, rb[0]);
     __syncthreads();
+    int t = 0;
+    for (; t + 3 < nk; t += 2) {
+      load_a((t + 1) * BK);
+      load_b((t + 2) * BK, rb[0]);
+      compute(0);
+      store(1, rb[1]);
+      __syncthreads();
+      load_a((t + 2) * BK);
+      load_b((t + 3) * BK, rb[1]);
+      compute(1);
+      store(0, rb[0]);
+      __syncthreads();
+    }
+    if (t + 1 < nk) load_a((t + 1) * BK);
+    if (t + 2 < nk) load_b((t + 2) * BK, rb[0]);
+    compute(0);
+    if (t + 1 < nk) {
+      store(1, rb[1]);
+      __syncthreads();
+      if (t + 2 < nk) load_a((t + 2) * BK);
+      compute(1);
+      if (t + 2 < nk) {
+        store(0, rb[0]);
+        __syncthreads();
+        compute(0);
+      }
+    }
+  } else {
+    load_a(0);
+    load_b(0, rb[0]);
+    store(0, rb[0]);
+    __syncthreads();
+    // the last slice is peeled off, so that the loop body is one block without conditions
+    for (int kt = 0; kt + 1 < nk; ++kt) {
+      const int cur = kt & 1;
+      load_a((kt + 1) * BK);
+      load_b((kt + 1) * BK, rb[0]);
+      compute(cur);
+      store(cur ^ 1, rb[0]);
+      __syncthreads();
+    }
+    compute((nk - 1) & 1);
   }
-  compute((nk - 1) & 1);
 
   // acc[i][j][r]: C(m, n) with m = 16 i + lane % 16, n = n0 + 32 wn + 16 j + lane / 16 + 4 r
 #pragma unroll

@@ -17,7 +17,9 @@ pytestmark = pytest.mark.gpu
                                    (40, 9000, 256), (64, 8192, 33), (9000, 40, 300), (8195, 17, 64),
                                    # short, wide FP64 products: the 64x256 kernel, a transposed A copied first
                                    (17, 9000, 64), (33, 10000, 96), (64, 16400, 512), (1, 8192, 16),
-                                   (33, 9001, 272), (49, 8200, 160)])
+                                   (33, 9001, 272), (49, 8200, 160),
+                                   # 3, 5 and 11 sixteen-deep k-slices: the short kernels' pipeline tails
+                                   (20, 8300, 48), (45, 8500, 80), (48, 8200, 176)])
 def test_gemm(ctx, prec, tol, ta, tb, M, N, K):
     rng = np.random.default_rng(M * 7 + N * 3 + K)
     A = rng.normal(size=(K, M) if ta == "T" else (M, K))
