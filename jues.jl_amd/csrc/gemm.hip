@@ -1671,6 +1671,19 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
 }
 
 template <typename T>
+void gemm_batch_sum(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda, long sA,
+                    const T* B, long ldb, long sB, T beta, T* C, long ldc, int batch, T* work, const char* tag) {
+  JUES_REQUIRE(batch >= 1 && M > 0 && N > 0, "gemm_batch_sum: empty product");
+  const char* name = tag ? tag : (sizeof(T) == 8 ? "gemm_f64" : "gemm_f32");
+  gemm_impl<T>(ctx, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, T(0), work, M, M * N, batch, tag, false);
+  KTimer t(ctx, name, 0.0, (double)sizeof(T) * ((double)batch + (beta != T(0) ? 2.0 : 1.0)) * M * N);
+  const long tot = M * N;
+  hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)std::max(1L, std::min((tot + 255) / 256, 65536L))),
+                     dim3(256), 0, ctx->stream, work, batch, M, N, C, ldc, beta);
+  JUES_HIP(hipGetLastError());
+}
+
+template <typename T>
 void gemm_pair(jues_ctx* ctx, bool tb, long M0, long N0, long K0, const T* A0, long lda0, const T* B0, long ldb0,
                T* C0, long ldc0, long M1, long N1, long K1, const T* A1, long lda1, const T* B1, long ldb1, T* C1,
                long ldc1, T alpha, T beta, const char* tag) {
@@ -1767,6 +1780,10 @@ template void gemm_pair<float>(jues_ctx*, bool, long, long, long, const float*, 
                                long, long, long, const float*, long, const float*, long, float*, long, float, float,
                                const char*);
 
+template void gemm_batch_sum<double>(jues_ctx*, bool, bool, long, long, long, double, const double*, long, long,
+                                     const double*, long, long, double, double*, long, int, double*, const char*);
+template void gemm_batch_sum<float>(jues_ctx*, bool, bool, long, long, long, float, const float*, long, long,
+                                    const float*, long, long, float, float*, long, int, float*, const char*);
 template void gemm<double>(jues_ctx*, bool, bool, long, long, long, double, const double*, long, long,
                            const double*, long, long, double, double*, long, long, int, const char*);
 template void gemm<float>(jues_ctx*, bool, bool, long, long, long, float, const float*, long, long,

@@ -117,6 +117,14 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
           long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch = 1,
           const char* tag = nullptr);
 
+// C = alpha sum_z op(A_z) op(B_z) + beta C over a batch (strides sA, sB): the batch's products are
+// written to work (batch * M * N elements) and added in a fixed order, so the result is the same
+// on every call.
+template <typename T>
+void gemm_batch_sum(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda, long sA,
+                    const T* B, long ldb, long sB, T beta, T* C, long ldc, int batch, T* work,
+                    const char* tag = nullptr);
+
 // Two independent products C_i = alpha A_i op(B_i) + beta C_i (A_i m-contiguous, op the same
 // for both) in one launch, so that the second fills the first one's last partial round of
 // workgroups; falls back to two gemm calls where the high-occupancy kernels do not apply.

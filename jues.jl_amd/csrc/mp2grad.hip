@@ -203,13 +203,13 @@ void mp2_grad_factors_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v,
 // E depends on C at four positions of (ia|jb) = sum g[mu,nu,lam,sig] Co[mu,i] Cv[nu,a] Co[lam,j]
 // Cv[sig,b].  The forward transform runs here with two of its intermediates kept:
 //   Z1[i,nu,lam,sig] = sum_mu Co[mu,i] g            X2[i,nu,lam,b] = sum_sig Z1 Cv[sig,b]
-//   X3[nu,i,b,lam]   = X2 permuted                  Y3[nu,i,b,j]   = sum_lam X3 Co[lam,j]
-//   K[i,a,j,b] = (X5[a,i,b,j] = sum_nu Cv[nu,a] Y3) permuted
-// X3 (positions 1 and 4 transformed) gives dE/dC at position 3 and Y3 (positions 1, 3, 4) at
-// position 2; positions 1 and 4 take one more half-transform, Y[mu,a,j,sig] = (mu a|j sig):
+//   X3[i,nu,j,b]     = sum_lam X2 Co[lam,j]         K[i,a,j,b]     = sum_nu X3 Cv[nu,a]
+// (the last two batched, so nothing n^2-sized is permuted).  X2 (positions 1 and 4 transformed)
+// gives dE/dC at position 3 and Y3[nu,i,b,j] = X3 permuted (positions 1, 3, 4) at position 2;
+// positions 1 and 4 take one more half-transform, Y[mu,a,j,sig] = (mu a|j sig):
 //   pos 1: dCo[mu,i]  += sum_{a,j,sig} Y[mu,a,j,sig] T1[i,a,j,sig],  T1 = sum_b G[i,a,j,b] Cv[sig,b]
 //   pos 2: dCv[nu,a]   = sum_{i,b,j} Y3[nu,i,b,j] G[i,a,j,b]
-//   pos 3: dCo[lam,j]  = sum_{nu,i,b} X3[nu,i,b,lam] Gt[i,nu,j,b],  Gt = sum_a G[i,a,j,b] Cv[nu,a]
+//   pos 3: dCo[lam,j]  = sum_b sum_{i,nu} X2[i,nu,lam,b] Gt[i,nu,j,b],  Gt = sum_a G[i,a,j,b] Cv[nu,a]
 //   pos 4: dCv[sig,b] += sum_{i,a,j} Yt[i,a,j,sig] G[i,a,j,b],     Yt = sum_mu Co[mu,i] Y
 // dg (FP64 in both forms) may be null.
 template <typename T>
@@ -221,27 +221,27 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
   const T* g = ao_dense<T>(ctx);
   T* dCo = dC;           // n x o  (columns 0..o-1 of dC, leading dimension n)
   T* dCv = dC + n * o;   // n x v
-  DevBuf Gb(N * sizeof(T)), X3b(nov * n * sizeof(T)), Y3b(nov * o * sizeof(T));
+  DevBuf Gb(N * sizeof(T)), X2b(nov * n * sizeof(T)), Y3b(nov * o * sizeof(T));
   T* G = Gb.as<T>();
-  T* X3 = X3b.as<T>();
+  T* X2 = X2b.as<T>();
   T* Y3 = Y3b.as<T>();
   {
-    // forward (ia|jb), the same four quarter-transforms as the (ia|jb) transform
-    DevBuf X2(nov * n * sizeof(T));
+    // forward (ia|jb), the same products as the (ia|jb) transform: X2[i,nu,lam,b] = (i nu|lam b)
+    // (kept for position 3), then lam and nu contracted in place, batched, to X3[i,nu,j,b] =
+    // (i nu|j b) and K[i,a,j,b]
     {
       DevBuf Z1(o * n3 * sizeof(T));
       gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z1.as<T>(), o, 0, 1, "xform_q1");
-      gemm<T>(ctx, false, false, o * n2, v, n, one, Z1.as<T>(), o * n2, 0, Cv, n, 0, zero, X2.as<T>(), o * n2, 0,
-              1, "xform_q2");
+      gemm<T>(ctx, false, false, o * n2, v, n, one, Z1.as<T>(), o * n2, 0, Cv, n, 0, zero, X2, o * n2, 0, 1,
+              "xform_q2");
     }
-    permute<T, T>(ctx, X2.as<T>(), {(long)o, n, n, (long)v}, {1, 0, 3, 2}, X3, 1.0, 0.0, "xform_perm");
-    X2.release();
-    gemm<T>(ctx, false, false, nov, o, n, one, X3, nov, 0, Co, n, 0, zero, Y3, nov, 0, 1, "xform_q3");
-    DevBuf X5(N * sizeof(T)), Kb(N * sizeof(T));
-    gemm<T>(ctx, true, false, v, ov * o, n, one, Cv, n, 0, Y3, n, 0, zero, X5.as<T>(), v, 0, 1, "xform_q4");
-    permute<T, T>(ctx, X5.as<T>(), {(long)v, (long)o, (long)v, (long)o}, {1, 0, 3, 2}, Kb.as<T>(), 1.0, 0.0,
-                  "xform_perm");
-    X5.release();
+    DevBuf X3(nov * o * sizeof(T)), Kb(N * sizeof(T));
+    gemm<T>(ctx, false, false, on, o, n, one, X2, on, on * n, Co, n, 0, zero, X3.as<T>(), on, on * o, v, "xform_q3");
+    gemm<T>(ctx, false, false, o, v, n, one, X3.as<T>(), o, on, Cv, n, 0, zero, Kb.as<T>(), o, ov, (int)(o * v),
+            "xform_q4");
+    // Y3[nu,i,b,j] = (i nu|j b), position 2's operand (an n o^2 v permutation, not n^2 o v)
+    permute<T, T>(ctx, X3.as<T>(), {(long)o, n, (long)o, (long)v}, {1, 0, 3, 2}, Y3, 1.0, 0.0, "mp2grad_s2");
+    X3.release();
     mp2_g<T>(ctx, Kb.as<T>(), eps, o, v, G, E, deps);
   }
   // position 2: Gp2[i,b,j,a] = G[i,a,j,b]; dCv = Y3[nu,(i,b,j)] Gp2[(i,b,j),a]
@@ -251,17 +251,16 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
     gemm<T>(ctx, false, false, n, v, ov * o, one, Y3, n, 0, Gp2.as<T>(), ov * o, 0, zero, dCv, n, 0, 1, "mp2grad_s2");
   }
   Y3b.release();
-  // position 3: Gt[i,nu,j,b] (batched over jb; also the first step of the dE/dg back-transform),
-  // Gtp[nu,i,b,j] = Gt permuted; dCo = X3[(nu,i,b),lam]^T Gtp[(nu,i,b),j]
+  // position 3: Gt[i,nu,j,b] (batched over jb; also the first step of the dE/dg back-transform);
+  // dCo[lam,j] = sum_b X2[(i,nu),lam,b]^T Gt[(i,nu),j,b], batched over b and summed in a fixed order
   DevBuf Gt(on * ov * sizeof(T));
   gemm<T>(ctx, false, true, o, n, v, one, G, o, ov, Cv, n, 0, zero, Gt.as<T>(), o, on, (int)ov, "mp2grad_s3");
   {
-    DevBuf Gtp(on * ov * sizeof(T));
-    permute<T, T>(ctx, Gt.as<T>(), {(long)o, n, (long)o, (long)v}, {1, 0, 3, 2}, Gtp.as<T>(), 1.0, 0.0,
-                  "mp2grad_s3");
-    gemm<T>(ctx, true, false, n, o, nov, one, X3, nov, 0, Gtp.as<T>(), nov, 0, zero, dCo, n, 0, 1, "mp2grad_s3");
+    DevBuf W(n * ov * sizeof(T));
+    gemm_batch_sum<T>(ctx, true, false, n, o, on, one, X2, on, on * n, Gt.as<T>(), on, on * o, zero, dCo, n, v,
+                      W.as<T>(), "mp2grad_s3");
   }
-  X3b.release();
+  X2b.release();
   // Y[mu,a,j,sig] = (mu a|j sig): X[mu,nu,j,sig] = sum_lam g[mu,nu,lam,sig] Co[lam,j] (batched over sig),
   // then Y[mu,a,(j,sig)] = sum_nu X[mu,nu,(j,sig)] Cv[nu,a] (batched over (j,sig))
   DevBuf Yb(nov * n * sizeof(T));

@@ -27,6 +27,14 @@ def gemm(ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch
         c[...] = alpha * (a @ b) + (beta * c if beta != 0 else 0.0)
 
 
+def gemm_batch_sum(ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, batch):
+    """C = alpha sum_z op(A_z) op(B_z) + beta C: the products to a workspace, then summed over z."""
+    W = np.zeros(batch * M * N)
+    gemm(ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, 0.0, W, M, M * N, batch)
+    c = _mat(C, 0, M, N, ldc)
+    c[...] = W.reshape((M, N, batch), order="F").sum(axis=2) + (beta * c if beta != 0 else 0.0)
+
+
 def permute(src, dims, perm, dst, alpha=1.0, beta=0.0):
     """dst = alpha permutedims(src, perm) + beta dst (out dim k = in dim perm[k]), column-major."""
     x = src[:int(np.prod(dims))].reshape(dims, order="F").transpose(perm)
@@ -52,14 +60,14 @@ def device_sequence(g, C, eps, o, pos4_stride_error=0):
     gf = g.ravel(order="F")
     Co, Cv = C[:, :o].ravel(order="F").copy(), C[:, o:].ravel(order="F").copy()
     z = np.zeros
-    # forward (ia|jb) with X3 = (i nu|lam b) as [nu,i,b,lam] and Y3 = (i nu|j b) as [nu,i,b,j] kept
-    Z1, X2, X3, Y3, X5, K = z(o * n3), z(nov * n), z(nov * n), z(nov * o), z(N), z(N)
+    # forward (ia|jb) with X2 = (i nu|lam b) as [i,nu,lam,b] kept, the last two quarters batched
+    # (X3 = (i nu|j b) as [i,nu,j,b]) and Y3 = X3 as [nu,i,b,j] kept
+    Z1, X2, X3, Y3, K = z(o * n3), z(nov * n), z(nov * o), z(nov * o), z(N)
     gemm(1, 0, o, n3, n, 1.0, Co, n, 0, gf, n, 0, 0.0, Z1, o, 0)
     gemm(0, 0, o * n2, v, n, 1.0, Z1, o * n2, 0, Cv, n, 0, 0.0, X2, o * n2, 0)
-    permute(X2, (o, n, n, v), (1, 0, 3, 2), X3)
-    gemm(0, 0, nov, o, n, 1.0, X3, nov, 0, Co, n, 0, 0.0, Y3, nov, 0)
-    gemm(1, 0, v, ov * o, n, 1.0, Cv, n, 0, Y3, n, 0, 0.0, X5, v, 0)
-    permute(X5, (v, o, v, o), (1, 0, 3, 2), K)
+    gemm(0, 0, on, o, n, 1.0, X2, on, on * n, Co, n, 0, 0.0, X3, on, on * o, v)
+    gemm(0, 0, o, v, n, 1.0, X3, o, on, Cv, n, 0, 0.0, K, o, ov, o * v)
+    permute(X3, (o, n, o, v), (1, 0, 3, 2), Y3)
     K4 = K.reshape(o, v, o, v, order="F")
     D = O.form_Dijab(eps, o, v).transpose(0, 2, 1, 3)
     G = (2.0 * (2.0 * K4 - K4.transpose(0, 3, 2, 1)) / D).ravel(order="F")
@@ -69,10 +77,9 @@ def device_sequence(g, C, eps, o, pos4_stride_error=0):
     permute(G, (o, v, o, v), (0, 3, 2, 1), Gp2)
     gemm(0, 0, n, v, ov * o, 1.0, Y3, n, 0, Gp2, ov * o, 0, 0.0, dCv, n, 0)
     # position 3
-    Gt, Gtp = z(on * ov), z(on * ov)
+    Gt = z(on * ov)
     gemm(0, 1, o, n, v, 1.0, G, o, ov, Cv, n, 0, 0.0, Gt, o, on, ov)
-    permute(Gt, (o, n, o, v), (1, 0, 3, 2), Gtp)
-    gemm(1, 0, n, o, nov, 1.0, X3, nov, 0, Gtp, nov, 0, 0.0, dCo, n, 0)
+    gemm_batch_sum(1, 0, n, o, on, 1.0, X2, on, on * n, Gt, on, on * o, 0.0, dCo, n, v)
     # (mu a|j sig): X[mu,nu,j,sig] batched over sig, then Y[mu,a,(j,sig)] batched over (j,sig)
     X, Y = z(n3 * o), z(nov * n)
     gemm(0, 0, n2, o, n, 1.0, gf, n2, n3, Co, n, 0, 0.0, X, n2, n2 * o, n)
