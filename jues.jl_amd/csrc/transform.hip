@@ -1,11 +1,15 @@
 // AO -> MO four-index transformation (Backend/Transformation.jl:39-93) and the
 // RMP2 energy reduction (MollerPlesset/RMP2.jl:27-41).
 //
-// Dense route: four quarter transforms, each one MFMA GEMM over the full AO
-// tensor.  The outer index (mu or sigma) with the smaller target dimension is
-// contracted first, then the other outer index; the last two quarters contract
-// nu and lambda in place as batched products (the smaller of d2, d3 first), so
-// the result leaves the last GEMM in [p, q, r, s] order.  Factor route:
+// Dense route: four quarter transforms, each one MFMA GEMM (plain or batched)
+// over the full AO tensor.  The outer index (mu or sigma) with the smaller target
+// dimension is contracted first.  If the inner index next to the other outer one
+// has a smaller target than it (lambda for mu first, as d3 = o < d4 = v in
+// (ia|jb)), that inner index is contracted second, batched, which makes the
+// second quarter 2 n^3 d1 d3 instead of 2 n^3 d1 d4; otherwise the other outer
+// index is, and the last two quarters contract nu and lambda in place (the
+// smaller of d2, d3 first).  Every order writes the result in [p, q, r, s]
+// order from its last GEMM, with no permutation pass.  Factor route:
 // (pq|rs) = sum_P (C1^T B_P C2)_pq (C3^T B_P C4)_rs, two batched GEMMs per side
 // and one GEMM over P; the n^4 AO tensor is never formed.
 #include "jues_internal.h"
@@ -80,36 +84,69 @@ void tei_transform_dev(jues_ctx* ctx, const T* C1, int d1, const T* C2, int d2, 
   if (ctx->gao.p) {
     const T* G = ao_dense<T>(ctx);
     const long n2 = n * n, n3 = n2 * n;
-    T* X2 = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(d1 * n2 * d4)));
-    if (d1 <= d4) {
+    const long d12 = (long)d1 * d2;
+    if (d1 <= d4 && d3 < d4) {
+      // mu first, then lam (a smaller target than sig's), batched over sig: the second quarter
+      // costs 2 n^3 d1 d3 instead of 2 n^3 d1 d4, and each product writes its result in the
+      // order the next one reads
+      //   X1[p, (nu,lam,sig)] = sum_mu C1[mu, p] g[mu, (nu,lam,sig)]
+      //   X2[(p,nu), r, sig]  = sum_lam X1[(p,nu), lam, sig] C3[lam, r]   (batched over sig)
+      //   X3[(p,nu,r), s]     = sum_sig X2[(p,nu,r), sig] C4[sig, s]
+      //   out[p, q, (r,s)]    = sum_nu X3[p, nu, (r,s)] C2[nu, q]        (batched over (r,s))
+      const long pn = (long)d1 * n, pnr = pn * d3;
       T* X1 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)(d1 * n3)));
       gemm<T>(ctx, true, false, d1, n3, n, T(1), C1, n, 0, G, n, 0, T(0), X1, d1, 0, 1, "xform_q1");
-      gemm<T>(ctx, false, false, d1 * n2, d4, n, T(1), X1, d1 * n2, 0, C4, n, 0, T(0), X2, d1 * n2, 0, 1,
-              "xform_q2");
-    } else {
+      T* X2 = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(pnr * n)));
+      gemm<T>(ctx, false, false, pn, d3, n, T(1), X1, pn, pn * n, C3, n, 0, T(0), X2, pn, pnr, (int)n, "xform_q2");
+      T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)(pnr * d4)));
+      gemm<T>(ctx, false, false, pnr, d4, n, T(1), X2, pnr, 0, C4, n, 0, T(0), X3, pnr, 0, 1, "xform_q3");
+      gemm<T>(ctx, false, false, d1, d2, n, T(1), X3, d1, pn, C2, n, 0, T(0), out, d1, d12, d3 * d4, "xform_q4");
+    } else if (d1 > d4 && d2 < d1) {
+      // the mirror image: sig first, then nu (a smaller target than mu's), batched over (lam,s)
+      //   Y1[(mu,nu,lam), s] = sum_sig g[(mu,nu,lam), sig] C4[sig, s]
+      //   Y2[mu, q, (lam,s)] = sum_nu Y1[mu, nu, (lam,s)] C2[nu, q]      (batched over (lam,s))
+      //   X3[p, (q,lam,s)]   = sum_mu C1[mu, p] Y2[mu, (q,lam,s)]
+      //   out[(p,q), r, s]   = sum_lam X3[(p,q), lam, s] C3[lam, r]      (batched over s)
       T* Y1 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)(n3 * d4)));
       gemm<T>(ctx, false, false, n3, d4, n, T(1), G, n3, 0, C4, n, 0, T(0), Y1, n3, 0, 1, "xform_q1");
-      gemm<T>(ctx, true, false, d1, n2 * d4, n, T(1), C1, n, 0, Y1, n, 0, T(0), X2, d1, 0, 1, "xform_q2");
-    }
-    // X2[p, nu, lam, s]: the last two quarters contract nu and lam where they lie, as batched
-    // products, so that the result is written in [p, q, r, s] order with no permutation pass
-    const long d12 = (long)d1 * d2;
-    if (d3 <= d2) {
-      // X3[(p,nu), r, s] = sum_lam X2[(p,nu), lam, s] C3[lam, r]      (batched over s)
-      // out[p, q, (r,s)] = sum_nu X3[p, nu, (r,s)] C2[nu, q]         (batched over (r,s))
-      T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)((long)d1 * n * d3 * d4)));
-      gemm<T>(ctx, false, false, (long)d1 * n, d3, n, T(1), X2, (long)d1 * n, (long)d1 * n2, C3, n, 0, T(0), X3,
-              (long)d1 * n, (long)d1 * n * d3, d4, "xform_q3");
-      gemm<T>(ctx, false, false, d1, d2, n, T(1), X3, d1, (long)d1 * n, C2, n, 0, T(0), out, d1, d12, d3 * d4,
-              "xform_q4");
-    } else {
-      // X3[p, q, (lam,s)] = sum_nu X2[p, nu, (lam,s)] C2[nu, q]      (batched over (lam,s))
-      // out[(p,q), r, s] = sum_lam X3[(p,q), lam, s] C3[lam, r]      (batched over s)
+      T* Y2 = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(n * d2 * n * d4)));
+      gemm<T>(ctx, false, false, n, d2, n, T(1), Y1, n, n2, C2, n, 0, T(0), Y2, n, n * d2, (int)(n * d4), "xform_q2");
       T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)(d12 * n * d4)));
-      gemm<T>(ctx, false, false, d1, d2, n, T(1), X2, d1, (long)d1 * n, C2, n, 0, T(0), X3, d1, d12, (int)(n * d4),
-              "xform_q3");
+      gemm<T>(ctx, true, false, d1, d2 * n * d4, n, T(1), C1, n, 0, Y2, n, 0, T(0), X3, d1, 0, 1, "xform_q3");
       gemm<T>(ctx, false, false, d12, d3, n, T(1), X3, d12, d12 * n, C3, n, 0, T(0), out, d12, d12 * d3, d4,
               "xform_q4");
+    } else {
+      // the two outer indices first (the one with the smaller target first), to X2[p, nu, lam, s]
+      T* X2 = static_cast<T*>(scratch(ctx, 1, sizeof(T) * (size_t)(d1 * n2 * d4)));
+      if (d1 <= d4) {
+        T* X1 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)(d1 * n3)));
+        gemm<T>(ctx, true, false, d1, n3, n, T(1), C1, n, 0, G, n, 0, T(0), X1, d1, 0, 1, "xform_q1");
+        gemm<T>(ctx, false, false, d1 * n2, d4, n, T(1), X1, d1 * n2, 0, C4, n, 0, T(0), X2, d1 * n2, 0, 1,
+                "xform_q2");
+      } else {
+        T* Y1 = static_cast<T*>(scratch(ctx, 0, sizeof(T) * (size_t)(n3 * d4)));
+        gemm<T>(ctx, false, false, n3, d4, n, T(1), G, n3, 0, C4, n, 0, T(0), Y1, n3, 0, 1, "xform_q1");
+        gemm<T>(ctx, true, false, d1, n2 * d4, n, T(1), C1, n, 0, Y1, n, 0, T(0), X2, d1, 0, 1, "xform_q2");
+      }
+      // then nu and lam where they lie, as batched products, so that the result is written in
+      // [p, q, r, s] order with no permutation pass
+      if (d3 <= d2) {
+        // X3[(p,nu), r, s] = sum_lam X2[(p,nu), lam, s] C3[lam, r]      (batched over s)
+        // out[p, q, (r,s)] = sum_nu X3[p, nu, (r,s)] C2[nu, q]         (batched over (r,s))
+        T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)((long)d1 * n * d3 * d4)));
+        gemm<T>(ctx, false, false, (long)d1 * n, d3, n, T(1), X2, (long)d1 * n, (long)d1 * n2, C3, n, 0, T(0), X3,
+                (long)d1 * n, (long)d1 * n * d3, d4, "xform_q3");
+        gemm<T>(ctx, false, false, d1, d2, n, T(1), X3, d1, (long)d1 * n, C2, n, 0, T(0), out, d1, d12, d3 * d4,
+                "xform_q4");
+      } else {
+        // X3[p, q, (lam,s)] = sum_nu X2[p, nu, (lam,s)] C2[nu, q]      (batched over (lam,s))
+        // out[(p,q), r, s] = sum_lam X3[(p,q), lam, s] C3[lam, r]      (batched over s)
+        T* X3 = static_cast<T*>(scratch(ctx, 2, sizeof(T) * (size_t)(d12 * n * d4)));
+        gemm<T>(ctx, false, false, d1, d2, n, T(1), X2, d1, (long)d1 * n, C2, n, 0, T(0), X3, d1, d12,
+                (int)(n * d4), "xform_q3");
+        gemm<T>(ctx, false, false, d12, d3, n, T(1), X3, d12, d12 * n, C3, n, 0, T(0), out, d12, d12 * d3, d4,
+                "xform_q4");
+      }
     }
   } else {
     const long naux = ctx->naux;
