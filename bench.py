@@ -245,6 +245,27 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def mp2_flop_counts(n: int, o: int, v: int) -> tuple[int, int]:
+    """Flops the dense (ia|jb) transform and the dense MP2 adjoint with dE/dg execute (GEMM products).
+
+    Transform, in the order its dispatch takes for (Co, Cv, Co, Cv): for o < v mu, then lam (both
+    occupied; lam batched over sig), then sig and nu; for o > v the mirror image (sig, nu, mu, lam);
+    for o = v mu, sig, lam, nu.  Adjoint: its forward contracts sig second, because it keeps
+    (i nu|lam b) for position 3 (F_T); then the half-transform (mu a|j sig) (2n^4 o + 2n^3 o v); the four dE/dC positions: three products of
+    2n^2 o^2 v (positions 1 and 3, the occupied quarter of position 4) and four of 2n o^2 v^2; the
+    back-transform to dE/dg, i contracted last (2n^4 o + 2n^3 o v + 2n^2 o^2 v).
+    """
+    if o < v:
+        F_X = 2 * n**4 * o + 2 * n**3 * o**2 + 2 * n**2 * o**2 * v + 2 * n * o**2 * v**2
+    elif o > v:
+        F_X = 2 * n**4 * v + 2 * n**3 * v**2 + 2 * n**2 * o * v**2 + 2 * n * o**2 * v**2
+    else:
+        F_X = 2 * n**4 * o + 2 * n**3 * o * v + 2 * n**2 * o**2 * v + 2 * n * o**2 * v**2
+    F_T = 2 * n**4 * o + 2 * n**3 * o * v + 2 * n**2 * o**2 * v + 2 * n * o**2 * v**2
+    F_G = F_T + 4 * n**4 * o + 4 * n**3 * o * v + 8 * n**2 * o**2 * v + 8 * n * o**2 * v**2
+    return F_X, F_G
+
+
 def mp2_bench() -> dict:
     """Transform, RMP2 energy and MP2 adjoint at n=256, o=40, v=216 on dense AO integrals
     (34.4 GB FP64 resident in HBM), FP64 and FP32-mixed; device times from HIP events."""
@@ -268,10 +289,7 @@ def mp2_bench() -> dict:
     t0 = time.perf_counter()
     ensure_ao(ctx, g)
     t_up = time.perf_counter() - t0
-    # (ia|jb) as the transform executes it: mu, then lam (both occupied, lam batched over sig), then
-    # sig and nu; the adjoint's forward contracts sig second (it keeps (i nu|lam b)): F_T there
-    F_X = 2 * n**4 * o + 2 * n**3 * o**2 + 2 * n**2 * o**2 * v + 2 * n * o**2 * v**2
-    F_T = 2 * n**4 * o + 2 * n**3 * o * v + 2 * n**2 * o**2 * v + 2 * n * o**2 * v**2
+    F_X, F_G = mp2_flop_counts(n, o, v)
     out = {"workload": f"RMP2 + adjoint, n={n} o={o} v={v}, dense AO integrals resident in HBM "
                        f"({n**4 * 8 / 1e9:.1f} GB FP64)", "upload_s": round(t_up, 2), "stages": {}}
 
@@ -296,11 +314,6 @@ def mp2_bench() -> dict:
     timed("transform_iajb_fp32", lambda: Transformation.tei_transform(g, Co, Cv, Co, Cv, prec="fp32"), F_X, 157.3)
     E64 = timed("rmp2_fp64", lambda: MollerPlesset.do_rmp2(w), F_X, 78.6)
     E32 = timed("rmp2_fp32_mixed", lambda: MollerPlesset.do_rmp2(w, prec="fp32_mixed"), F_X, 157.3)
-    # adjoint as executed: forward (ia|jb) with its (i nu|lam b) and (i nu|j b) intermediates kept (F_T);
-    # the half-transform (mu a|j sig) (2n^4 o + 2n^3 o v); the four dE/dC positions: three products of
-    # 2n^2 o^2 v (positions 1 and 3, the occupied quarter of position 4) and four of 2n o^2 v^2; the
-    # back-transform to dE/dg, i contracted last (2n^4 o + 2n^3 o v + 2n^2 o^2 v)
-    F_G = F_T + 4 * n**4 * o + 4 * n**3 * o * v + 8 * n**2 * o**2 * v + 8 * n * o**2 * v**2
     res = timed("rmp2_grad_fp64_with_dEdg", lambda: MollerPlesset.rmp2_grad(w), F_G, 78.6)
     res32 = timed("rmp2_grad_fp32_mixed_with_dEdg", lambda: MollerPlesset.rmp2_grad(w, prec="fp32_mixed"), F_G, 157.3)
     # FP32-mixed adjoint against FP64: max-abs differences relative to the FP64 maxima (dE/dg in chunks
