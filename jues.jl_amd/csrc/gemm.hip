@@ -113,6 +113,8 @@ struct GemmSide {
   const T* A1;
   const T* B1;
   int K1;
+  // FP32 products that write an FP64 result (the OUT64 instantiation; C is not used there)
+  double* C64 = nullptr;
 };
 
 // Shared-memory layout of one operand slice.  When the operand's contiguous
@@ -318,8 +320,9 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 __device__ inline int g32_chunk(int q, int r) { return q ^ ((r >> 2) & 3); }
 
 // Workgroups [0, nt0) run product p0 (nt0 padded up to a multiple of NXCD when a second
-// product follows), the rest p1.
-template <int KIND, bool TB, bool TA = false, bool SEG = false>
+// product follows), the rest p1.  OUT64: C = alpha A op(B) is written to the FP64 array C64
+// (beta = 0).
+template <int KIND, bool TB, bool TA = false, bool SEG = false, bool OUT64 = false>
 __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, GemmSide<float> p1, int nt0,
                                                            float alpha, float beta) {
   constexpr int BM = 128, BN = 128, BK = 16;
@@ -333,7 +336,7 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   const long lda = g.lda, ldb = g.ldb, ldc = g.ldc;
   const float* __restrict__ A = g.A + (long)blockIdx.z * g.sA;
   const float* __restrict__ B = g.B + (long)blockIdx.z * g.sB;
-  float* __restrict__ C = g.C + (long)blockIdx.z * g.sC;
+  float* __restrict__ C = OUT64 ? nullptr : g.C + (long)blockIdx.z * g.sC;
   // a second contraction segment (A1, B1 over K1 columns) only in the SEG instantiations
   const int K1 = SEG ? g.K1 : 0;
   const float* A1 = K1 > 0 ? g.A1 + (long)blockIdx.z * g.sA : A;
@@ -517,6 +520,23 @@ __global__ __launch_bounds__(256, 4) void gemm32_nn_kernel(GemmSide<float> p0, G
   }
   compute((nk - 1) & 1);
 
+  if constexpr (OUT64) {
+    double* __restrict__ C64 = g.C64 + (long)blockIdx.z * g.sC;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + wm * 64 + i * 16 + (lane & 15);
+      if (m >= M) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn * 64 + j * 16 + 4 * (lane >> 4) + r;
+          if (n < N) C64[(long)m + (long)n * ldc] = (double)(alpha * acc[i][j][r]);
+        }
+      }
+    }
+    return;
+  }
   if (beta != 0.f && m0 + BM <= M && n0 + BN <= N) {
     // interior tile accumulating into C: per 16-row block, all of the lane's C loads before
     // its stores (an element-wise load -> store round trip would serialise the epilogue)
@@ -1668,6 +1688,21 @@ template <typename T>
 void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, const T* A, long lda, long sA,
           const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch, const char* tag) {
   gemm_impl<T>(ctx, ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, tag, true);
+}
+
+bool gemm32_out64(jues_ctx* ctx, long M, long N, long K, const float* A, long lda, const float* B, long ldb,
+                  double* C, long ldc, const char* tag) {
+  if (M <= 0 || N <= 0 || K <= 0 || !occ_kernel_ok<float>(false, false, K, lda, ldb)) return false;
+  JUES_REQUIRE(lda >= M && ldb >= K && ldc >= M, "gemm32_out64: leading dimension too small");
+  GemmSide<float> p = make_side<float>(M, N, K, A, lda, 0, B, ldb, 0, nullptr, ldc, 0);
+  p.C64 = C;
+  const long tiles = (long)p.tiles_m * p.tiles_n;
+  JUES_REQUIRE(tiles < (1L << 31), "gemm: too many tiles");
+  KTimer timer(ctx, tag ? tag : "gemm_f32", 2.0 * M * N * K, 4.0 * ((double)M * K + (double)K * N) + 8.0 * M * N);
+  hipLaunchKernelGGL((gemm32_nn_kernel<0, false, false, false, true>), dim3((unsigned)tiles, 1, 1), dim3(NT), 0,
+                     ctx->stream, p, p, (int)tiles, 1.0f, 0.0f);
+  JUES_HIP(hipGetLastError());
+  return true;
 }
 
 template <typename T>

@@ -117,6 +117,11 @@ void gemm(jues_ctx* ctx, bool ta, bool tb, long M, long N, long K, T alpha, cons
           long sA, const T* B, long ldb, long sB, T beta, T* C, long ldc, long sC, int batch = 1,
           const char* tag = nullptr);
 
+// C = A B in FP32 MFMA work with the FP64 result written directly (NN, beta = 0, one product), on
+// the high-occupancy FP32 kernel; returns false, launching nothing, where that kernel does not apply.
+bool gemm32_out64(jues_ctx* ctx, long M, long N, long K, const float* A, long lda, const float* B, long ldb,
+                  double* C, long ldc, const char* tag = nullptr);
+
 // C = alpha sum_z op(A_z) op(B_z) + beta C over a batch (strides sA, sB): the batch's products are
 // written to work (batch * M * N elements) and added in a fixed order, so the result is the same
 // on every call.

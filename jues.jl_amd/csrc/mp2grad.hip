@@ -305,10 +305,10 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
     if constexpr (sizeof(T) == sizeof(double)) {
       gemm<double>(ctx, false, false, n, n3, o, 1.0, Co, n, 0, Q.as<double>(), o, 0, 0.0, dg, n, 0, 1,
                    "mp2grad_back_dg");
-    } else {
-      // the last product writes the n^4 result; with K = o it is bound by that write, so it runs in FP64 on
-      // FP64 copies of Q and Co and writes dE/dg once, in the output type (an FP32 product and a
-      // conversion would write and re-read an FP32 copy of the n^4 tensor)
+    } else if (!gemm32_out64(ctx, n, n3, o, Co, n, Q.as<float>(), o, dg, n, "mp2grad_back_dg")) {
+      // the last product writes the n^4 result once, in FP64: FP32 MFMA work with an FP64 epilogue
+      // (above), else FP64 on FP64 copies of Q and Co (an FP32 result and a conversion would write
+      // and re-read an FP32 copy of the n^4 tensor)
       DevBuf Q64(on * n2 * sizeof(double)), Co64((long)n * o * sizeof(double));
       convert<float, double>(ctx, Q.as<float>(), Q64.as<double>(), on * n2);
       convert<float, double>(ctx, Co, Co64.as<double>(), (long)n * o);
