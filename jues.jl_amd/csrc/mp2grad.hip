@@ -226,9 +226,9 @@ void mp2_grad_dev(jues_ctx* ctx, const T* Co, const T* Cv, int o, int v, const d
   T* X2 = X2b.as<T>();
   T* Y3 = Y3b.as<T>();
   {
-    // forward (ia|jb), the same products as the (ia|jb) transform: X2[i,nu,lam,b] = (i nu|lam b)
-    // (kept for position 3), then lam and nu contracted in place, batched, to X3[i,nu,j,b] =
-    // (i nu|j b) and K[i,a,j,b]
+    // forward (ia|jb) with sig contracted second (the (ia|jb) transform contracts lam second, which
+    // costs less, but position 3 needs X2[i,nu,lam,b] = (i nu|lam b)), then lam and nu contracted in
+    // place, batched, to X3[i,nu,j,b] = (i nu|j b) and K[i,a,j,b]
     {
       DevBuf Z1(o * n3 * sizeof(T));
       gemm<T>(ctx, true, false, o, n3, n, one, Co, n, 0, g, n, 0, zero, Z1.as<T>(), o, 0, 1, "xform_q1");
